@@ -1,0 +1,162 @@
+/*
+ * oracle.h -- FP64 CPU restatement of monofonIC's nLPT field pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links or calls
+ * this code; only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg load it, and only as the checker.
+ *
+ * Every function restates a reference routine literally (same loop order,
+ * same index maps, same scalar factors) and cites the reference file:line it
+ * follows.  Arrays use the reference's in-place r2c layout
+ * (src/grid_fft.cc:43-112):
+ *   real space : relem(i,j,k) = data[(i*n + j)*(n+2) + k]
+ *   k space    : kelem(i,j,k) = cdata[(i*n + j)*(n/2+1) + k]   (complex)
+ * for cubic grids of side n.  Complex values are interleaved (re,im) doubles.
+ */
+#ifndef MFX_ORACLE_H
+#define MFX_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------- 1D / 3D FFT (unitary, FFTW sign conventions) ---------------- */
+/* out-of-place complex DFT of length n, sign -1 forward, +1 backward, unnormalised */
+void orc_dft1d(const double *in, double *out, int n, int sign);
+/* in-place 3D r2c on the padded real layout, then multiply by 1/sqrt(n^3)
+ * (grid_fft.cc:209-238 fftw_execute + ApplyNorm) */
+void orc_fft_r2c(double *data, int n);
+/* in-place 3D c2r, Hermitian semantics of FFTW c2r (imag parts of the kz=0 and
+ * kz=n/2 coefficients ignored), then multiply by 1/sqrt(n^3) (grid_fft.cc:242-270) */
+void orc_fft_c2r(double *data, int n);
+
+/* ---------- k-space helpers (include/grid_fft.hh:282-390) --------------- */
+/* get_k component: (i - [i>n/2] n) * kfac */
+double orc_get_k(size_t i, size_t n, double kfac);
+/* gradient component value (imaginary part): 0 at i == n/2 */
+double orc_grad_k(size_t i, size_t n, double kfac);
+
+/* ---------- multipliers used by BaseConvolver (convolution.hh:74-292) ---- */
+/* A multiplier is a sum of up to two terms  coef * prod_d f_d(i_d)^p_d, where
+ * f_d is get_k (nyq=0) or the Nyquist-zeroed gradient (nyq=1). */
+typedef struct {
+  int nterms;
+  double coef_re[2], coef_im[2];
+  int pow[2][3];
+  int nyq[2];
+} orc_mult_t;
+
+enum { ORC_OP_ASSIGN = 0, ORC_OP_ADD = 1, ORC_OP_SUB = 2, ORC_OP_MULADD = 3 };
+
+/* OrszagConvolver::convolve2 (convolution.hh:512-535): pad_insert both inputs
+ * (569-592), c2r at 3n/2, pointwise product, r2c, unpad + op (617-653). */
+void orc_convolve2(const double *fa, const orc_mult_t *ma, const double *fb, const orc_mult_t *mb,
+                   double *out, int op, double opfac, int n, double boxlen);
+/* OrszagConvolver::convolve3 (convolution.hh:542-549) */
+void orc_convolve3(const double *fa, const orc_mult_t *ma, const double *fb, const orc_mult_t *mb,
+                   const double *fc, const orc_mult_t *mc, double *out, int op, double opfac, int n,
+                   double boxlen);
+/* convenience: Hessian multiplier built from gradient() (convolution.hh:136-160) */
+void orc_mult_hessian(orc_mult_t *m, int a, int b);
+void orc_mult_sum_hessians(orc_mult_t *m, int a, int b, int c, int d);
+void orc_mult_diff_hessians(orc_mult_t *m, int a, int b, int c, int d);
+void orc_mult_gradient(orc_mult_t *m, int a);          /* gradient(): +i k, Nyquist zeroed */
+void orc_mult_getk_gradient(orc_mult_t *m, int a);     /* (0,-k) from get_k, no zeroing */
+void orc_mult_getk_hessian(orc_mult_t *m, int a, int b); /* -k_a k_b from get_k */
+void orc_mult_identity(orc_mult_t *m);
+
+/* ---------- k-space operators ------------------------------------------- */
+/* ic_generator.cc:341-346 */
+void orc_noise_normalise(double *wn, int n, double volfac, int fixing, int inversion);
+/* ic_generator.cc:450-455: phi = -wn*A(|k|)/|k|^2, then zero_DC_mode.  The
+ * amplitude is looked up in amp_table[n2] with n2 = ix^2+iy^2+iz^2 (signed
+ * wave numbers), evaluated by the caller at k = kfac*sqrt(n2). */
+void orc_phi1(double *phi, const double *wn, int n, double boxlen, const double *amp_table);
+/* grid_fft.hh:440-449 apply_InverseLaplacian + zero_DC_mode */
+void orc_inverse_laplacian(double *f, int n, double boxlen);
+/* operator*= in k space */
+void orc_scale(double *f, int n, double s);
+
+/* ---------- LPT sequence (ic_generator.cc:464-563) ---------------------- */
+/* Given phi (k space, before scaling by g1) computes phi2, phi3, A3[0..2]
+ * (k space, inverse Laplacian applied, NOT yet scaled by growth factors).
+ * order = 2 or 3.  phi3/A3 may be NULL when order == 2. */
+void orc_lpt_terms(const double *phi, double *phi2, double *phi3, double *A3x, double *A3y,
+                   double *A3z, int n, double boxlen, int order);
+
+/* ---------- assembly (ic_generator.cc:821-944, particle_generator.hh:499-638) */
+/* Builds the k-space field for displacement (kind 0) or velocity (kind 1)
+ * component idim from the already growth-scaled potentials, applies unit
+ * factor, zero_DC, c2r; writes the real field into out (padded real layout). */
+void orc_assemble_component(const double *phi, const double *phi2, const double *phi3,
+                            const double *A3x, const double *A3y, const double *A3z,
+                            double *out, int n, double boxlen, int order, int kind, int idim,
+                            double vfac, double unit);
+/* pos[p] = q_d*lunit + field (sc lattice, particle order (i,j,k)) */
+void orc_set_positions(const double *field, double *pos, int n, int idim, double lunit);
+
+/* ---------- reductions / diagnostics (grid_fft.cc:1133-1210) ------------- */
+int orc_power_spectrum(const double *f, int n, double boxlen, double *bin_k, double *bin_P,
+                       double *bin_eP, double *bin_count, int maxbins);
+
+/* ---------- random numbers ---------------------------------------------- */
+/* MT19937 (Matsumoto & Nishimura 1998) with GSL seeding (0 -> 4357) */
+typedef struct { uint32_t mt[624]; int mti; } orc_mt_t;
+void orc_mt_seed(orc_mt_t *s, unsigned long seed);
+uint32_t orc_mt_next(orc_mt_t *s);
+/* Leva (1992) ratio-of-uniforms normal deviate, as used through
+ * gsl_ran_ugaussian_ratio_method (random_music_wnoise_generator.cc:613) */
+double orc_gauss_ratio(orc_mt_t *s);
+/* MUSIC1 white noise fill (random_music_wnoise_generator.cc:35-66,588-627,747-796;
+ * random_music.cc:92-116) into the real-space padded layout, FP64 */
+void orc_music1_fill(double *data, int n, long baseseed, int cubesize);
+/* same, but with the cube values rounded to float before the mean removal
+ * (the FLOAT build stores real_t=float) */
+void orc_music1_fill_f32(float *data, int n, long baseseed, int cubesize);
+
+/* Threefry-4x64-20 (Salmon et al. 2011; threefry.h) */
+void orc_threefry4x64_20(const uint64_t ctr[4], const uint64_t key[4], uint64_t out[4]);
+/* Panphasia-HO uniform stream for one cell, first `count` values
+ * (uniform_rand_threefry4x64.c:254-493) */
+void orc_panphasia_uniforms(size_t l, size_t j1, size_t j2, size_t j3, size_t seed, double *u,
+                            int count);
+/* Box-Muller in double (high_order_panphasia_routines.c:196-237) */
+void orc_box_muller(const double *u, double *g, int count);
+/* cell white noise: first Box-Muller deviate of the cell's stream, level l = log2 n */
+void orc_threefry_fill(double *data, int n, size_t seed);
+
+/* NGenIC (random_ngenic.cc): RANLUX ranlxd1 + column fill */
+typedef struct { double xdbl[12]; double carry; unsigned int ir, jr, is, is_old, pr; } orc_ranlxd_t;
+void orc_ranlxd_seed(orc_ranlxd_t *s, unsigned long seed, int luxury);
+double orc_ranlxd_uniform(orc_ranlxd_t *s);
+void orc_ngenic_seedtable(unsigned int *table, int n, long seed);
+void orc_ngenic_fill(double *cdata, int n, const unsigned int *table);
+
+/* ---------- cosmology (include/cosmology_calculator.hh) ------------------ */
+typedef struct {
+  double h, Omega_m, Omega_b, Omega_DE, Omega_r, Omega_k, w_0, w_a, n_s, sigma_8, Tcmb;
+  /* derived */
+  double pnorm, sqrtpnorm, Dnow;
+  /* E&H */
+  double eh[24];
+  /* growth table (log a, log D, log f) + natural cubic spline second derivatives */
+  int ntab;
+  double *la, *lD, *lf, *d2D, *d2f;
+} orc_cosmo_t;
+void orc_cosmo_init(orc_cosmo_t *c, double H0, double Omega_m, double Omega_b, double n_s,
+                    double sigma_8, double Tcmb, int zero_radiation);
+void orc_cosmo_free(orc_cosmo_t *c);
+double orc_cosmo_transfer(const orc_cosmo_t *c, double k);
+double orc_cosmo_Dplus(const orc_cosmo_t *c, double a);
+double orc_cosmo_f(const orc_cosmo_t *c, double a);
+double orc_cosmo_vfact(const orc_cosmo_t *c, double a);
+double orc_cosmo_H(const orc_cosmo_t *c, double a);
+double orc_cosmo_amplitude(const orc_cosmo_t *c, double k);
+double orc_cosmo_sigma8_unnorm(const orc_cosmo_t *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

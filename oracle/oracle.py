@@ -1,0 +1,310 @@
+"""ctypes bindings for the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` as the checker.  The shipped package never
+imports this module.
+
+Grids are numpy float64 arrays in the reference's in-place r2c layout
+(src/grid_fft.cc:43-112): real space shape (n, n, n+2), k space viewed as
+complex128 of shape (n, n, n//2+1).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ORC_OP_ASSIGN, ORC_OP_ADD, ORC_OP_SUB, ORC_OP_MULADD = 0, 1, 2, 3
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.orc_threefry4x64_20.argtypes = [ctypes.POINTER(ctypes.c_uint64)] * 3
+        L.orc_mt_next.restype = ctypes.c_uint32
+        L.orc_gauss_ratio.restype = ctypes.c_double
+        L.orc_ranlxd_uniform.restype = ctypes.c_double
+        L.orc_get_k.restype = ctypes.c_double
+        L.orc_grad_k.restype = ctypes.c_double
+        for f in ("orc_cosmo_transfer", "orc_cosmo_Dplus", "orc_cosmo_f", "orc_cosmo_vfact",
+                  "orc_cosmo_H", "orc_cosmo_amplitude"):
+            getattr(L, f).restype = ctypes.c_double
+            getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_cosmo_sigma8_unnorm.restype = ctypes.c_double
+        L.orc_cosmo_sigma8_unnorm.argtypes = [ctypes.c_void_p]
+        L.orc_cosmo_init.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 6 + [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def real_shape(n):
+    return (n, n, n + 2)
+
+
+def kview(a):
+    """complex view (n, n, n//2+1) of a padded real-layout float64 array"""
+    n = a.shape[0]
+    return a.view(np.complex128).reshape(n, n, n // 2 + 1)
+
+
+# ---------------------------------------------------------------- FFT
+def fft_r2c(a):
+    n = a.shape[0]
+    lib().orc_fft_r2c(_p(a), ctypes.c_int(n))
+    return a
+
+
+def fft_c2r(a):
+    n = a.shape[0]
+    lib().orc_fft_c2r(_p(a), ctypes.c_int(n))
+    return a
+
+
+def dft1d(x, sign):
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.empty_like(x)
+    lib().orc_dft1d(_p(x), _p(out), ctypes.c_int(x.size), ctypes.c_int(sign))
+    return out
+
+
+# ---------------------------------------------------------------- multipliers
+class Mult(ctypes.Structure):
+    _fields_ = [("nterms", ctypes.c_int), ("coef_re", ctypes.c_double * 2),
+                ("coef_im", ctypes.c_double * 2), ("pow", (ctypes.c_int * 3) * 2),
+                ("nyq", ctypes.c_int * 2)]
+
+
+def mult(kind, *dims):
+    m = Mult()
+    f = {"hessian": "orc_mult_hessian", "sum": "orc_mult_sum_hessians",
+         "diff": "orc_mult_diff_hessians", "grad": "orc_mult_gradient",
+         "getk_grad": "orc_mult_getk_gradient", "getk_hess": "orc_mult_getk_hessian",
+         "identity": "orc_mult_identity"}[kind]
+    getattr(lib(), f)(ctypes.byref(m), *[ctypes.c_int(d) for d in dims])
+    return m
+
+
+def convolve2(fa, ma, fb, mb, out, op, opfac, boxlen):
+    n = fa.shape[0]
+    lib().orc_convolve2(_p(fa), ctypes.byref(ma), _p(fb), ctypes.byref(mb), _p(out),
+                        ctypes.c_int(op), ctypes.c_double(opfac), ctypes.c_int(n),
+                        ctypes.c_double(boxlen))
+
+
+def convolve3(fa, ma, fb, mb, fc, mc, out, op, opfac, boxlen):
+    n = fa.shape[0]
+    lib().orc_convolve3(_p(fa), ctypes.byref(ma), _p(fb), ctypes.byref(mb), _p(fc),
+                        ctypes.byref(mc), _p(out), ctypes.c_int(op), ctypes.c_double(opfac),
+                        ctypes.c_int(n), ctypes.c_double(boxlen))
+
+
+# ---------------------------------------------------------------- operators
+def noise_normalise(wn, volfac, fixing=False, inversion=False):
+    lib().orc_noise_normalise(_p(wn), ctypes.c_int(wn.shape[0]), ctypes.c_double(volfac),
+                              ctypes.c_int(int(fixing)), ctypes.c_int(int(inversion)))
+
+
+def phi1(phi, wn, boxlen, amp_table):
+    amp_table = np.ascontiguousarray(amp_table, dtype=np.float64)
+    lib().orc_phi1(_p(phi), _p(wn), ctypes.c_int(phi.shape[0]), ctypes.c_double(boxlen), _p(amp_table))
+
+
+def inverse_laplacian(f, boxlen):
+    lib().orc_inverse_laplacian(_p(f), ctypes.c_int(f.shape[0]), ctypes.c_double(boxlen))
+
+
+def scale(f, s):
+    lib().orc_scale(_p(f), ctypes.c_int(f.shape[0]), ctypes.c_double(s))
+
+
+def lpt_terms(phi, boxlen, order):
+    n = phi.shape[0]
+    z = lambda: np.zeros(real_shape(n))
+    phi2 = z()
+    phi3, ax, ay, az = (z(), z(), z(), z()) if order > 2 else (None, None, None, None)
+    nul = ctypes.c_void_p(0)
+    lib().orc_lpt_terms(_p(phi), _p(phi2), _p(phi3) if order > 2 else nul,
+                        _p(ax) if order > 2 else nul, _p(ay) if order > 2 else nul,
+                        _p(az) if order > 2 else nul, ctypes.c_int(n), ctypes.c_double(boxlen),
+                        ctypes.c_int(order))
+    return phi2, phi3, (ax, ay, az)
+
+
+def assemble_component(phi, phi2, phi3, A3, boxlen, order, kind, idim, vfac, unit=1.0):
+    n = phi.shape[0]
+    out = np.zeros(real_shape(n))
+    nul = ctypes.c_void_p(0)
+    g = lambda a: _p(a) if a is not None else nul
+    lib().orc_assemble_component(g(phi), g(phi2), g(phi3), g(A3[0]), g(A3[1]), g(A3[2]), _p(out),
+                                 ctypes.c_int(n), ctypes.c_double(boxlen), ctypes.c_int(order),
+                                 ctypes.c_int(kind), ctypes.c_int(idim), ctypes.c_double(vfac),
+                                 ctypes.c_double(unit))
+    return out
+
+
+def set_positions(field, idim, lunit=1.0):
+    n = field.shape[0]
+    pos = np.zeros(n * n * n)
+    lib().orc_set_positions(_p(field), _p(pos), ctypes.c_int(n), ctypes.c_int(idim),
+                            ctypes.c_double(lunit))
+    return pos
+
+
+def power_spectrum(f, boxlen):
+    n = f.shape[0]
+    mb = n
+    k, P, eP, c = (np.zeros(mb) for _ in range(4))
+    nb = lib().orc_power_spectrum(_p(f), ctypes.c_int(n), ctypes.c_double(boxlen), _p(k), _p(P),
+                                  _p(eP), _p(c), ctypes.c_int(mb))
+    return k[:nb], P[:nb], eP[:nb], c[:nb]
+
+
+# ---------------------------------------------------------------- RNG
+class MT(ctypes.Structure):
+    _fields_ = [("mt", ctypes.c_uint32 * 624), ("mti", ctypes.c_int)]
+
+
+def mt19937(seed, count):
+    s = MT()
+    lib().orc_mt_seed(ctypes.byref(s), ctypes.c_ulong(seed))
+    return np.array([lib().orc_mt_next(ctypes.byref(s)) for _ in range(count)], dtype=np.uint64)
+
+
+def gauss_ratio(seed, count):
+    s = MT()
+    lib().orc_mt_seed(ctypes.byref(s), ctypes.c_ulong(seed))
+    return np.array([lib().orc_gauss_ratio(ctypes.byref(s)) for _ in range(count)])
+
+
+def music1_fill(n, seed, cubesize=32, f32=False):
+    if f32:
+        a = np.zeros(real_shape(n), dtype=np.float32)
+        lib().orc_music1_fill_f32(_p(a), ctypes.c_int(n), ctypes.c_long(seed), ctypes.c_int(cubesize))
+    else:
+        a = np.zeros(real_shape(n))
+        lib().orc_music1_fill(_p(a), ctypes.c_int(n), ctypes.c_long(seed), ctypes.c_int(cubesize))
+    return a
+
+
+def threefry4x64_20(ctr, key):
+    c = (ctypes.c_uint64 * 4)(*ctr)
+    k = (ctypes.c_uint64 * 4)(*key)
+    o = (ctypes.c_uint64 * 4)()
+    lib().orc_threefry4x64_20(c, k, o)
+    return [int(v) for v in o]
+
+
+def panphasia_uniforms(l, j1, j2, j3, seed, count):
+    u = np.zeros(count)
+    lib().orc_panphasia_uniforms(ctypes.c_size_t(l), ctypes.c_size_t(j1), ctypes.c_size_t(j2),
+                                 ctypes.c_size_t(j3), ctypes.c_size_t(seed), _p(u), ctypes.c_int(count))
+    return u
+
+
+def threefry_fill(n, seed):
+    a = np.zeros(real_shape(n))
+    lib().orc_threefry_fill(_p(a), ctypes.c_int(n), ctypes.c_size_t(seed))
+    return a
+
+
+class Ranlxd(ctypes.Structure):
+    _fields_ = [("xdbl", ctypes.c_double * 12), ("carry", ctypes.c_double), ("ir", ctypes.c_uint),
+                ("jr", ctypes.c_uint), ("is_", ctypes.c_uint), ("is_old", ctypes.c_uint),
+                ("pr", ctypes.c_uint)]
+
+
+def ranlxd1(seed, count):
+    s = Ranlxd()
+    lib().orc_ranlxd_seed(ctypes.byref(s), ctypes.c_ulong(seed), ctypes.c_int(202))
+    return np.array([lib().orc_ranlxd_uniform(ctypes.byref(s)) for _ in range(count)])
+
+
+def ngenic_seedtable(n, seed):
+    t = np.zeros(n * n, dtype=np.uint32)
+    lib().orc_ngenic_seedtable(_p(t), ctypes.c_int(n), ctypes.c_long(seed))
+    return t
+
+
+def ngenic_fill(n, table):
+    a = np.zeros(real_shape(n))
+    table = np.ascontiguousarray(table, dtype=np.uint32)
+    lib().orc_ngenic_fill(_p(a), ctypes.c_int(n), _p(table))
+    return a
+
+
+# ---------------------------------------------------------------- cosmology
+class Cosmo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in
+                ("h", "Omega_m", "Omega_b", "Omega_DE", "Omega_r", "Omega_k", "w_0", "w_a", "n_s",
+                 "sigma_8", "Tcmb", "pnorm", "sqrtpnorm", "Dnow")] + \
+               [("eh", ctypes.c_double * 24), ("ntab", ctypes.c_int)] + \
+               [(k, ctypes.c_void_p) for k in ("la", "lD", "lf", "d2D", "d2f")]
+
+
+class Cosmology:
+    """FP64 cosmology calculator restated from include/cosmology_calculator.hh"""
+
+    def __init__(self, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96, sigma_8=0.81, Tcmb=2.7255,
+                 zero_radiation=False):
+        self.c = Cosmo()
+        lib().orc_cosmo_init(ctypes.byref(self.c), H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb,
+                             int(zero_radiation))
+
+    def __del__(self):
+        try:
+            lib().orc_cosmo_free(ctypes.byref(self.c))
+        except Exception:
+            pass
+
+    def _f(self, name, x):
+        return getattr(lib(), name)(ctypes.byref(self.c), ctypes.c_double(x))
+
+    def transfer(self, k):
+        return self._f("orc_cosmo_transfer", k)
+
+    def Dplus(self, a):
+        return self._f("orc_cosmo_Dplus", a)
+
+    def f(self, a):
+        return self._f("orc_cosmo_f", a)
+
+    def vfact(self, a):
+        return self._f("orc_cosmo_vfact", a)
+
+    def H(self, a):
+        return self._f("orc_cosmo_H", a)
+
+    def amplitude(self, k):
+        return self._f("orc_cosmo_amplitude", k)
+
+    @property
+    def pnorm(self):
+        return self.c.pnorm
+
+    def amplitude_table(self, n, boxlen):
+        """A(k) at k = kfac*sqrt(n2) for every integer n2 <= 3 (n/2)^2"""
+        kfac = 2.0 * np.pi / boxlen
+        nmax = 3 * (n // 2) ** 2 + 1
+        t = np.zeros(nmax)
+        for n2 in range(1, nmax):
+            t[n2] = self.amplitude(kfac * np.sqrt(n2))
+        return t
