@@ -1,0 +1,237 @@
+/*
+ * mfx.h -- C ABI of the MI355X (gfx950) nLPT field pipeline.
+ *
+ * Plain C, plain pointers and sizes.  Each entry point names the monofonIC
+ * interface it replaces (file:line in the monofonIC source tree).  Every call
+ * returns 0 on success and a nonzero status otherwise; mfx_last_error()
+ * returns the message of the last failure on the calling thread.
+ *
+ * Device memory of a grid belongs to its mfx_grid handle (this replaces the
+ * fftw_malloc ownership of Grid_FFT, src/grid_fft.cc:50,97).  All work of a
+ * context is issued in order on one HIP stream; calls return after enqueueing
+ * unless documented otherwise (downloads, reductions and mfx_ctx_synchronize
+ * block).  Grids are cubic or rectangular with even last dimension; the memory
+ * layout is the reference's in-place r2c layout:
+ *   real space relem(i,j,k) = data[(i*n1 + j)*(n2+2) + k]
+ *   k space    kelem(i,j,k) = cdata[(i*n1 + j)*(n2/2+1) + k]
+ */
+#ifndef MFX_H
+#define MFX_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFX_OK 0
+#define MFX_EINVAL 1 /* invalid argument or unsupported size */
+#define MFX_EHIP 2   /* HIP runtime failure */
+#define MFX_ESTATE 3 /* call not valid in the grid's current space */
+
+#define MFX_FLOAT 4
+#define MFX_DOUBLE 8
+
+/* space_t (include/grid_fft.hh:29) */
+#define MFX_KSPACE 0
+#define MFX_RSPACE 1
+
+/* output operators op::assign_to/add_to/subtract_from/multiply_add_to
+ * (include/operators.hh:25-35) */
+#define MFX_OP_ASSIGN 0
+#define MFX_OP_ADD 1
+#define MFX_OP_SUB 2
+#define MFX_OP_MULADD 3
+
+typedef struct mfx_ctx mfx_ctx;
+typedef struct mfx_grid mfx_grid;
+typedef struct mfx_conv mfx_conv;
+typedef struct mfx_cosmo mfx_cosmo;
+
+/* ------------------------------------------------------------ context */
+const char *mfx_last_error(void);
+int mfx_version(void);
+/* one HIP stream on `device`; replaces the FFTW/OpenMP set-up of src/main.cc:156-182 */
+int mfx_ctx_create(int device, mfx_ctx **out);
+int mfx_ctx_destroy(mfx_ctx *c);
+int mfx_ctx_synchronize(mfx_ctx *c);
+/* stream-ordered timers: record event `slot` (0..15), elapsed ms between two
+ * recorded slots (blocks until the later one completed) */
+int mfx_event_record(mfx_ctx *c, int slot);
+int mfx_event_elapsed(mfx_ctx *c, int slot0, int slot1, float *ms);
+/* per-kernel profiling with HIP events around every launch: name, launches,
+ * total ms, total algorithmic bytes */
+int mfx_profile_enable(mfx_ctx *c, int on);
+int mfx_profile_reset(mfx_ctx *c);
+int mfx_profile_count(mfx_ctx *c, int *count);
+int mfx_profile_get(mfx_ctx *c, int idx, char *name, size_t namelen, int *launches, double *ms, double *bytes);
+/* device memory owned by the context (bytes), e.g. FP64 particle arrays */
+int mfx_device_alloc(mfx_ctx *c, size_t bytes, void **ptr);
+int mfx_device_free(mfx_ctx *c, void *ptr);
+int mfx_memcpy_d2h(mfx_ctx *c, void *host, const void *dev, size_t bytes);
+int mfx_memcpy_h2d(mfx_ctx *c, void *dev, const void *host, size_t bytes);
+
+/* ------------------------------------------------------------ Grid_FFT
+ * Grid_FFT<real_t,false> ctor/allocate/reset (include/grid_fft.hh:77-101,
+ * src/grid_fft.cc:43-112).  precision MFX_FLOAT or MFX_DOUBLE. */
+int mfx_grid_create(mfx_ctx *c, size_t n0, size_t n1, size_t n2, double L0, double L1, double L2, int precision,
+                    int space, mfx_grid **out);
+int mfx_grid_destroy(mfx_grid *g);
+/* ntot_ (number of reals), space_, precision */
+int mfx_grid_info(const mfx_grid *g, size_t *ntot, int *space, int *precision);
+int mfx_grid_data(mfx_grid *g, void **dev_ptr);
+/* whole ntot_ buffer; nbytes must equal ntot*precision */
+int mfx_grid_upload(mfx_grid *g, const void *host, size_t nbytes);
+int mfx_grid_download(const mfx_grid *g, void *host, size_t nbytes);
+/* Grid_FFT::zero (grid_fft.hh:150-156) */
+int mfx_grid_zero(mfx_grid *g);
+/* Grid_FFT::copy_from (grid_fft.hh:158-178) */
+int mfx_grid_copy_from(mfx_grid *dst, const mfx_grid *src);
+/* FourierTransformForward/Backward (src/grid_fft.cc:209-270): unitary
+ * 1/sqrt(N) normalisation, no-op when already in the target space;
+ * do_transform = 0 only flips the space flag */
+int mfx_grid_fft_forward(mfx_grid *g, int do_transform);
+int mfx_grid_fft_backward(mfx_grid *g, int do_transform);
+/* operator*= (grid_fft.hh:392-403), applied in the current space */
+int mfx_grid_scale(mfx_grid *g, double s);
+/* apply_InverseLaplacian (grid_fft.hh:440-449): forward FFT if needed,
+ * -x/|k|^2, zero DC */
+int mfx_grid_inverse_laplacian(mfx_grid *g);
+/* zero_DC_mode (grid_fft.hh:844-887) */
+int mfx_grid_zero_dc(mfx_grid *g);
+/* mean() and std() (grid_fft.hh:507-595), real space only */
+int mfx_grid_stats(mfx_grid *g, double *mean, double *std);
+/* Compute_PowerSpectrum (src/grid_fft.cc:1133-1210) */
+int mfx_grid_power_spectrum(mfx_grid *g, double *bin_k, double *bin_P, double *bin_eP, double *bin_count,
+                            int maxbins, int *nbins);
+
+/* ------------------------------------------------------------ k-space operators of ic_generator::run */
+/* noise normalisation, fixing and inversion (src/ic_generator.cc:341-346) */
+int mfx_noise_normalise(mfx_grid *wn, double volfac, int fixing, int inversion);
+/* phi = -wn * A(|k|)/|k|^2 and zero_DC_mode (src/ic_generator.cc:450-457);
+ * amp_table[n2] = A(kfac*sqrt(n2)) in FP64 for n2 = ix^2+iy^2+iz^2 <= 3(n/2)^2 */
+int mfx_phi1(mfx_grid *phi, const mfx_grid *wn, const double *amp_table, size_t ntable);
+
+/* ------------------------------------------------------------ OrszagConvolver
+ * OrszagConvolver<real_t>(N,L) (include/convolution.hh:463-470).  A Fourier
+ * multiplier is sum_t coef_t * prod_d f_d(i_d)^pow_t[d], with f_d the
+ * Nyquist-zeroed gradient wave number (nyq = 1, Grid_FFT::gradient) or get_k
+ * (nyq = 0). */
+typedef struct {
+  int nterms;
+  double coef_re[2], coef_im[2];
+  int pow[2][3];
+  int nyq[2];
+} mfx_mult;
+
+int mfx_conv_create(mfx_ctx *c, size_t n, double L, int precision, mfx_conv **out);
+int mfx_conv_destroy(mfx_conv *cv);
+/* generic OrszagConvolver::convolve2 (convolution.hh:512-535) with the
+ * multipliers evaluated inside pad_insert, op applied by unpad (617-653) */
+int mfx_conv2(mfx_conv *cv, mfx_grid *a, const mfx_mult *ma, mfx_grid *b, const mfx_mult *mb, mfx_grid *out,
+              int op, double opfac);
+/* OrszagConvolver::convolve3 (convolution.hh:542-549) */
+int mfx_conv3(mfx_conv *cv, mfx_grid *a, const mfx_mult *ma, mfx_grid *b, const mfx_mult *mb, mfx_grid *c,
+              const mfx_mult *mc, mfx_grid *out, int op, double opfac);
+/* BaseConvolver named variants (convolution.hh:74-292) */
+int mfx_conv_gradients(mfx_conv *cv, mfx_grid *inl, int d1l, mfx_grid *inr, int d1r, mfx_grid *out, int op,
+                       double opfac);
+int mfx_conv_gradient_and_hessian(mfx_conv *cv, mfx_grid *inl, int d1l, mfx_grid *inr, const int d2r[2],
+                                  mfx_grid *out, int op, double opfac);
+int mfx_conv_hessians(mfx_conv *cv, mfx_grid *inl, const int d2l[2], mfx_grid *inr, const int d2r[2],
+                      mfx_grid *out, int op, double opfac);
+int mfx_conv_hessians3(mfx_conv *cv, mfx_grid *inl, const int d2l[2], mfx_grid *inm, const int d2m[2],
+                       mfx_grid *inr, const int d2r[2], mfx_grid *out, int op, double opfac);
+int mfx_conv_sum_of_hessians(mfx_conv *cv, mfx_grid *inl, const int d2l[2], mfx_grid *inr, const int d2r1[2],
+                             const int d2r2[2], mfx_grid *out, int op, double opfac);
+int mfx_conv_difference_of_hessians(mfx_conv *cv, mfx_grid *inl, const int d2l[2], mfx_grid *inr,
+                                    const int d2r1[2], const int d2r2[2], mfx_grid *out, int op, double opfac);
+int mfx_conv_multiply_field(mfx_conv *cv, mfx_grid *inl, mfx_grid *inr, mfx_grid *out, int op, double opfac);
+
+/* All nLPT source terms of src/ic_generator.cc:464-547 in one call: phi2,
+ * phi3 = 3a + 3b and A3 with the inverse Laplacian applied (growth factors
+ * not applied).  Same results as the 5 (2LPT) / 33 (3LPT) convolve calls of
+ * the reference, computed from the Hessian tensors of phi and phi2 so that
+ * each padded transform is done once.  order 2: phi3/A3 may be NULL. */
+int mfx_lpt_sources(mfx_conv *cv, mfx_grid *phi, mfx_grid *phi2, mfx_grid *phi3, mfx_grid *A3x, mfx_grid *A3y,
+                    mfx_grid *A3z, int order);
+
+/* ------------------------------------------------------------ assembly
+ * k-space displacement (kind 0) or velocity (kind 1) component idim built from
+ * the growth-scaled potentials, times unit/L, zero DC
+ * (src/ic_generator.cc:821-860, 877-919); tmp is left in k space */
+int mfx_assemble_kspace(mfx_grid *tmp, mfx_grid *phi, mfx_grid *phi2, mfx_grid *phi3, mfx_grid *A3x,
+                        mfx_grid *A3y, mfx_grid *A3z, int order, int kind, int idim, double vfac, double unit);
+/* set_positions / set_velocities for the sc lattice, 64-bit reals
+ * (include/particle_generator.hh:499-543, 597-638): out_dev is a device
+ * array of n^3 doubles, particle order (i,j,k); kind 0 positions, 1 velocities */
+int mfx_set_particles(mfx_grid *field, double *out_dev, int idim, double lunit, int kind);
+
+/* ------------------------------------------------------------ RNG plugins
+ * RNG_plugin::Fill_Grid (include/random_plugin.hh:26-39) */
+/* MUSIC1 (src/plugins/random_music.cc:92-116, random_music_wnoise_generator.cc
+ * :35-66,588-627,747-796): real-space N(0,1), leaves the grid in real space */
+int mfx_rng_music1_fill(mfx_grid *g, long seed, int cubesize);
+/* NGENIC (src/plugins/random_ngenic.cc:221-345): k-space noise, leaves the
+ * grid in k space */
+int mfx_rng_ngenic_fill(mfx_grid *g, long seed);
+/* Panphasia-HO Threefry-4x64-20 cell stream (external/panphasia_ho/
+ * uniform_rand_threefry4x64.c:254-493): first Box-Muller deviate of each
+ * cell (l = log2 n, j = (i,j,k)), real space */
+int mfx_rng_threefry_fill(mfx_grid *g, unsigned long long seed);
+/* raw streams (test hooks): MT19937 words, ranlxd1 uniforms, Threefry cell
+ * uniforms (cells = 4 x ncells of (l, j1, j2, j3)); outputs are host arrays */
+int mfx_rng_mt19937_raw(mfx_ctx *c, const unsigned long *seeds, int nseeds, int count, unsigned *out);
+int mfx_rng_ranlxd1_raw(mfx_ctx *c, const unsigned *seeds, int nseeds, int count, double *out);
+int mfx_rng_threefry_uniforms(mfx_ctx *c, const unsigned long long *cells, int ncells, unsigned long long seed,
+                              int count, double *out);
+
+/* ------------------------------------------------------------ cosmology (host, FP64)
+ * cosmology::calculator (include/cosmology_calculator.hh:166-210) with the
+ * Eisenstein & Hu transfer function (src/plugins/transfer_eisenstein.cc) */
+#define MFX_COSMO_TRANSFER 0
+#define MFX_COSMO_DPLUS 1
+#define MFX_COSMO_F 2
+#define MFX_COSMO_VFACT 3
+#define MFX_COSMO_H 4
+#define MFX_COSMO_AMPLITUDE 5
+#define MFX_COSMO_PNORM 6
+int mfx_cosmo_create(double H0, double Omega_m, double Omega_b, double n_s, double sigma_8, double Tcmb,
+                     int zero_radiation, mfx_cosmo **out);
+int mfx_cosmo_destroy(mfx_cosmo *cc);
+int mfx_cosmo_query(const mfx_cosmo *cc, int what, double x, double *out);
+int mfx_cosmo_amplitude_table(const mfx_cosmo *cc, size_t n, double boxlen, double *out, size_t ntable);
+
+/* ------------------------------------------------------------ LPT driver
+ * ic_generator::run for the sc lattice and generic (field_lagrangian) or
+ * particle output (src/ic_generator.cc:90-971): noise, phi(1..3), A(3),
+ * growth scaling and FP64 position/velocity assembly, all on the device. */
+typedef struct {
+  size_t n;
+  double boxlen, zstart;
+  int lpt_order;        /* 1, 2 or 3 */
+  int precision;        /* bulk FFT/convolution precision: MFX_FLOAT or MFX_DOUBLE */
+  int generator;        /* 0 MUSIC1, 1 NGENIC, 2 Threefry */
+  long seed;
+  int fixing, inversion;
+  double H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb;
+  int zero_radiation;
+  int use_fused_lpt;    /* 1: mfx_lpt_sources, 0: the reference's convolve call sequence */
+} mfx_icgen_params;
+
+typedef struct mfx_icgen mfx_icgen;
+int mfx_icgen_create(mfx_ctx *c, const mfx_icgen_params *p, mfx_icgen **out);
+int mfx_icgen_destroy(mfx_icgen *ic);
+/* one full realisation; results stay on the device */
+int mfx_icgen_run(mfx_icgen *ic);
+/* device pointers of the FP64 particle arrays: what 0..2 positions x,y,z,
+ * 3..5 velocities x,y,z (n^3 doubles each, order (i,j,k)) */
+int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
+/* grids of the realisation: 0 wnoise, 1 phi, 2 phi2, 3 phi3, 4..6 A3, 7 tmp */
+int mfx_icgen_grid(mfx_icgen *ic, int which, mfx_grid **g);
+/* scalars: Dplus(astart), vfact(astart) */
+int mfx_icgen_scalars(mfx_icgen *ic, double *dplus, double *vfact);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

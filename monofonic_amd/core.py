@@ -1,0 +1,487 @@
+"""Python mirror of monofonIC's field-pipeline interfaces over the gfx950 C ABI.
+
+Names, argument meaning and error behaviour follow the reference:
+``Grid_FFT`` (include/grid_fft.hh), ``OrszagConvolver`` (include/convolution.hh),
+the ``op::`` output operators (include/operators.hh), the RNG plugins
+(include/random_plugin.hh) and ``cosmology::calculator``
+(include/cosmology_calculator.hh).  All arithmetic runs on the GPU (or, for the
+FP64 cosmology scalars, in the native host library); numpy arrays are only used
+to move data in and out.
+"""
+import ctypes
+from collections import namedtuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, load
+
+kspace_id, rspace_id = _lib.MFX_KSPACE, _lib.MFX_RSPACE
+
+
+def _prec(precision):
+    if precision in ("double", "f64", np.float64, 8):
+        return _lib.MFX_DOUBLE
+    if precision in ("float", "f32", np.float32, 4):
+        return _lib.MFX_FLOAT
+    raise ValueError(f"unknown precision {precision!r}")
+
+
+class Context:
+    """One HIP stream on one device (replaces the FFTW/OpenMP set-up of src/main.cc)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        call("mfx_ctx_create", device, ctypes.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        call("mfx_ctx_synchronize", self._h)
+
+    def record(self, slot):
+        call("mfx_event_record", self._h, slot)
+
+    def elapsed_ms(self, s0, s1):
+        ms = ctypes.c_float()
+        call("mfx_event_elapsed", self._h, s0, s1, ctypes.byref(ms))
+        return ms.value
+
+    def profile(self, on=True):
+        call("mfx_profile_enable", self._h, int(on))
+
+    def profile_reset(self):
+        call("mfx_profile_reset", self._h)
+
+    def profile_stats(self):
+        n = ctypes.c_int()
+        call("mfx_profile_count", self._h, ctypes.byref(n))
+        out = {}
+        buf = ctypes.create_string_buffer(128)
+        for i in range(n.value):
+            cnt, ms, by = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+            call("mfx_profile_get", self._h, i, buf, 128, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(by))
+            out[buf.value.decode()] = {"launches": cnt.value, "ms": ms.value, "bytes": by.value}
+        return out
+
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        call("mfx_device_alloc", self._h, nbytes, ctypes.byref(p))
+        return p
+
+    def free(self, p):
+        call("mfx_device_free", self._h, p)
+
+    def d2h(self, dev, nbytes, dtype):
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        call("mfx_memcpy_d2h", self._h, out.ctypes.data_as(ctypes.c_void_p), dev, nbytes)
+        return out
+
+    def close(self):
+        if self._h:
+            call("mfx_ctx_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Grid_FFT:
+    """Grid_FFT<real_t,false> (include/grid_fft.hh:40-914) on one GPU."""
+
+    def __init__(self, ctx, N, L, precision="double", initialspace=rspace_id):
+        N = (N, N, N) if np.isscalar(N) else tuple(int(x) for x in N)
+        L = (L, L, L) if np.isscalar(L) else tuple(float(x) for x in L)
+        self.ctx = ctx
+        self.n_ = N
+        self.length_ = L
+        self._h = ctypes.c_void_p()
+        call("mfx_grid_create", ctx.handle, N[0], N[1], N[2], L[0], L[1], L[2], _prec(precision), initialspace,
+             ctypes.byref(self._h))
+        nt, sp, pr = ctypes.c_size_t(), ctypes.c_int(), ctypes.c_int()
+        call("mfx_grid_info", self._h, ctypes.byref(nt), ctypes.byref(sp), ctypes.byref(pr))
+        self.ntot_ = nt.value
+        self.dtype = np.float64 if pr.value == 8 else np.float32
+        self.kfac_ = tuple(2.0 * np.pi / x for x in L)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def space_(self):
+        sp = ctypes.c_int()
+        call("mfx_grid_info", self._h, None, ctypes.byref(sp), None)
+        return sp.value
+
+    def reset(self):
+        if self._h:
+            call("mfx_grid_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.reset()
+        except Exception:
+            pass
+
+    # --- data movement (host mirror) ---
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.ntot_:
+            raise ValueError("array size must equal ntot_")
+        call("mfx_grid_upload", self._h, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+
+    def download(self):
+        a = np.empty(self.ntot_, dtype=self.dtype)
+        call("mfx_grid_download", self._h, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+        n0, n1, n2 = self.n_
+        return a.reshape(n0, n1, n2 + 2)
+
+    def kdata(self):
+        """k-space view (n0, n1, n2/2+1) complex"""
+        a = self.download()
+        ct = np.complex128 if self.dtype == np.float64 else np.complex64
+        return a.view(ct).reshape(self.n_[0], self.n_[1], self.n_[2] // 2 + 1)
+
+    def rdata(self):
+        return self.download()[:, :, :self.n_[2]]
+
+    # --- reference methods ---
+    def zero(self):
+        call("mfx_grid_zero", self._h)
+
+    def copy_from(self, g):
+        call("mfx_grid_copy_from", self._h, g.handle)
+
+    def FourierTransformForward(self, do_transform=True):
+        call("mfx_grid_fft_forward", self._h, int(do_transform))
+
+    def FourierTransformBackward(self, do_transform=True):
+        call("mfx_grid_fft_backward", self._h, int(do_transform))
+
+    def __imul__(self, x):
+        call("mfx_grid_scale", self._h, float(x))
+        return self
+
+    def __itruediv__(self, x):
+        call("mfx_grid_scale", self._h, 1.0 / float(x))
+        return self
+
+    def apply_InverseLaplacian(self):
+        call("mfx_grid_inverse_laplacian", self._h)
+        return self
+
+    def zero_DC_mode(self):
+        call("mfx_grid_zero_dc", self._h)
+
+    def _stats(self):
+        m, s = ctypes.c_double(), ctypes.c_double()
+        call("mfx_grid_stats", self._h, ctypes.byref(m), ctypes.byref(s))
+        return m.value, s.value
+
+    def mean(self):
+        return self._stats()[0]
+
+    def std(self):
+        return self._stats()[1]
+
+    def Compute_PowerSpectrum(self):
+        nb = self.n_[0]
+        bk, bP, beP, bc = (np.zeros(nb) for _ in range(4))
+        nbins = ctypes.c_int()
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        call("mfx_grid_power_spectrum", self._h, p(bk), p(bP), p(beP), p(bc), nb, ctypes.byref(nbins))
+        k = nbins.value
+        return bk[:k], bP[:k], beP[:k], bc[:k]
+
+
+# ------------------------------------------------------------------ output operators (include/operators.hh)
+Op = namedtuple("Op", "grid code fac")
+
+
+class op:
+    @staticmethod
+    def assign_to(g):
+        return Op(g, _lib.MFX_OP_ASSIGN, 0.0)
+
+    @staticmethod
+    def add_to(g):
+        return Op(g, _lib.MFX_OP_ADD, 0.0)
+
+    @staticmethod
+    def subtract_from(g):
+        return Op(g, _lib.MFX_OP_SUB, 0.0)
+
+    @staticmethod
+    def multiply_add_to(g, x):
+        return Op(g, _lib.MFX_OP_MULADD, float(x))
+
+
+def _i2(d):
+    return (ctypes.c_int * 2)(int(d[0]), int(d[1]))
+
+
+class OrszagConvolver:
+    """OrszagConvolver<real_t> (include/convolution.hh:429-669) with the
+    BaseConvolver convolve_* family (:74-292)."""
+
+    def __init__(self, ctx, N, L, precision="double"):
+        self.ctx = ctx
+        self._h = ctypes.c_void_p()
+        N = N[0] if not np.isscalar(N) else N
+        L = L[0] if not np.isscalar(L) else L
+        call("mfx_conv_create", ctx.handle, int(N), float(L), _prec(precision), ctypes.byref(self._h))
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("mfx_conv_destroy", self._h)
+                self._h = ctypes.c_void_p()
+        except Exception:
+            pass
+
+    def convolve_Gradients(self, inl, d1l, inr, d1r, o):
+        call("mfx_conv_gradients", self._h, inl.handle, int(d1l[0]), inr.handle, int(d1r[0]), o.grid.handle,
+             o.code, o.fac)
+
+    def convolve_Gradient_and_Hessian(self, inl, d1l, inr, d2r, o):
+        call("mfx_conv_gradient_and_hessian", self._h, inl.handle, int(d1l[0]), inr.handle, _i2(d2r),
+             o.grid.handle, o.code, o.fac)
+
+    def convolve_Hessians(self, inl, d2l, inr, d2r, *rest):
+        if len(rest) == 1:
+            o = rest[0]
+            call("mfx_conv_hessians", self._h, inl.handle, _i2(d2l), inr.handle, _i2(d2r), o.grid.handle, o.code,
+                 o.fac)
+        else:
+            g3, d3, o = rest
+            call("mfx_conv_hessians3", self._h, inl.handle, _i2(d2l), inr.handle, _i2(d2r), g3.handle, _i2(d3),
+                 o.grid.handle, o.code, o.fac)
+
+    def convolve_SumOfHessians(self, inl, d2l, inr, d2r1, d2r2, o):
+        call("mfx_conv_sum_of_hessians", self._h, inl.handle, _i2(d2l), inr.handle, _i2(d2r1), _i2(d2r2),
+             o.grid.handle, o.code, o.fac)
+
+    def convolve_DifferenceOfHessians(self, inl, d2l, inr, d2r1, d2r2, o):
+        call("mfx_conv_difference_of_hessians", self._h, inl.handle, _i2(d2l), inr.handle, _i2(d2r1), _i2(d2r2),
+             o.grid.handle, o.code, o.fac)
+
+    def multiply_field(self, inl, inr, o):
+        call("mfx_conv_multiply_field", self._h, inl.handle, inr.handle, o.grid.handle, o.code, o.fac)
+
+    def convolve2(self, a, ma, b, mb, o):
+        call("mfx_conv2", self._h, a.handle, ctypes.byref(ma), b.handle, ctypes.byref(mb), o.grid.handle, o.code,
+             o.fac)
+
+    def lpt_sources(self, phi, phi2, phi3=None, A3=(None, None, None), order=2):
+        h = lambda g: g.handle if g is not None else None
+        call("mfx_lpt_sources", self._h, phi.handle, phi2.handle, h(phi3), h(A3[0]), h(A3[1]), h(A3[2]), order)
+
+
+# ------------------------------------------------------------------ RNG plugins (include/random_plugin.hh)
+class RNG_music:
+    """MUSIC1 plugin (src/plugins/random_music.cc)"""
+
+    def __init__(self, seed, cubesize=32):
+        self.seed, self.cubesize = int(seed), int(cubesize)
+
+    def isMultiscale(self):
+        return True
+
+    def Fill_Grid(self, g):
+        call("mfx_rng_music1_fill", g.handle, self.seed, self.cubesize)
+
+
+class RNG_ngenic:
+    """NGENIC plugin (src/plugins/random_ngenic.cc)"""
+
+    def __init__(self, seed):
+        self.seed = int(seed)
+
+    def isMultiscale(self):
+        return False
+
+    def Fill_Grid(self, g):
+        call("mfx_rng_ngenic_fill", g.handle, self.seed)
+
+
+class RNG_threefry:
+    """Panphasia-HO Threefry-4x64-20 cell stream (first Box-Muller deviate per cell)"""
+
+    def __init__(self, seed):
+        self.seed = int(seed)
+
+    def isMultiscale(self):
+        return False
+
+    def Fill_Grid(self, g):
+        call("mfx_rng_threefry_fill", g.handle, self.seed)
+
+
+def select_RNG_plugin(name, seed):
+    """random_plugin.cc:136-191"""
+    table = {"MUSIC1": RNG_music, "NGENIC": RNG_ngenic, "PANPHASIA_THREEFRY": RNG_threefry}
+    if name not in table:
+        raise RuntimeError(f"Unknown random number generator plugin '{name}'")
+    return table[name](seed)
+
+
+def mt19937_raw(ctx, seeds, count):
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    out = np.zeros(len(seeds) * count, dtype=np.uint32)
+    call("mfx_rng_mt19937_raw", ctx.handle, seeds.ctypes.data_as(ctypes.c_void_p), len(seeds), count,
+         out.ctypes.data_as(ctypes.c_void_p))
+    return out.reshape(len(seeds), count)
+
+
+def ranlxd1_raw(ctx, seeds, count):
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    out = np.zeros(len(seeds) * count)
+    call("mfx_rng_ranlxd1_raw", ctx.handle, seeds.ctypes.data_as(ctypes.c_void_p), len(seeds), count,
+         out.ctypes.data_as(ctypes.c_void_p))
+    return out.reshape(len(seeds), count)
+
+
+def threefry_uniforms(ctx, cells, seed, count):
+    cells = np.ascontiguousarray(cells, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros(len(cells) * count)
+    call("mfx_rng_threefry_uniforms", ctx.handle, cells.ctypes.data_as(ctypes.c_void_p), len(cells), int(seed),
+         count, out.ctypes.data_as(ctypes.c_void_p))
+    return out.reshape(len(cells), count)
+
+
+# ------------------------------------------------------------------ cosmology (host FP64)
+class CosmologyCalculator:
+    """cosmology::calculator with the Eisenstein & Hu transfer function"""
+    TRANSFER, DPLUS, F, VFACT, H, AMPLITUDE, PNORM = range(7)
+
+    def __init__(self, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96, sigma_8=0.81, Tcmb=2.7255,
+                 zero_radiation=False):
+        load()
+        self._h = ctypes.c_void_p()
+        call("mfx_cosmo_create", H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb, int(zero_radiation),
+             ctypes.byref(self._h))
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("mfx_cosmo_destroy", self._h)
+        except Exception:
+            pass
+
+    def _q(self, what, x=0.0):
+        out = ctypes.c_double()
+        call("mfx_cosmo_query", self._h, what, float(x), ctypes.byref(out))
+        return out.value
+
+    def get_transfer(self, k):
+        return self._q(self.TRANSFER, k)
+
+    def get_growth_factor(self, a):
+        return self._q(self.DPLUS, a)
+
+    def get_f(self, a):
+        return self._q(self.F, a)
+
+    def get_vfact(self, a):
+        return self._q(self.VFACT, a)
+
+    def H_of_a(self, a):
+        return self._q(self.H, a)
+
+    def get_amplitude(self, k):
+        return self._q(self.AMPLITUDE, k)
+
+    @property
+    def pnorm(self):
+        return self._q(self.PNORM)
+
+    def amplitude_table(self, n, boxlen):
+        nt = 3 * (n // 2) ** 2 + 1
+        out = np.zeros(nt)
+        call("mfx_cosmo_amplitude_table", self._h, n, float(boxlen), out.ctypes.data_as(ctypes.c_void_p), nt)
+        return out
+
+
+# ------------------------------------------------------------------ k-space operators of ic_generator::run
+def noise_normalise(wn, volfac, fixing=False, inversion=False):
+    call("mfx_noise_normalise", wn.handle, float(volfac), int(fixing), int(inversion))
+
+
+def phi1(phi, wn, amp_table):
+    t = np.ascontiguousarray(amp_table, dtype=np.float64)
+    call("mfx_phi1", phi.handle, wn.handle, t.ctypes.data_as(ctypes.c_void_p), t.size)
+
+
+def assemble_kspace(tmp, phi, phi2, phi3, A3, order, kind, idim, vfac, unit=1.0):
+    h = lambda g: g.handle if g is not None else None
+    call("mfx_assemble_kspace", tmp.handle, phi.handle, h(phi2), h(phi3), h(A3[0]), h(A3[1]), h(A3[2]), order,
+         kind, idim, float(vfac), float(unit))
+
+
+# ------------------------------------------------------------------ LPT driver (src/ic_generator.cc:90-971)
+GENERATORS = {"MUSIC1": 0, "NGENIC": 1, "PANPHASIA_THREEFRY": 2}
+
+
+class ICGenerator:
+    """Device-resident ic_generator::run for the sc lattice (FP64 particles)."""
+
+    def __init__(self, ctx, GridRes, BoxLength, zstart, LPTorder=3, precision="double", generator="MUSIC1",
+                 seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
+                 sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True):
+        self.ctx = ctx
+        self.n = int(GridRes)
+        p = _lib.IcgenParams()
+        p.n, p.boxlen, p.zstart, p.lpt_order = self.n, float(BoxLength), float(zstart), int(LPTorder)
+        p.precision, p.generator, p.seed = _prec(precision), GENERATORS[generator], int(seed)
+        p.fixing, p.inversion = int(DoFixing), int(DoInversion)
+        p.H0, p.Omega_m, p.Omega_b, p.n_s, p.sigma_8, p.Tcmb = H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb
+        p.zero_radiation, p.use_fused_lpt = int(ZeroRadiation), int(fused)
+        self._h = ctypes.c_void_p()
+        call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("mfx_icgen_destroy", self._h)
+        except Exception:
+            pass
+
+    def run(self):
+        call("mfx_icgen_run", self._h)
+
+    def scalars(self):
+        d, v = ctypes.c_double(), ctypes.c_double()
+        call("mfx_icgen_scalars", self._h, ctypes.byref(d), ctypes.byref(v))
+        return d.value, v.value
+
+    def particles_dev(self, what):
+        p = ctypes.c_void_p()
+        call("mfx_icgen_particles", self._h, what, ctypes.byref(p))
+        return p
+
+    def particles(self, what):
+        """host copy of positions (0..2) or velocities (3..5), shape (n,n,n)"""
+        n = self.n
+        return self.ctx.d2h(self.particles_dev(what), n ** 3 * 8, np.float64).reshape(n, n, n)
+
+    def grid_handle(self, which):
+        p = ctypes.c_void_p()
+        call("mfx_icgen_grid", self._h, which, ctypes.byref(p))
+        return p
+
+    def grid_data(self, which):
+        """download grid `which` (0 wnoise, 1 phi, 2 phi2, 3 phi3, 4..6 A3, 7 tmp) as its raw padded array"""
+        h = self.grid_handle(which)
+        nt, sp, pr = ctypes.c_size_t(), ctypes.c_int(), ctypes.c_int()
+        call("mfx_grid_info", h, ctypes.byref(nt), ctypes.byref(sp), ctypes.byref(pr))
+        dt = np.float64 if pr.value == 8 else np.float32
+        a = np.empty(nt.value, dtype=dt)
+        call("mfx_grid_download", h, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+        return a.reshape(self.n, self.n, self.n + 2)

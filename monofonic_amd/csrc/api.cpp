@@ -1,0 +1,1225 @@
+// api.cpp -- C ABI (include/mfx.h) over the gfx950 kernels, and the host
+// counterpart of ic_generator::run for the sc lattice.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/mfx.h"
+#include "cosmology.hpp"
+#include "ops.hpp"
+
+using namespace mfx;
+
+namespace mfx {
+void ngenic_seedtable(unsigned* table, int n, long seed);
+}
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+template <class F>
+static int guard(F&& f) {
+  try {
+    f();
+    return MFX_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return MFX_EINVAL;
+  } catch (...) {
+    g_err = "unknown error";
+    return MFX_EINVAL;
+  }
+}
+
+namespace mfx {
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(MFX_EHIP, std::string("kernel launch ") + what + ": " + hipGetErrorString(e));
+}
+
+Ctx::~Ctx() {
+  for (auto& kv : tw) (void)hipFree(kv.second);
+  for (auto& r : pending) {
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+  }
+}
+
+template <typename T>
+const cx<T>* Ctx::twiddles(int L) {
+  auto key = std::make_pair(L, (int)sizeof(T));
+  auto it = tw.find(key);
+  if (it != tw.end()) return reinterpret_cast<const cx<T>*>(it->second);
+  std::vector<cx<T>> h(L);
+  for (int n = 0; n < L; ++n) {
+    // exp(-2 pi i n/L) in extended precision, rounded once to T
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)n / (long double)L;
+    h[n].x = (T)cosl(a);
+    h[n].y = (T)(-sinl(a));
+  }
+  void* d = nullptr;
+  MFX_HIP(hipMalloc(&d, sizeof(cx<T>) * L));
+  MFX_HIP(hipMemcpy(d, h.data(), sizeof(cx<T>) * L, hipMemcpyHostToDevice));
+  tw[key] = d;
+  return reinterpret_cast<const cx<T>*>(d);
+}
+template const cx<float>* Ctx::twiddles<float>(int);
+template const cx<double>* Ctx::twiddles<double>(int);
+
+void* Ctx::alloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 16;
+  MFX_HIP(hipMalloc(&p, bytes));
+  return p;
+}
+void Ctx::release(void* p) {
+  if (p) MFX_HIP(hipFree(p));
+}
+}  // namespace mfx
+
+// ------------------------------------------------------------------ handles
+struct mfx_ctx {
+  Ctx c;
+  hipEvent_t ev[16] = {};
+};
+
+struct mfx_grid {
+  mfx_ctx* ctx;
+  size_t n[3];
+  double L[3];
+  int prec;
+  int space;
+  size_t ntot;
+  void* d;
+};
+
+struct mfx_conv {
+  mfx_ctx* ctx;
+  int n;
+  double L;
+  int prec;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  void* lpt_ws = nullptr;
+  size_t lpt_bytes = 0;
+  mfx_grid* fbuf = nullptr;  // convolve3 intermediate (convolution.hh:469)
+};
+
+struct mfx_cosmo {
+  std::unique_ptr<Cosmology> c;
+};
+
+static void require_grid(const mfx_grid* g) { MFX_REQUIRE(g != nullptr, "null grid handle"); }
+
+extern "C" {
+
+const char* mfx_last_error(void) { return g_err.c_str(); }
+int mfx_version(void) { return 1; }
+
+int mfx_ctx_create(int device, mfx_ctx** out) {
+  return guard([&] {
+    MFX_REQUIRE(out != nullptr, "null output pointer");
+    int ndev = 0;
+    MFX_HIP(hipGetDeviceCount(&ndev));
+    MFX_REQUIRE(device >= 0 && device < ndev, "invalid HIP device index");
+    MFX_HIP(hipSetDevice(device));
+    auto* c = new mfx_ctx();
+    c->c.device = device;
+    MFX_HIP(hipStreamCreate(&c->c.stream));
+    for (auto& e : c->ev) MFX_HIP(hipEventCreate(&e));
+    *out = c;
+  });
+}
+
+int mfx_ctx_destroy(mfx_ctx* c) {
+  return guard([&] {
+    if (!c) return;
+    (void)hipSetDevice(c->c.device);
+    (void)hipStreamSynchronize(c->c.stream);
+    for (auto& e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->c.stream);
+    delete c;
+  });
+}
+
+int mfx_ctx_synchronize(mfx_ctx* c) {
+  return guard([&] {
+    MFX_REQUIRE(c, "null context");
+    MFX_HIP(hipStreamSynchronize(c->c.stream));
+  });
+}
+
+int mfx_event_record(mfx_ctx* c, int slot) {
+  return guard([&] {
+    MFX_REQUIRE(c && slot >= 0 && slot < 16, "invalid event slot");
+    MFX_HIP(hipEventRecord(c->ev[slot], c->c.stream));
+  });
+}
+
+int mfx_event_elapsed(mfx_ctx* c, int s0, int s1, float* ms) {
+  return guard([&] {
+    MFX_REQUIRE(c && s0 >= 0 && s0 < 16 && s1 >= 0 && s1 < 16 && ms, "invalid event slot");
+    MFX_HIP(hipEventSynchronize(c->ev[s1]));
+    MFX_HIP(hipEventElapsedTime(ms, c->ev[s0], c->ev[s1]));
+  });
+}
+
+static void collect(Ctx& c) {
+  if (c.pending.empty()) return;
+  MFX_HIP(hipStreamSynchronize(c.stream));
+  for (auto& r : c.pending) {
+    float ms = 0;
+    MFX_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+    auto& s = c.stats[r.name];
+    s.count += 1;
+    s.ms += ms;
+    s.bytes += r.bytes;
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+  }
+  c.pending.clear();
+}
+
+int mfx_profile_enable(mfx_ctx* c, int on) {
+  return guard([&] {
+    MFX_REQUIRE(c, "null context");
+    collect(c->c);
+    c->c.profile = on != 0;
+  });
+}
+int mfx_profile_reset(mfx_ctx* c) {
+  return guard([&] {
+    MFX_REQUIRE(c, "null context");
+    collect(c->c);
+    c->c.stats.clear();
+  });
+}
+int mfx_profile_count(mfx_ctx* c, int* count) {
+  return guard([&] {
+    MFX_REQUIRE(c && count, "null argument");
+    collect(c->c);
+    *count = (int)c->c.stats.size();
+  });
+}
+int mfx_profile_get(mfx_ctx* c, int idx, char* name, size_t namelen, int* launches, double* ms, double* bytes) {
+  return guard([&] {
+    MFX_REQUIRE(c, "null context");
+    collect(c->c);
+    MFX_REQUIRE(idx >= 0 && idx < (int)c->c.stats.size(), "profile index out of range");
+    auto it = c->c.stats.begin();
+    std::advance(it, idx);
+    if (name && namelen) {
+      std::strncpy(name, it->first.c_str(), namelen - 1);
+      name[namelen - 1] = 0;
+    }
+    if (launches) *launches = it->second.count;
+    if (ms) *ms = it->second.ms;
+    if (bytes) *bytes = it->second.bytes;
+  });
+}
+
+int mfx_device_alloc(mfx_ctx* c, size_t bytes, void** ptr) {
+  return guard([&] {
+    MFX_REQUIRE(c && ptr, "null argument");
+    *ptr = c->c.alloc(bytes);
+  });
+}
+int mfx_device_free(mfx_ctx* c, void* ptr) {
+  return guard([&] {
+    MFX_REQUIRE(c, "null context");
+    MFX_HIP(hipStreamSynchronize(c->c.stream));
+    c->c.release(ptr);
+  });
+}
+int mfx_memcpy_d2h(mfx_ctx* c, void* host, const void* dev, size_t bytes) {
+  return guard([&] {
+    MFX_REQUIRE(c && host && dev, "null argument");
+    MFX_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->c.stream));
+    MFX_HIP(hipStreamSynchronize(c->c.stream));
+  });
+}
+int mfx_memcpy_h2d(mfx_ctx* c, void* dev, const void* host, size_t bytes) {
+  return guard([&] {
+    MFX_REQUIRE(c && host && dev, "null argument");
+    MFX_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->c.stream));
+    MFX_HIP(hipStreamSynchronize(c->c.stream));
+  });
+}
+
+// ------------------------------------------------------------------ grids
+int mfx_grid_create(mfx_ctx* c, size_t n0, size_t n1, size_t n2, double L0, double L1, double L2, int precision,
+                    int space, mfx_grid** out) {
+  return guard([&] {
+    MFX_REQUIRE(c && out, "null argument");
+    MFX_REQUIRE(precision == MFX_FLOAT || precision == MFX_DOUBLE, "precision must be MFX_FLOAT or MFX_DOUBLE");
+    MFX_REQUIRE(space == MFX_KSPACE || space == MFX_RSPACE, "invalid space");
+    MFX_REQUIRE(n0 > 0 && n1 > 0 && n2 > 0 && n2 % 2 == 0, "grid sizes must be positive, last dimension even");
+    MFX_REQUIRE(L0 > 0 && L1 > 0 && L2 > 0, "box lengths must be positive");
+    auto* g = new mfx_grid();
+    g->ctx = c;
+    g->n[0] = n0;
+    g->n[1] = n1;
+    g->n[2] = n2;
+    g->L[0] = L0;
+    g->L[1] = L1;
+    g->L[2] = L2;
+    g->prec = precision;
+    g->space = space;
+    g->ntot = (n2 + 2) * n1 * n0;
+    try {
+      g->d = c->c.alloc(g->ntot * (size_t)precision);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    *out = g;
+  });
+}
+
+int mfx_grid_destroy(mfx_grid* g) {
+  return guard([&] {
+    if (!g) return;
+    MFX_HIP(hipStreamSynchronize(g->ctx->c.stream));
+    g->ctx->c.release(g->d);
+    delete g;
+  });
+}
+
+int mfx_grid_info(const mfx_grid* g, size_t* ntot, int* space, int* precision) {
+  return guard([&] {
+    require_grid(g);
+    if (ntot) *ntot = g->ntot;
+    if (space) *space = g->space;
+    if (precision) *precision = g->prec;
+  });
+}
+
+int mfx_grid_data(mfx_grid* g, void** p) {
+  return guard([&] {
+    require_grid(g);
+    MFX_REQUIRE(p, "null argument");
+    *p = g->d;
+  });
+}
+
+int mfx_grid_upload(mfx_grid* g, const void* host, size_t nbytes) {
+  return guard([&] {
+    require_grid(g);
+    MFX_REQUIRE(host && nbytes == g->ntot * (size_t)g->prec, "upload size must equal ntot*precision");
+    MFX_HIP(hipMemcpyAsync(g->d, host, nbytes, hipMemcpyHostToDevice, g->ctx->c.stream));
+    MFX_HIP(hipStreamSynchronize(g->ctx->c.stream));
+  });
+}
+
+int mfx_grid_download(const mfx_grid* g, void* host, size_t nbytes) {
+  return guard([&] {
+    require_grid(g);
+    MFX_REQUIRE(host && nbytes == g->ntot * (size_t)g->prec, "download size must equal ntot*precision");
+    MFX_HIP(hipMemcpyAsync(host, g->d, nbytes, hipMemcpyDeviceToHost, g->ctx->c.stream));
+    MFX_HIP(hipStreamSynchronize(g->ctx->c.stream));
+  });
+}
+
+static void grid_zero(mfx_grid* g) {
+  if (g->prec == MFX_FLOAT)
+    fill<float>(g->ctx->c, (float*)g->d, (long long)g->ntot, 0.0f);
+  else
+    fill<double>(g->ctx->c, (double*)g->d, (long long)g->ntot, 0.0);
+}
+
+int mfx_grid_zero(mfx_grid* g) {
+  return guard([&] {
+    require_grid(g);
+    grid_zero(g);
+  });
+}
+
+static void same_shape(const mfx_grid* a, const mfx_grid* b) {
+  MFX_REQUIRE(a->n[0] == b->n[0] && a->n[1] == b->n[1] && a->n[2] == b->n[2] && a->prec == b->prec,
+              "grids differ in size or precision");
+}
+
+int mfx_grid_copy_from(mfx_grid* dst, const mfx_grid* src) {
+  return guard([&] {
+    require_grid(dst);
+    require_grid(src);
+    same_shape(dst, src);
+    dst->space = src->space;
+    MFX_HIP(hipMemcpyAsync(dst->d, src->d, dst->ntot * (size_t)dst->prec, hipMemcpyDeviceToDevice,
+                           dst->ctx->c.stream));
+  });
+}
+
+static double norm_fac(const mfx_grid* g) {
+  return 1.0 / std::sqrt((double)g->n[0] * (double)g->n[1] * (double)g->n[2]);
+}
+
+static void grid_forward(mfx_grid* g, bool do_transform) {
+  if (g->space == MFX_KSPACE) return;
+  if (do_transform) {
+    if (g->prec == MFX_FLOAT)
+      fft_r2c_3d<float>(g->ctx->c, (float*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], (float)norm_fac(g));
+    else
+      fft_r2c_3d<double>(g->ctx->c, (double*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], norm_fac(g));
+  }
+  g->space = MFX_KSPACE;
+}
+
+static void grid_backward(mfx_grid* g, bool do_transform) {
+  if (g->space == MFX_RSPACE) return;
+  if (do_transform) {
+    if (g->prec == MFX_FLOAT)
+      fft_c2r_3d<float>(g->ctx->c, (float*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], (float)norm_fac(g));
+    else
+      fft_c2r_3d<double>(g->ctx->c, (double*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], norm_fac(g));
+  }
+  g->space = MFX_RSPACE;
+}
+
+int mfx_grid_fft_forward(mfx_grid* g, int do_transform) {
+  return guard([&] {
+    require_grid(g);
+    grid_forward(g, do_transform != 0);
+  });
+}
+
+int mfx_grid_fft_backward(mfx_grid* g, int do_transform) {
+  return guard([&] {
+    require_grid(g);
+    grid_backward(g, do_transform != 0);
+  });
+}
+
+int mfx_grid_scale(mfx_grid* g, double s) {
+  return guard([&] {
+    require_grid(g);
+    if (g->prec == MFX_FLOAT)
+      scale<float>(g->ctx->c, (float*)g->d, (long long)g->ntot, s);
+    else
+      scale<double>(g->ctx->c, (double*)g->d, (long long)g->ntot, s);
+  });
+}
+
+static void require_cubic(const mfx_grid* g) {
+  MFX_REQUIRE(g->n[0] == g->n[1] && g->n[1] == g->n[2] && g->L[0] == g->L[1] && g->L[1] == g->L[2],
+              "operation implemented for cubic grids only");
+}
+
+int mfx_grid_inverse_laplacian(mfx_grid* g) {
+  return guard([&] {
+    require_grid(g);
+    require_cubic(g);
+    grid_forward(g, true);
+    if (g->prec == MFX_FLOAT)
+      inverse_laplacian<float>(g->ctx->c, (cx<float>*)g->d, (int)g->n[0], g->L[0]);
+    else
+      inverse_laplacian<double>(g->ctx->c, (cx<double>*)g->d, (int)g->n[0], g->L[0]);
+  });
+}
+
+int mfx_grid_stats(mfx_grid* g, double* mean, double* sd) {
+  return guard([&] {
+    require_grid(g);
+    MFX_REQUIRE(g->space == MFX_RSPACE, "stats are computed in real space");
+    double s[2];
+    if (g->prec == MFX_FLOAT)
+      sum2<float>(g->ctx->c, (float*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], (long long)g->n[2] + 2, s);
+    else
+      sum2<double>(g->ctx->c, (double*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], (long long)g->n[2] + 2, s);
+    const double cnt = (double)g->n[0] * g->n[1] * g->n[2];
+    const double m1 = s[0] / cnt, m2 = s[1] / cnt;
+    if (mean) *mean = m1;
+    if (sd) *sd = std::sqrt(m2 - m1 * m1);
+  });
+}
+
+int mfx_grid_zero_dc(mfx_grid* g) {
+  return guard([&] {
+    require_grid(g);
+    if (g->space == MFX_KSPACE) {
+      // k-space branch: cdata_[0] = 0 (grid_fft.hh:846-850)
+      MFX_HIP(hipMemsetAsync(g->d, 0, 2 * (size_t)g->prec, g->ctx->c.stream));
+    } else {
+      // real-space branch: subtract the mean (grid_fft.hh:851-886)
+      double s[2];
+      const long long pitch = (long long)g->n[2] + 2;
+      if (g->prec == MFX_FLOAT)
+        sum2<float>(g->ctx->c, (float*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], pitch, s);
+      else
+        sum2<double>(g->ctx->c, (double*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], pitch, s);
+      const double m = s[0] / ((double)g->n[0] * g->n[1] * g->n[2]);
+      if (g->prec == MFX_FLOAT)
+        add_const<float>(g->ctx->c, (float*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], pitch, -m);
+      else
+        add_const<double>(g->ctx->c, (double*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], pitch, -m);
+    }
+  });
+}
+
+int mfx_grid_power_spectrum(mfx_grid* g, double* bk, double* bP, double* beP, double* bc, int maxbins, int* nbins) {
+  return guard([&] {
+    require_grid(g);
+    require_cubic(g);
+    MFX_REQUIRE(bk && bP && beP && bc && nbins, "null argument");
+    grid_forward(g, true);
+    if (g->prec == MFX_FLOAT)
+      *nbins = power_spectrum<float>(g->ctx->c, (cx<float>*)g->d, (int)g->n[0], g->L[0], bk, bP, beP, bc, maxbins);
+    else
+      *nbins = power_spectrum<double>(g->ctx->c, (cx<double>*)g->d, (int)g->n[0], g->L[0], bk, bP, beP, bc, maxbins);
+  });
+}
+
+int mfx_noise_normalise(mfx_grid* wn, double volfac, int fixing, int inversion) {
+  return guard([&] {
+    require_grid(wn);
+    require_cubic(wn);
+    MFX_REQUIRE(wn->space == MFX_KSPACE, "noise normalisation acts on k-space noise");
+    if (wn->prec == MFX_FLOAT)
+      noise_normalise<float>(wn->ctx->c, (cx<float>*)wn->d, (int)wn->n[0], volfac, fixing, inversion);
+    else
+      noise_normalise<double>(wn->ctx->c, (cx<double>*)wn->d, (int)wn->n[0], volfac, fixing, inversion);
+  });
+}
+
+static void phi1_impl(mfx_grid* phi, const mfx_grid* wn, const double* amp_dev) {
+  if (phi->prec == MFX_FLOAT)
+    phi1<float>(phi->ctx->c, (cx<float>*)phi->d, (const cx<float>*)wn->d, (int)phi->n[0], phi->L[0], amp_dev);
+  else
+    phi1<double>(phi->ctx->c, (cx<double>*)phi->d, (const cx<double>*)wn->d, (int)phi->n[0], phi->L[0], amp_dev);
+}
+
+int mfx_phi1(mfx_grid* phi, const mfx_grid* wn, const double* amp_table, size_t ntable) {
+  return guard([&] {
+    require_grid(phi);
+    require_grid(wn);
+    require_cubic(phi);
+    same_shape(phi, wn);
+    MFX_REQUIRE(wn->space == MFX_KSPACE, "white noise must be in k space");
+    const size_t nh = phi->n[0] / 2;
+    MFX_REQUIRE(amp_table && ntable >= 3 * nh * nh + 1, "amplitude table too short (need 3*(n/2)^2+1 entries)");
+    Ctx& c = phi->ctx->c;
+    double* d = (double*)c.alloc(sizeof(double) * ntable);
+    MFX_HIP(hipMemcpyAsync(d, amp_table, sizeof(double) * ntable, hipMemcpyHostToDevice, c.stream));
+    phi->space = MFX_KSPACE;
+    phi1_impl(phi, wn, d);
+    MFX_HIP(hipStreamSynchronize(c.stream));
+    c.release(d);
+  });
+}
+
+// ------------------------------------------------------------------ convolver
+int mfx_conv_create(mfx_ctx* c, size_t n, double L, int precision, mfx_conv** out) {
+  return guard([&] {
+    MFX_REQUIRE(c && out, "null argument");
+    MFX_REQUIRE(n % 2 == 0 && n >= 16, "Orszag convolver requires an even grid size >= 16");
+    MFX_REQUIRE(precision == MFX_FLOAT || precision == MFX_DOUBLE, "invalid precision");
+    auto* cv = new mfx_conv();
+    cv->ctx = c;
+    cv->n = (int)n;
+    cv->L = L;
+    cv->prec = precision;
+    cv->ws_bytes = conv_workspace_bytes((int)n, precision);
+    try {
+      cv->ws = c->c.alloc(cv->ws_bytes);
+    } catch (...) {
+      delete cv;
+      throw;
+    }
+    *out = cv;
+  });
+}
+
+int mfx_conv_destroy(mfx_conv* cv) {
+  return guard([&] {
+    if (!cv) return;
+    MFX_HIP(hipStreamSynchronize(cv->ctx->c.stream));
+    cv->ctx->c.release(cv->ws);
+    cv->ctx->c.release(cv->lpt_ws);
+    if (cv->fbuf) {
+      cv->ctx->c.release(cv->fbuf->d);
+      delete cv->fbuf;
+    }
+    delete cv;
+  });
+}
+
+static MultSpec to_spec(const mfx_mult* m) {
+  MFX_REQUIRE(m != nullptr, "null multiplier");
+  MFX_REQUIRE(m->nterms >= 1 && m->nterms <= 2, "multiplier must have 1 or 2 terms");
+  MultSpec s;
+  std::memset(&s, 0, sizeof(s));
+  s.nterms = m->nterms;
+  for (int t = 0; t < 2; ++t) {
+    s.coef_re[t] = m->coef_re[t];
+    s.coef_im[t] = m->coef_im[t];
+    s.nyq[t] = m->nyq[t];
+    for (int d = 0; d < 3; ++d) {
+      MFX_REQUIRE(t >= m->nterms || (m->pow[t][d] >= 0 && m->pow[t][d] <= 4), "multiplier power out of range");
+      s.pow[t][d] = m->pow[t][d];
+    }
+  }
+  return s;
+}
+
+static void conv_check(mfx_conv* cv, const mfx_grid* g) {
+  require_grid(g);
+  MFX_REQUIRE(g->n[0] == (size_t)cv->n && g->n[1] == (size_t)cv->n && g->n[2] == (size_t)cv->n,
+              "grid size does not match the convolver");
+  MFX_REQUIRE(g->prec == cv->prec, "grid precision does not match the convolver");
+}
+
+static void conv2_impl(mfx_conv* cv, mfx_grid* a, const MultSpec& ma, mfx_grid* b, const MultSpec& mb,
+                       mfx_grid* out, int op, double opfac) {
+  conv_check(cv, a);
+  conv_check(cv, b);
+  conv_check(cv, out);
+  MFX_REQUIRE(op >= MFX_OP_ASSIGN && op <= MFX_OP_MULADD, "invalid output operator");
+  // BaseConvolver: inputs are transformed to Fourier space first
+  grid_forward(a, true);
+  grid_forward(b, true);
+  // the output grid is declared k-space (FourierTransformForward(false))
+  MFX_REQUIRE(out->space == MFX_KSPACE || op == MFX_OP_ASSIGN, "output grid of an accumulating op must be in k space");
+  out->space = MFX_KSPACE;
+  Ctx& c = cv->ctx->c;
+  if (cv->prec == MFX_FLOAT)
+    conv2<float>(c, (const cx<float>*)a->d, ma, (const cx<float>*)b->d, mb, (cx<float>*)out->d, op, opfac, cv->n,
+                 cv->L, cv->ws);
+  else
+    conv2<double>(c, (const cx<double>*)a->d, ma, (const cx<double>*)b->d, mb, (cx<double>*)out->d, op, opfac, cv->n,
+                  cv->L, cv->ws);
+}
+
+int mfx_conv2(mfx_conv* cv, mfx_grid* a, const mfx_mult* ma, mfx_grid* b, const mfx_mult* mb, mfx_grid* out, int op,
+              double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv, "null convolver");
+    conv2_impl(cv, a, to_spec(ma), b, to_spec(mb), out, op, opfac);
+  });
+}
+
+static mfx_grid* conv_fbuf(mfx_conv* cv) {
+  if (!cv->fbuf) {
+    auto* g = new mfx_grid();
+    g->ctx = cv->ctx;
+    g->n[0] = g->n[1] = g->n[2] = cv->n;
+    g->L[0] = g->L[1] = g->L[2] = cv->L;
+    g->prec = cv->prec;
+    g->space = MFX_KSPACE;
+    g->ntot = ((size_t)cv->n + 2) * cv->n * cv->n;
+    try {
+      g->d = cv->ctx->c.alloc(g->ntot * (size_t)g->prec);
+    } catch (...) {
+      delete g;
+      throw;
+    }
+    cv->fbuf = g;
+  }
+  return cv->fbuf;
+}
+
+static MultSpec identity_spec() {
+  MultSpec s;
+  std::memset(&s, 0, sizeof(s));
+  s.nterms = 1;
+  s.coef_re[0] = 1.0;
+  return s;
+}
+
+static void conv3_impl(mfx_conv* cv, mfx_grid* a, const MultSpec& ma, mfx_grid* b, const MultSpec& mb, mfx_grid* c,
+                       const MultSpec& mc, mfx_grid* out, int op, double opfac) {
+  mfx_grid* fb = conv_fbuf(cv);
+  conv2_impl(cv, a, ma, b, mb, fb, MFX_OP_ASSIGN, 0.0);
+  conv2_impl(cv, fb, identity_spec(), c, mc, out, op, opfac);
+}
+
+int mfx_conv3(mfx_conv* cv, mfx_grid* a, const mfx_mult* ma, mfx_grid* b, const mfx_mult* mb, mfx_grid* c,
+              const mfx_mult* mc, mfx_grid* out, int op, double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv, "null convolver");
+    conv3_impl(cv, a, to_spec(ma), b, to_spec(mb), c, to_spec(mc), out, op, opfac);
+  });
+}
+
+// multipliers of BaseConvolver (convolution.hh:74-292)
+static void dim_ok(int d) { MFX_REQUIRE(d >= 0 && d < 3, "direction index must be 0, 1 or 2"); }
+static MultSpec term_spec(double cre, double cim, int a, int b, int nyq) {
+  MultSpec s;
+  std::memset(&s, 0, sizeof(s));
+  s.nterms = 1;
+  s.coef_re[0] = cre;
+  s.coef_im[0] = cim;
+  if (a >= 0) s.pow[0][a] += 1;
+  if (b >= 0) s.pow[0][b] += 1;
+  s.nyq[0] = nyq;
+  return s;
+}
+static MultSpec hess_spec(const int* d) {
+  dim_ok(d[0]);
+  dim_ok(d[1]);
+  return term_spec(-1.0, 0.0, d[0], d[1], 1);
+}
+static MultSpec two_hess_spec(const int* d1, const int* d2, double sign2) {
+  MultSpec s = hess_spec(d1);
+  MultSpec t = hess_spec(d2);
+  s.nterms = 2;
+  s.coef_re[1] = -sign2;
+  s.coef_im[1] = 0.0;
+  for (int q = 0; q < 3; ++q) s.pow[1][q] = t.pow[0][q];
+  s.nyq[1] = 1;
+  return s;
+}
+
+int mfx_conv_gradients(mfx_conv* cv, mfx_grid* inl, int d1l, mfx_grid* inr, int d1r, mfx_grid* out, int op,
+                       double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv, "null convolver");
+    dim_ok(d1l);
+    dim_ok(d1r);
+    conv2_impl(cv, inl, term_spec(0.0, 1.0, d1l, -1, 1), inr, term_spec(0.0, 1.0, d1r, -1, 1), out, op, opfac);
+  });
+}
+
+int mfx_conv_gradient_and_hessian(mfx_conv* cv, mfx_grid* inl, int d1l, mfx_grid* inr, const int d2r[2],
+                                  mfx_grid* out, int op, double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv && d2r, "null argument");
+    dim_ok(d1l);
+    dim_ok(d2r[0]);
+    dim_ok(d2r[1]);
+    conv2_impl(cv, inl, term_spec(0.0, -1.0, d1l, -1, 0), inr, term_spec(-1.0, 0.0, d2r[0], d2r[1], 0), out, op,
+               opfac);
+  });
+}
+
+int mfx_conv_hessians(mfx_conv* cv, mfx_grid* inl, const int d2l[2], mfx_grid* inr, const int d2r[2], mfx_grid* out,
+                      int op, double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv && d2l && d2r, "null argument");
+    conv2_impl(cv, inl, hess_spec(d2l), inr, hess_spec(d2r), out, op, opfac);
+  });
+}
+
+int mfx_conv_hessians3(mfx_conv* cv, mfx_grid* inl, const int d2l[2], mfx_grid* inm, const int d2m[2], mfx_grid* inr,
+                       const int d2r[2], mfx_grid* out, int op, double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv && d2l && d2m && d2r, "null argument");
+    conv3_impl(cv, inl, hess_spec(d2l), inm, hess_spec(d2m), inr, hess_spec(d2r), out, op, opfac);
+  });
+}
+
+int mfx_conv_sum_of_hessians(mfx_conv* cv, mfx_grid* inl, const int d2l[2], mfx_grid* inr, const int d2r1[2],
+                             const int d2r2[2], mfx_grid* out, int op, double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv && d2l && d2r1 && d2r2, "null argument");
+    conv2_impl(cv, inl, hess_spec(d2l), inr, two_hess_spec(d2r1, d2r2, +1.0), out, op, opfac);
+  });
+}
+
+int mfx_conv_difference_of_hessians(mfx_conv* cv, mfx_grid* inl, const int d2l[2], mfx_grid* inr, const int d2r1[2],
+                                    const int d2r2[2], mfx_grid* out, int op, double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv && d2l && d2r1 && d2r2, "null argument");
+    conv2_impl(cv, inl, hess_spec(d2l), inr, two_hess_spec(d2r1, d2r2, -1.0), out, op, opfac);
+  });
+}
+
+int mfx_conv_multiply_field(mfx_conv* cv, mfx_grid* inl, mfx_grid* inr, mfx_grid* out, int op, double opfac) {
+  return guard([&] {
+    MFX_REQUIRE(cv, "null convolver");
+    conv2_impl(cv, inl, identity_spec(), inr, identity_spec(), out, op, opfac);
+  });
+}
+
+// the reference's call sequence (src/ic_generator.cc:471-545) through conv2
+static void lpt_sequence(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3[3], int order) {
+  auto H = [](int a, int b) {
+    int d[2] = {a, b};
+    return hess_spec(d);
+  };
+  auto S = [](int a, int b, int c, int d, double sgn) {
+    int d1[2] = {a, b}, d2[2] = {c, d};
+    return two_hess_spec(d1, d2, sgn);
+  };
+  conv2_impl(cv, phi, H(0, 0), phi, S(1, 1, 2, 2, 1), phi2, MFX_OP_ASSIGN, 0);
+  conv2_impl(cv, phi, H(1, 1), phi, H(2, 2), phi2, MFX_OP_ADD, 0);
+  conv2_impl(cv, phi, H(0, 1), phi, H(0, 1), phi2, MFX_OP_SUB, 0);
+  conv2_impl(cv, phi, H(0, 2), phi, H(0, 2), phi2, MFX_OP_SUB, 0);
+  conv2_impl(cv, phi, H(1, 2), phi, H(1, 2), phi2, MFX_OP_SUB, 0);
+  grid_forward(phi2, true);
+  if (phi2->prec == MFX_FLOAT)
+    inverse_laplacian<float>(cv->ctx->c, (cx<float>*)phi2->d, cv->n, cv->L);
+  else
+    inverse_laplacian<double>(cv->ctx->c, (cx<double>*)phi2->d, cv->n, cv->L);
+  if (order < 3) return;
+  conv3_impl(cv, phi, H(0, 0), phi, H(1, 1), phi, H(2, 2), phi3, MFX_OP_ASSIGN, 0);
+  conv3_impl(cv, phi, H(0, 1), phi, H(0, 2), phi, H(1, 2), phi3, MFX_OP_MULADD, 2.0);
+  conv3_impl(cv, phi, H(1, 2), phi, H(1, 2), phi, H(0, 0), phi3, MFX_OP_SUB, 0);
+  conv3_impl(cv, phi, H(0, 2), phi, H(0, 2), phi, H(1, 1), phi3, MFX_OP_SUB, 0);
+  conv3_impl(cv, phi, H(0, 1), phi, H(0, 1), phi, H(2, 2), phi3, MFX_OP_SUB, 0);
+  conv2_impl(cv, phi, H(0, 0), phi2, S(1, 1, 2, 2, 1), phi3, MFX_OP_MULADD, -5.0 / 7.0);
+  conv2_impl(cv, phi, H(1, 1), phi2, S(2, 2, 0, 0, 1), phi3, MFX_OP_MULADD, -5.0 / 7.0);
+  conv2_impl(cv, phi, H(2, 2), phi2, S(0, 0, 1, 1, 1), phi3, MFX_OP_MULADD, -5.0 / 7.0);
+  conv2_impl(cv, phi, H(0, 1), phi2, H(0, 1), phi3, MFX_OP_MULADD, +10.0 / 7.0);
+  conv2_impl(cv, phi, H(0, 2), phi2, H(0, 2), phi3, MFX_OP_MULADD, +10.0 / 7.0);
+  conv2_impl(cv, phi, H(1, 2), phi2, H(1, 2), phi3, MFX_OP_MULADD, +10.0 / 7.0);
+  if (phi3->prec == MFX_FLOAT)
+    inverse_laplacian<float>(cv->ctx->c, (cx<float>*)phi3->d, cv->n, cv->L);
+  else
+    inverse_laplacian<double>(cv->ctx->c, (cx<double>*)phi3->d, cv->n, cv->L);
+  for (int idim = 0; idim < 3; ++idim) {
+    const int p = (idim + 1) % 3, pp = (idim + 2) % 3;
+    conv2_impl(cv, phi2, H(idim, p), phi, H(idim, pp), A3[idim], MFX_OP_ASSIGN, 0);
+    conv2_impl(cv, phi2, H(idim, pp), phi, H(idim, p), A3[idim], MFX_OP_SUB, 0);
+    conv2_impl(cv, phi, H(p, pp), phi2, S(p, p, pp, pp, -1), A3[idim], MFX_OP_ADD, 0);
+    conv2_impl(cv, phi2, H(p, pp), phi, S(p, p, pp, pp, -1), A3[idim], MFX_OP_SUB, 0);
+    if (A3[idim]->prec == MFX_FLOAT)
+      inverse_laplacian<float>(cv->ctx->c, (cx<float>*)A3[idim]->d, cv->n, cv->L);
+    else
+      inverse_laplacian<double>(cv->ctx->c, (cx<double>*)A3[idim]->d, cv->n, cv->L);
+  }
+}
+
+static void lpt_fused(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3[3], int order) {
+  const size_t need = lpt_workspace_bytes(cv->n, cv->prec, order);
+  if (cv->lpt_bytes < need) {
+    MFX_HIP(hipStreamSynchronize(cv->ctx->c.stream));
+    cv->ctx->c.release(cv->lpt_ws);
+    cv->lpt_ws = nullptr;
+    cv->lpt_bytes = 0;
+    cv->lpt_ws = cv->ctx->c.alloc(need);
+    cv->lpt_bytes = need;
+  }
+  grid_forward(phi, true);
+  phi2->space = MFX_KSPACE;
+  if (order > 2) {
+    phi3->space = MFX_KSPACE;
+    for (int d = 0; d < 3; ++d) A3[d]->space = MFX_KSPACE;
+  }
+  auto P = [](mfx_grid* g) { return g ? g->d : nullptr; };
+  if (cv->prec == MFX_FLOAT)
+    lpt_sources<float>(cv->ctx->c, (const cx<float>*)phi->d, (cx<float>*)phi2->d, (cx<float>*)P(phi3),
+                       (cx<float>*)P(A3[0]), (cx<float>*)P(A3[1]), (cx<float>*)P(A3[2]), cv->n, cv->L, order,
+                       cv->lpt_ws);
+  else
+    lpt_sources<double>(cv->ctx->c, (const cx<double>*)phi->d, (cx<double>*)phi2->d, (cx<double>*)P(phi3),
+                        (cx<double>*)P(A3[0]), (cx<double>*)P(A3[1]), (cx<double>*)P(A3[2]), cv->n, cv->L, order,
+                        cv->lpt_ws);
+}
+
+int mfx_lpt_sources(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3x, mfx_grid* A3y,
+                    mfx_grid* A3z, int order) {
+  return guard([&] {
+    MFX_REQUIRE(cv, "null convolver");
+    MFX_REQUIRE(order == 2 || order == 3, "order must be 2 or 3");
+    conv_check(cv, phi);
+    conv_check(cv, phi2);
+    mfx_grid* A3[3] = {A3x, A3y, A3z};
+    if (order > 2) {
+      conv_check(cv, phi3);
+      for (auto* g : A3) conv_check(cv, g);
+    }
+    lpt_fused(cv, phi, phi2, phi3, A3, order);
+  });
+}
+
+// ------------------------------------------------------------------ assembly
+int mfx_assemble_kspace(mfx_grid* tmp, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3x, mfx_grid* A3y,
+                        mfx_grid* A3z, int order, int kind, int idim, double vfac, double unit) {
+  return guard([&] {
+    require_grid(tmp);
+    require_grid(phi);
+    require_cubic(tmp);
+    same_shape(tmp, phi);
+    MFX_REQUIRE(order >= 1 && order <= 3, "order must be 1, 2 or 3");
+    MFX_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (displacement) or 1 (velocity)");
+    MFX_REQUIRE(idim >= 0 && idim < 3, "idim must be 0, 1 or 2");
+    mfx_grid* in[5] = {phi2, phi3, A3x, A3y, A3z};
+    const int nin = order == 1 ? 0 : order == 2 ? 1 : 5;
+    for (int q = 0; q < nin; ++q) {
+      require_grid(in[q]);
+      same_shape(tmp, in[q]);
+    }
+    grid_forward(phi, true);
+    for (int q = 0; q < nin; ++q) grid_forward(in[q], true);
+    tmp->space = MFX_KSPACE;
+    auto P = [&](mfx_grid* g, int q) { return (q < nin && g) ? g->d : phi->d; };
+    if (tmp->prec == MFX_FLOAT)
+      assemble<float>(tmp->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2, 0),
+                      (const cx<float>*)P(phi3, 1), (const cx<float>*)P(A3x, 2), (const cx<float>*)P(A3y, 3),
+                      (const cx<float>*)P(A3z, 4), (int)tmp->n[0], tmp->L[0], order, kind, idim, vfac, unit);
+    else
+      assemble<double>(tmp->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2, 0),
+                       (const cx<double>*)P(phi3, 1), (const cx<double>*)P(A3x, 2), (const cx<double>*)P(A3y, 3),
+                       (const cx<double>*)P(A3z, 4), (int)tmp->n[0], tmp->L[0], order, kind, idim, vfac, unit);
+  });
+}
+
+int mfx_set_particles(mfx_grid* field, double* out_dev, int idim, double lunit, int kind) {
+  return guard([&] {
+    require_grid(field);
+    require_cubic(field);
+    MFX_REQUIRE(out_dev, "null output array");
+    MFX_REQUIRE(field->space == MFX_RSPACE, "particle assembly reads a real-space field");
+    MFX_REQUIRE(idim >= 0 && idim < 3 && (kind == 0 || kind == 1), "invalid idim/kind");
+    if (field->prec == MFX_FLOAT)
+      set_particles<float>(field->ctx->c, (const float*)field->d, out_dev, (int)field->n[0], idim, lunit, kind);
+    else
+      set_particles<double>(field->ctx->c, (const double*)field->d, out_dev, (int)field->n[0], idim, lunit, kind);
+  });
+}
+
+// ------------------------------------------------------------------ RNG
+static bool is_pow2(size_t x) { return x && !(x & (x - 1)); }
+
+static void music1_impl(mfx_grid* g, long seed, int cubesize) {
+  require_cubic(g);
+  // random_music.cc:70-74 and 186-190
+  MFX_REQUIRE(is_pow2(g->n[0]), "MUSIC1 random number plugin requires GridRes to be a power of 2");
+  MFX_REQUIRE(seed > 0, "Seed values need to be >0");
+  MFX_REQUIRE(cubesize > 0, "cube size must be positive");
+  if (g->prec == MFX_FLOAT)
+    music1_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], seed, cubesize, 1);
+  else
+    music1_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], seed, cubesize, 1);
+  g->space = MFX_RSPACE;
+}
+
+int mfx_rng_music1_fill(mfx_grid* g, long seed, int cubesize) {
+  return guard([&] {
+    require_grid(g);
+    music1_impl(g, seed, cubesize);
+  });
+}
+
+static void ngenic_impl(mfx_grid* g, long seed) {
+  require_cubic(g);
+  const int n = (int)g->n[0];
+  std::vector<unsigned> table((size_t)n * n);
+  ngenic_seedtable(table.data(), n, seed);
+  Ctx& c = g->ctx->c;
+  unsigned* dt = (unsigned*)c.alloc(sizeof(unsigned) * table.size());
+  MFX_HIP(hipMemcpyAsync(dt, table.data(), sizeof(unsigned) * table.size(), hipMemcpyHostToDevice, c.stream));
+  if (g->prec == MFX_FLOAT)
+    ngenic_fill<float>(c, (cx<float>*)g->d, n, dt);
+  else
+    ngenic_fill<double>(c, (cx<double>*)g->d, n, dt);
+  MFX_HIP(hipStreamSynchronize(c.stream));
+  c.release(dt);
+  g->space = MFX_KSPACE;
+}
+
+int mfx_rng_ngenic_fill(mfx_grid* g, long seed) {
+  return guard([&] {
+    require_grid(g);
+    ngenic_impl(g, seed);
+  });
+}
+
+static void threefry_impl(mfx_grid* g, unsigned long long seed) {
+  require_cubic(g);
+  if (g->prec == MFX_FLOAT)
+    threefry_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], seed);
+  else
+    threefry_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], seed);
+  g->space = MFX_RSPACE;
+}
+
+int mfx_rng_threefry_fill(mfx_grid* g, unsigned long long seed) {
+  return guard([&] {
+    require_grid(g);
+    threefry_impl(g, seed);
+  });
+}
+
+int mfx_rng_mt19937_raw(mfx_ctx* c, const unsigned long* seeds, int nseeds, int count, unsigned* out) {
+  return guard([&] {
+    MFX_REQUIRE(c && seeds && out && nseeds > 0 && count > 0, "invalid argument");
+    Ctx& x = c->c;
+    unsigned long* ds = (unsigned long*)x.alloc(sizeof(unsigned long) * nseeds);
+    unsigned* dout = (unsigned*)x.alloc(sizeof(unsigned) * (size_t)nseeds * count);
+    MFX_HIP(hipMemcpyAsync(ds, seeds, sizeof(unsigned long) * nseeds, hipMemcpyHostToDevice, x.stream));
+    mt19937_uniforms(x, dout, ds, nseeds, count);
+    MFX_HIP(hipMemcpyAsync(out, dout, sizeof(unsigned) * (size_t)nseeds * count, hipMemcpyDeviceToHost, x.stream));
+    MFX_HIP(hipStreamSynchronize(x.stream));
+    x.release(ds);
+    x.release(dout);
+  });
+}
+
+int mfx_rng_ranlxd1_raw(mfx_ctx* c, const unsigned* seeds, int nseeds, int count, double* out) {
+  return guard([&] {
+    MFX_REQUIRE(c && seeds && out && nseeds > 0 && count > 0, "invalid argument");
+    Ctx& x = c->c;
+    const int np = (nseeds + 63) / 64 * 64;
+    std::vector<unsigned> s(np, 1u);
+    std::memcpy(s.data(), seeds, sizeof(unsigned) * nseeds);
+    unsigned* ds = (unsigned*)x.alloc(sizeof(unsigned) * np);
+    double* dout = (double*)x.alloc(sizeof(double) * (size_t)np * count);
+    MFX_HIP(hipMemcpyAsync(ds, s.data(), sizeof(unsigned) * np, hipMemcpyHostToDevice, x.stream));
+    ranlxd_uniforms(x, dout, ds, np, count);
+    MFX_HIP(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)nseeds * count, hipMemcpyDeviceToHost, x.stream));
+    MFX_HIP(hipStreamSynchronize(x.stream));
+    x.release(ds);
+    x.release(dout);
+  });
+}
+
+int mfx_rng_threefry_uniforms(mfx_ctx* c, const unsigned long long* cells, int ncells, unsigned long long seed,
+                              int count, double* out) {
+  return guard([&] {
+    MFX_REQUIRE(c && cells && out && ncells > 0 && count > 0, "invalid argument");
+    MFX_REQUIRE((seed >> 32) == 0, "Threefry seed must satisfy 0 <= seed < 2^32");
+    for (int i = 0; i < ncells; ++i) {
+      const unsigned long long l = cells[4 * i];
+      MFX_REQUIRE(l < 64, "level out of range (0-63)");
+      for (int d = 1; d < 4; ++d) MFX_REQUIRE((cells[4 * i + d] >> l) == 0, "cell coordinate out of range");
+    }
+    Ctx& x = c->c;
+    unsigned long long* dc = (unsigned long long*)x.alloc(sizeof(unsigned long long) * 4 * ncells);
+    double* dout = (double*)x.alloc(sizeof(double) * (size_t)ncells * count);
+    MFX_HIP(hipMemcpyAsync(dc, cells, sizeof(unsigned long long) * 4 * ncells, hipMemcpyHostToDevice, x.stream));
+    threefry_uniforms(x, dout, ncells, dc, seed, count);
+    MFX_HIP(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)ncells * count, hipMemcpyDeviceToHost, x.stream));
+    MFX_HIP(hipStreamSynchronize(x.stream));
+    x.release(dc);
+    x.release(dout);
+  });
+}
+
+// ------------------------------------------------------------------ cosmology
+int mfx_cosmo_create(double H0, double Omega_m, double Omega_b, double n_s, double sigma_8, double Tcmb,
+                     int zero_radiation, mfx_cosmo** out) {
+  return guard([&] {
+    MFX_REQUIRE(out, "null output pointer");
+    auto* c = new mfx_cosmo();
+    try {
+      c->c.reset(new Cosmology(H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb, zero_radiation != 0));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int mfx_cosmo_destroy(mfx_cosmo* c) {
+  return guard([&] { delete c; });
+}
+
+int mfx_cosmo_query(const mfx_cosmo* cc, int what, double x, double* out) {
+  return guard([&] {
+    MFX_REQUIRE(cc && out, "null argument");
+    const Cosmology& c = *cc->c;
+    switch (what) {
+      case MFX_COSMO_TRANSFER: *out = c.transfer(x); break;
+      case MFX_COSMO_DPLUS: *out = c.Dplus(x); break;
+      case MFX_COSMO_F: *out = c.f(x); break;
+      case MFX_COSMO_VFACT: *out = c.vfact(x); break;
+      case MFX_COSMO_H: *out = c.H(x); break;
+      case MFX_COSMO_AMPLITUDE: *out = c.amplitude(x); break;
+      case MFX_COSMO_PNORM: *out = c.pnorm(); break;
+      default: throw Error(MFX_EINVAL, "unknown cosmology query");
+    }
+  });
+}
+
+static void amp_table(const Cosmology& c, size_t n, double boxlen, double* out, size_t ntable) {
+  const double kfac = 2.0 * M_PI / boxlen;
+  out[0] = 0.0;
+  for (size_t n2 = 1; n2 < ntable; ++n2) out[n2] = c.amplitude(kfac * std::sqrt((double)n2));
+}
+
+int mfx_cosmo_amplitude_table(const mfx_cosmo* cc, size_t n, double boxlen, double* out, size_t ntable) {
+  return guard([&] {
+    MFX_REQUIRE(cc && out, "null argument");
+    MFX_REQUIRE(ntable >= 3 * (n / 2) * (n / 2) + 1, "amplitude table too short");
+    amp_table(*cc->c, n, boxlen, out, ntable);
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ LPT driver
+struct mfx_icgen {
+  mfx_ctx* ctx;
+  mfx_icgen_params p;
+  std::unique_ptr<Cosmology> cosmo;
+  mfx_grid* g[8] = {};  // wnoise, phi, phi2, phi3, A3x, A3y, A3z, tmp
+  mfx_conv* conv = nullptr;
+  double* part[6] = {};
+  double* amp = nullptr;
+  double dplus = 0, vfact = 0;
+};
+
+static void icgen_free(mfx_icgen* ic) {
+  for (auto*& gg : ic->g)
+    if (gg) {
+      mfx_grid_destroy(gg);
+      gg = nullptr;
+    }
+  if (ic->conv) mfx_conv_destroy(ic->conv);
+  for (auto*& p : ic->part)
+    if (p) {
+      ic->ctx->c.release(p);
+      p = nullptr;
+    }
+  if (ic->amp) ic->ctx->c.release(ic->amp);
+}
+
+extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen** out) {
+  return guard([&] {
+    MFX_REQUIRE(c && p && out, "null argument");
+    MFX_REQUIRE(p->lpt_order >= 1 && p->lpt_order <= 3, "LPTorder must be 1, 2 or 3");
+    // ic_generator.cc:118-122
+    MFX_REQUIRE(!(p->n % 2 != 0 && p->lpt_order > 1), "Orszag convolver for LPTorder>1 requires even grid size!");
+    MFX_REQUIRE(p->generator >= 0 && p->generator <= 2, "generator must be 0 (MUSIC1), 1 (NGENIC) or 2 (Threefry)");
+    auto* ic = new mfx_icgen();
+    ic->ctx = c;
+    ic->p = *p;
+    try {
+      ic->cosmo.reset(new Cosmology(p->H0, p->Omega_m, p->Omega_b, p->n_s, p->sigma_8, p->Tcmb, p->zero_radiation != 0));
+      const double astart = 1.0 / (1.0 + p->zstart);
+      ic->dplus = ic->cosmo->Dplus(astart);
+      ic->vfact = ic->cosmo->vfact(astart);
+      const int nq = p->lpt_order == 1 ? 2 : p->lpt_order == 2 ? 3 : 7;
+      for (int q = 0; q < 8; ++q) {
+        if (q >= nq && q != 7) continue;
+        int st = mfx_grid_create(c, p->n, p->n, p->n, p->boxlen, p->boxlen, p->boxlen, p->precision, MFX_RSPACE,
+                                 &ic->g[q]);
+        if (st) throw Error(st, g_err);
+      }
+      if (p->lpt_order > 1) {
+        int st = mfx_conv_create(c, p->n, p->boxlen, p->precision, &ic->conv);
+        if (st) throw Error(st, g_err);
+      }
+      for (auto*& pp : ic->part) pp = (double*)c->c.alloc(sizeof(double) * p->n * p->n * p->n);
+      const size_t nt = 3 * (p->n / 2) * (p->n / 2) + 1;
+      std::vector<double> t(nt);
+      amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);
+      ic->amp = (double*)c->c.alloc(sizeof(double) * nt);
+      MFX_HIP(hipMemcpy(ic->amp, t.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+      // twiddle tables are created before the first timed run
+      if (p->precision == MFX_FLOAT) {
+        c->c.twiddles<float>((int)p->n);
+        c->c.twiddles<float>((int)p->n / 2);
+      } else {
+        c->c.twiddles<double>((int)p->n);
+        c->c.twiddles<double>((int)p->n / 2);
+      }
+    } catch (...) {
+      icgen_free(ic);
+      delete ic;
+      throw;
+    }
+    *out = ic;
+  });
+}
+
+extern "C" int mfx_icgen_destroy(mfx_icgen* ic) {
+  return guard([&] {
+    if (!ic) return;
+    MFX_HIP(hipStreamSynchronize(ic->ctx->c.stream));
+    icgen_free(ic);
+    delete ic;
+  });
+}
+
+extern "C" int mfx_icgen_run(mfx_icgen* ic) {
+  return guard([&] {
+    MFX_REQUIRE(ic, "null generator");
+    const mfx_icgen_params& p = ic->p;
+    mfx_grid *wn = ic->g[0], *phi = ic->g[1], *phi2 = ic->g[2], *phi3 = ic->g[3], *tmp = ic->g[7];
+    mfx_grid* A3[3] = {ic->g[4], ic->g[5], ic->g[6]};
+    const double D = ic->dplus;
+    const double g1 = -D, g2 = -3.0 / 7.0 * D * D, g3 = 1.0 / 3.0 * D * D * D, g3c = 1.0 / 7.0 * D * D * D;
+    const double volfac = std::pow(p.boxlen / (double)p.n / 2.0 / M_PI, 1.5);
+    // white noise (ic_generator.cc:249-257)
+    if (p.generator == 0) {
+      music1_impl(wn, p.seed, 32);
+    } else if (p.generator == 1) {
+      ngenic_impl(wn, p.seed);
+    } else {
+      threefry_impl(wn, (unsigned long long)p.seed);
+    }
+    grid_forward(wn, true);
+    // normalisation (341-346) and phi(1) (450-457)
+    if (wn->prec == MFX_FLOAT)
+      noise_normalise<float>(ic->ctx->c, (cx<float>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
+    else
+      noise_normalise<double>(ic->ctx->c, (cx<double>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
+    phi->space = MFX_KSPACE;
+    phi1_impl(phi, wn, ic->amp);
+    // phi(2), phi(3), A(3)
+    if (p.lpt_order > 1) {
+      if (p.use_fused_lpt)
+        lpt_fused(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
+      else
+        lpt_sequence(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
+    }
+    // growth factors (550-563)
+    auto sc = [&](mfx_grid* g, double s) {
+      if (g->prec == MFX_FLOAT)
+        scale<float>(ic->ctx->c, (float*)g->d, (long long)g->ntot, s);
+      else
+        scale<double>(ic->ctx->c, (double*)g->d, (long long)g->ntot, s);
+    };
+    sc(phi, g1);
+    if (p.lpt_order > 1) sc(phi2, g2);
+    if (p.lpt_order > 2) {
+      sc(phi3, g3);
+      for (auto* a : A3) sc(a, g3c);
+    }
+    // positions and velocities (821-931), lunit = vunit = 1
+    for (int kind = 0; kind < 2; ++kind)
+      for (int idim = 0; idim < 3; ++idim) {
+        tmp->space = MFX_KSPACE;
+        auto P = [](mfx_grid* g) { return g ? g->d : nullptr; };
+        if (tmp->prec == MFX_FLOAT)
+          assemble<float>(ic->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2),
+                          (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
+                          (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact, 1.0);
+        else
+          assemble<double>(ic->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2),
+                           (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]), (const cx<double>*)P(A3[1]),
+                           (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact,
+                           1.0);
+        grid_backward(tmp, true);
+        if (tmp->prec == MFX_FLOAT)
+          set_particles<float>(ic->ctx->c, (const float*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0, kind);
+        else
+          set_particles<double>(ic->ctx->c, (const double*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0,
+                                kind);
+      }
+  });
+}
+
+extern "C" int mfx_icgen_particles(mfx_icgen* ic, int what, double** dev_ptr) {
+  return guard([&] {
+    MFX_REQUIRE(ic && dev_ptr && what >= 0 && what < 6, "invalid argument");
+    *dev_ptr = ic->part[what];
+  });
+}
+
+extern "C" int mfx_icgen_grid(mfx_icgen* ic, int which, mfx_grid** g) {
+  return guard([&] {
+    MFX_REQUIRE(ic && g && which >= 0 && which < 8, "invalid argument");
+    MFX_REQUIRE(ic->g[which] != nullptr, "grid not allocated at this LPT order");
+    *g = ic->g[which];
+  });
+}
+
+extern "C" int mfx_icgen_scalars(mfx_icgen* ic, double* dplus, double* vfact) {
+  return guard([&] {
+    MFX_REQUIRE(ic, "null generator");
+    if (dplus) *dplus = ic->dplus;
+    if (vfact) *vfact = ic->vfact;
+  });
+}

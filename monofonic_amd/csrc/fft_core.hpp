@@ -1,0 +1,203 @@
+// fft_core.hpp -- register/LDS Stockham FFT building blocks for gfx950.
+//
+// A sequence of length L is transformed by TPS = L/EPT threads; thread t
+// holds elements t + TPS*e (e < EPT) in registers before and after every
+// stage.  Each radix-R stage performs EPT/R butterflies per thread, writes its
+// outputs to LDS in Stockham (self-sorting) order and reads back the next
+// stage's inputs, so the result is in natural order without a bit-reversal
+// pass.  Because thread t always owns positions t + TPS*e, a row whose
+// elements are contiguous in HBM is loaded coalesced by consecutive threads,
+// and a tile of C adjacent columns is loaded coalesced by consecutive column
+// lanes: no separate transpose kernel is needed for either axis.
+//
+// Twiddles come from a per-length table w[n] = exp(-2 pi i n / L) computed in
+// FP64 on the host and rounded once to T.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mfx {
+
+template <typename T>
+struct cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) {
+  cx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, T s) { return mk(a.x * s, a.y * s); }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * w (forward) or a * conj(w) (inverse)
+template <bool INV, typename T>
+__device__ __forceinline__ cx<T> twmul(cx<T> a, cx<T> w) {
+  if (INV) return mk(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
+  return mk(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) { return mk(a.x, -a.y); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename T>
+__device__ __forceinline__ cx<T> mul_mi(cx<T> a) {
+  if (INV) return mk(-a.y, a.x);
+  return mk(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- butterflies
+template <int R, bool INV, typename T>
+struct BF;
+
+template <bool INV, typename T>
+struct BF<2, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    cx<T> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+  }
+};
+
+template <bool INV, typename T>
+struct BF<3, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    const T s3 = T(0.866025403784438646763723170752936183);
+    cx<T> t1 = a[1] + a[2];
+    cx<T> t2 = a[0] - t1 * T(0.5);
+    cx<T> t3 = (a[1] - a[2]) * s3;
+    a[0] = a[0] + t1;
+    cx<T> m = mul_mi<INV>(t3);  // -i t3 forward
+    a[1] = t2 + m;
+    a[2] = t2 - m;
+  }
+};
+
+template <bool INV, typename T>
+struct BF<4, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    cx<T> t0 = a[0] + a[2], t1 = a[0] - a[2], t2 = a[1] + a[3], t3 = mul_mi<INV>(a[1] - a[3]);
+    a[0] = t0 + t2;
+    a[2] = t0 - t2;
+    a[1] = t1 + t3;
+    a[3] = t1 - t3;
+  }
+};
+
+template <bool INV, typename T>
+struct BF<8, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    const T r2 = T(0.707106781186547524400844362104849039);
+    cx<T> e[4] = {a[0], a[2], a[4], a[6]};
+    cx<T> o[4] = {a[1], a[3], a[5], a[7]};
+    BF<4, INV, T>::run(e);
+    BF<4, INV, T>::run(o);
+    // o[k] *= w8^k, w8 = exp(-+ 2 pi i / 8)
+    o[1] = twmul<INV>(o[1], mk(r2, -r2));
+    o[2] = mul_mi<INV>(o[2]);
+    o[3] = twmul<INV>(o[3], mk(-r2, -r2));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      a[k] = e[k] + o[k];
+      a[k + 4] = e[k] - o[k];
+    }
+  }
+};
+
+// ---------------------------------------------------------------- plans
+// Radix schedule per length: EPT must be a multiple of every radix.
+template <int L>
+struct Plan;
+#define MFX_PLAN(L_, EPT_, ...)                            \
+  template <>                                              \
+  struct Plan<L_> {                                        \
+    static constexpr int EPT = EPT_;                       \
+    static constexpr int R[] = {__VA_ARGS__};              \
+    static constexpr int NR = sizeof(R) / sizeof(int);     \
+  };
+
+MFX_PLAN(8, 8, 8)
+MFX_PLAN(12, 12, 3, 4)
+MFX_PLAN(16, 4, 4, 4)
+MFX_PLAN(24, 12, 3, 2, 4)
+MFX_PLAN(32, 8, 4, 8)
+MFX_PLAN(36, 12, 3, 3, 4)
+MFX_PLAN(48, 12, 3, 4, 4)
+MFX_PLAN(64, 8, 8, 8)
+MFX_PLAN(72, 12, 3, 3, 2, 4)
+MFX_PLAN(96, 24, 3, 4, 8)
+MFX_PLAN(128, 8, 8, 4, 4)
+MFX_PLAN(144, 12, 3, 3, 4, 4)
+MFX_PLAN(192, 24, 3, 8, 8)
+MFX_PLAN(256, 8, 8, 8, 4)
+MFX_PLAN(288, 24, 3, 3, 4, 8)
+MFX_PLAN(384, 24, 3, 8, 4, 4)
+MFX_PLAN(512, 8, 8, 8, 8)
+MFX_PLAN(768, 24, 3, 8, 8, 4)
+MFX_PLAN(1024, 8, 8, 8, 4, 4)
+MFX_PLAN(1536, 24, 3, 8, 8, 8)
+#undef MFX_PLAN
+
+template <int L>
+struct PlanInfo {
+  static constexpr int EPT = Plan<L>::EPT;
+  static constexpr int TPS = L / EPT;
+};
+
+// one Stockham stage: radix R, NS = product of the radices already applied
+template <typename T, int L, int EPT, int R, int NS, bool INV>
+__device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ lds, int ls, int t,
+                                          const cx<T>* __restrict__ tw) {
+  constexpr int TPS = L / EPT;
+  constexpr int NB = EPT / R;
+  static_assert(EPT % R == 0, "radix must divide EPT");
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    cx<T> a[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) a[q] = v[m + q * NB];
+    const int j = t + TPS * m;
+    const int k = j % NS;
+    if (NS > 1) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], tw[q * k * (L / (NS * R))]);
+    }
+    BF<R, INV, T>::run(a);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) lds[(base + q * NS) * ls] = a[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) v[e] = lds[(t + TPS * e) * ls];
+  __syncthreads();
+}
+
+template <typename T, int L, bool INV, int S, int NS>
+__device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t,
+                                           const cx<T>* tw) {
+  if constexpr (S < Plan<L>::NR) {
+    constexpr int R = Plan<L>::R[S];
+    fft_stage<T, L, Plan<L>::EPT, R, NS, INV>(v, lds, ls, t, tw);
+    stage_loop<T, L, INV, S + 1, NS * R>(v, lds, ls, t, tw);
+  }
+}
+
+// Full length-L transform of the registers v (positions t + TPS*e).
+// lds points at this sequence's L-element exchange area with element stride ls.
+// All threads of the block must call it (it contains barriers).
+template <typename T, int L, bool INV>
+__device__ __forceinline__ void fft_exec(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t,
+                                         const cx<T>* __restrict__ tw) {
+  stage_loop<T, L, INV, 0, 1>(v, lds, ls, t, tw);
+}
+
+}  // namespace mfx

@@ -1,0 +1,111 @@
+// mfx_internal.hpp -- host-side context, error handling, profiling and the
+// internal C++ launch interface shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "fft_core.hpp"
+
+namespace mfx {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MFX_HIP(call)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw ::mfx::Error(2, std::string(#call) + " failed: " + hipGetErrorString(e_));        \
+  } while (0)
+
+#define MFX_REQUIRE(cond, msg)                         \
+  do {                                                 \
+    if (!(cond)) throw ::mfx::Error(1, (msg));         \
+  } while (0)
+
+// one profiled kernel launch
+struct ProfRec {
+  std::string name;
+  double bytes;
+  hipEvent_t e0, e1;
+};
+
+struct KStat {
+  int count = 0;
+  double ms = 0.0;
+  double bytes = 0.0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<std::pair<int, int>, void*> tw;  // (length, sizeof(T)) -> exp(-2 pi i n/L) table
+  bool profile = false;
+  std::vector<ProfRec> pending;
+  std::map<std::string, KStat> stats;
+  ~Ctx();
+  template <typename T>
+  const cx<T>* twiddles(int L);
+  void* alloc(size_t bytes);
+  void release(void* p);
+};
+
+// Records events around a launch when profiling is on.  `bytes` is the
+// algorithmic HBM traffic of the launch (one read of every input element the
+// kernel needs plus one write of every output element).
+struct LaunchScope {
+  Ctx& c;
+  ProfRec rec;
+  bool on;
+  LaunchScope(Ctx& ctx, const char* name, double bytes) : c(ctx), on(ctx.profile) {
+    if (on) {
+      rec.name = name;
+      rec.bytes = bytes;
+      MFX_HIP(hipEventCreate(&rec.e0));
+      MFX_HIP(hipEventCreate(&rec.e1));
+      MFX_HIP(hipEventRecord(rec.e0, c.stream));
+    }
+  }
+  ~LaunchScope() {
+    if (on) {
+      (void)hipEventRecord(rec.e1, c.stream);
+      c.pending.push_back(rec);
+    }
+  }
+};
+
+void check_launch(const char* what);
+
+// ------------------------------------------------------------------ internal API
+enum { OP_ASSIGN = 0, OP_ADD = 1, OP_SUB = 2, OP_MULADD = 3 };
+
+// Fourier multiplier: sum over terms of coef * prod_d f_d(i_d)^pow_d where
+// f_d = get_k (nyq == 0) or the gradient wave number zeroed at Nyquist (nyq == 1)
+struct MultSpec {
+  int nterms;
+  double coef_re[2], coef_im[2];
+  int pow[2][3];
+  int nyq[2];
+};
+
+template <typename T>
+void fft_r2c_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale);
+template <typename T>
+void fft_c2r_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale);
+
+// Orszag 3/2 product of two n-grid k-space fields with multipliers, written
+// through op into out (k space n-grid).  ws must hold conv_workspace_bytes.
+template <typename T>
+void conv2(Ctx& c, const cx<T>* fa, const MultSpec& ma, const cx<T>* fb, const MultSpec& mb,
+           cx<T>* out, int op, double opfac, int n, double boxlen, void* ws);
+size_t conv_workspace_bytes(int n, int prec);
+int conv_pitch(int n);
+
+}  // namespace mfx

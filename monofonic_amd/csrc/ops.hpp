@@ -1,0 +1,51 @@
+// ops.hpp -- launchers for the per-mode operators, assembly, reductions,
+// white-noise generators and the fused LPT source engine.
+#pragma once
+#include "mfx_internal.hpp"
+
+namespace mfx {
+
+template <typename T>
+void noise_normalise(Ctx& c, cx<T>* wn, int n, double volfac, int fixing, int inversion);
+template <typename T>
+void phi1(Ctx& c, cx<T>* phi, const cx<T>* wn, int n, double boxlen, const double* amp);
+template <typename T>
+void inverse_laplacian(Ctx& c, cx<T>* f, int n, double boxlen);
+template <typename T>
+void scale(Ctx& c, T* f, long long cnt, double s);
+template <typename T>
+void fill(Ctx& c, T* f, long long cnt, T v);
+template <typename T>
+void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
+              const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, int kind, int idim, double vfac,
+              double unit);
+template <typename T>
+void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind);
+template <typename T>
+void add_const(Ctx& c, T* f, int n0, int n1, int n2, long long pitch, double v);
+template <typename T>
+void sum2(Ctx& c, const T* f, int n0, int n1, int n2, long long pitch, double* out2);
+template <typename T>
+int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, double* bP, double* beP, double* bc,
+                   int maxbins);
+
+// white noise
+template <typename T>
+void music1_fill(Ctx& c, T* data, int n, long baseseed, int cubesize, int zeromean);
+template <typename T>
+void threefry_fill(Ctx& c, T* data, int n, unsigned long long seed);
+void threefry_uniforms(Ctx& c, double* out, int ncells, const unsigned long long* cells /*l,j1,j2,j3*/,
+                       unsigned long long seed, int count);
+template <typename T>
+void ngenic_fill(Ctx& c, cx<T>* data, int n, const unsigned int* seedtable_dev);
+void ranlxd_uniforms(Ctx& c, double* out, const unsigned int* seeds, int nseeds, int count);
+void mt19937_uniforms(Ctx& c, unsigned int* out, const unsigned long* seeds, int nseeds, int count);
+void ngenic_seedtable(unsigned* table, int n, long seed);
+
+// fused LPT source engine
+size_t lpt_workspace_bytes(int n, int prec, int order);
+template <typename T>
+void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x, cx<T>* a3y, cx<T>* a3z, int n,
+                 double boxlen, int order, void* ws);
+
+}  // namespace mfx

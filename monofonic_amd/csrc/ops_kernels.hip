@@ -1,0 +1,369 @@
+// ops_kernels.hip -- per-mode k-space operators, LPT assembly, reductions
+// and the on-device white-noise generators for gfx950.
+//
+// All k-space kernels walk the reference's k layout kelem(i,j,k) =
+// cdata[(i*n + j)*(n/2+1) + k] with one thread per complex mode; wave-number
+// helpers restate get_k / gradient (include/grid_fft.hh:282-296, 376-385).
+// Scalar factors that the reference keeps in FP64 (amplitude table, growth
+// and velocity factors, position assembly) are applied in double regardless of
+// the storage precision T of the bulk fields.
+#include "ops.hpp"
+
+#include <cmath>
+
+namespace mfx {
+
+__device__ __forceinline__ double kd_of(int i, int n, double kfac) {
+  return ((double)i - (double)(i > (n >> 1)) * (double)n) * kfac;
+}
+__device__ __forceinline__ double gd_of(int i, int n, double kfac) {
+  return (i != (n >> 1)) ? ((double)i - (double)(i > (n >> 1)) * (double)n) * kfac : 0.0;
+}
+
+// ------------------------------------------------------------------ noise normalisation
+// ic_generator.cc:341-346: wn <- (inv ? -1 : 1) * [fixing ? wn/|wn| : wn] / volfac
+template <typename T>
+__global__ void k_noise_norm(cx<T>* wn, long long cnt, double ivolfac, int fixing, double sgn) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cnt) return;
+  cx<T> v = wn[idx];
+  double re = v.x, im = v.y;
+  if (fixing) {
+    double a = hypot(re, im);
+    if (a != 0.0) {
+      re /= a;
+      im /= a;
+    }
+  }
+  wn[idx] = mk(T(sgn * re * ivolfac), T(sgn * im * ivolfac));
+}
+
+// ic_generator.cc:450-457: phi = -wn * A(|k|) / |k|^2, zero_DC_mode.
+// A is tabulated (FP64) on the integer |n|^2 = ix^2+iy^2+iz^2.
+template <typename T>
+__global__ void k_phi1(cx<T>* phi, const cx<T>* wn, int n, double kfac, const double* __restrict__ amp) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  int k = (int)(idx % nh);
+  long long ij = idx / nh;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  int ix = i - (i > n / 2) * n, iy = j - (j > n / 2) * n, iz = k - (k > n / 2) * n;
+  double kx = kd_of(i, n, kfac), ky = kd_of(j, n, kfac), kz = kd_of(k, n, kfac);
+  double k2 = kx * kx + ky * ky + kz * kz;
+  long long n2 = (long long)ix * ix + (long long)iy * iy + (long long)iz * iz;
+  cx<T> w = wn[idx];
+  double f = (idx == 0) ? 0.0 : amp[n2] / k2;
+  phi[idx] = mk(T(-(double)w.x * f), T(-(double)w.y * f));
+}
+
+// grid_fft.hh:440-449
+template <typename T>
+__global__ void k_invlap(cx<T>* f, int n, T kfac) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  int k = (int)(idx % nh);
+  long long ij = idx / nh;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  T kx = (T(i) - T(i > n / 2) * T(n)) * kfac, ky = (T(j) - T(j > n / 2) * T(n)) * kfac,
+    kz = (T(k) - T(k > n / 2) * T(n)) * kfac;
+  T k2 = kx * kx + ky * ky + kz * kz;
+  cx<T> v = f[idx];
+  f[idx] = (idx == 0) ? mk(T(0), T(0)) : mk(-v.x / k2, -v.y / k2);
+}
+
+template <typename T>
+__global__ void k_scale(T* f, long long cnt, double s) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < cnt) f[idx] = T((double)f[idx] * s);
+}
+
+template <typename T>
+__global__ void k_fill(T* f, long long cnt, T v) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < cnt) f[idx] = v;
+}
+
+// ------------------------------------------------------------------ assembly
+// ic_generator.cc:821-860 (kind 0, displacement) and 877-919 (kind 1,
+// velocity) with op::fourier_gradient (operators.hh:38-61, vfac_corr = 1),
+// followed by zero_DC_mode.  Potentials are already growth-scaled.
+template <typename T>
+__global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
+                           const cx<T>* a3y, const cx<T>* a3z, int n, double k0, int order, int kind, int idim,
+                           double vfac, double unit_over_L) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  int k = (int)(idx % nh);
+  long long ij = idx / nh;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  const int ijk[3] = {i, j, k};
+  double g[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) g[d] = gd_of(ijk[d], n, 1.0) * k0;
+  double pr, pi;
+  if (kind == 0) {
+    pr = phi[idx].x;
+    pi = phi[idx].y;
+    if (order > 1) {
+      pr += phi2[idx].x;
+      pi += phi2[idx].y;
+    }
+    if (order > 2) {
+      pr += phi3[idx].x;
+      pi += phi3[idx].y;
+    }
+  } else {
+    pr = vfac * phi[idx].x;
+    pi = vfac * phi[idx].y;
+    if (order > 1) {
+      pr += 2 * vfac * phi2[idx].x;
+      pi += 2 * vfac * phi2[idx].y;
+    }
+    if (order > 2) {
+      pr += 3 * vfac * phi3[idx].x;
+      pi += 3 * vfac * phi3[idx].y;
+    }
+  }
+  const int idp = (idim + 1) % 3, idpp = (idim + 2) % 3;
+  double tr = -g[idim] * pi, ti = g[idim] * pr;
+  if (order > 2) {
+    const cx<T>* A3[3] = {a3x, a3y, a3z};
+    cx<T> a2 = A3[idpp][idx], a1 = A3[idp][idx];
+    double cr = (-g[idp] * (double)a2.y) - (-g[idpp] * (double)a1.y);
+    double ci = (g[idp] * (double)a2.x) - (g[idpp] * (double)a1.x);
+    if (kind == 1) {
+      cr *= 3 * vfac;
+      ci *= 3 * vfac;
+    }
+    tr += cr;
+    ti += ci;
+  }
+  tr *= unit_over_L;
+  ti *= unit_over_L;
+  tmp[idx] = (idx == 0) ? mk(T(0), T(0)) : mk(T(tr), T(ti));
+}
+
+// particle_generator.hh:525-541 (sc lattice): pos = q_d * lunit + field, FP64;
+// velocities (:622-632) copy the field.  kind 0 -> positions, 1 -> velocities
+template <typename T>
+__global__ void k_set_particles(const T* field, double* out, int n, int idim, double lunit, int kind) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long long)n * n * n) return;
+  int k = (int)(p % n);
+  long long ij = p / n;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  double f = (double)field[ij * (n + 2) + k];
+  if (kind == 0) {
+    const int ijk[3] = {i, j, k};
+    double q = ((double)ijk[idim] + 0.0) / (double)n;
+    out[p] = q * lunit + f;
+  } else {
+    out[p] = f;
+  }
+}
+
+template <typename T>
+__global__ void k_add_const(T* f, int n0, int n1, int n2, long long pitch, double v) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long long)n0 * n1 * n2) return;
+  long long r = p / n2;
+  int k = (int)(p % n2);
+  f[r * pitch + k] = T((double)f[r * pitch + k] + v);
+}
+
+// ------------------------------------------------------------------ reductions
+// sum and sum of squares (double) of the real-space field (grid_fft.hh:507-595)
+template <typename T>
+__global__ void k_sum2(const T* f, int n0, int n1, int n2, long long pitch, double* part) {
+  __shared__ double s1[256], s2[256];
+  double a = 0, b = 0;
+  const long long cnt = (long long)n0 * n1 * n2;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += (long long)gridDim.x * blockDim.x) {
+    long long r = p / n2;
+    int k = (int)(p % n2);
+    double v = (double)f[r * pitch + k];
+    a += v;
+    b += v * v;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
+// power spectrum bins (grid_fft.cc:1133-1210): per-bin sums of |z|^2, |z|^4, k, count
+template <typename T>
+__global__ void k_pk(const cx<T>* f, int n, double kfac, int nbins, double* bins) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  int k = (int)(idx % nh);
+  long long ij = idx / nh;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  double kx = kd_of(i, n, kfac), ky = kd_of(j, n, kfac), kz = kd_of(k, n, kfac);
+  double kk = sqrt(kx * kx + ky * ky + kz * kz);
+  const double kmin = kfac, kmax = kfac * (double)(n / 2);
+  if (!(kk >= kmin && kk < kmax)) return;
+  int b = (int)(kk / kmin);
+  if (b >= nbins) return;
+  cx<T> z = f[idx];
+  double v = (double)z.x * z.x + (double)z.y * z.y;
+  double w = (k == 0) ? 1.0 : 2.0;
+  atomicAdd(&bins[4 * b + 0], w * v);
+  atomicAdd(&bins[4 * b + 1], w * v * v);
+  atomicAdd(&bins[4 * b + 2], w * kk);
+  atomicAdd(&bins[4 * b + 3], w);
+}
+
+// ------------------------------------------------------------------ launchers
+static inline unsigned nblk(long long cnt, int bs) { return (unsigned)((cnt + bs - 1) / bs); }
+
+template <typename T>
+void noise_normalise(Ctx& c, cx<T>* wn, int n, double volfac, int fixing, int inversion) {
+  long long cnt = (long long)n * n * (n / 2 + 1);
+  LaunchScope ls(c, "noise_normalise", 2.0 * cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_noise_norm<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, wn, cnt, 1.0 / volfac, fixing,
+                     inversion ? -1.0 : 1.0);
+  check_launch("noise_normalise");
+}
+
+template <typename T>
+void phi1(Ctx& c, cx<T>* phi, const cx<T>* wn, int n, double boxlen, const double* amp) {
+  long long cnt = (long long)n * n * (n / 2 + 1);
+  LaunchScope ls(c, "phi1", 2.0 * cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_phi1<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, phi, wn, n, 2.0 * M_PI / boxlen, amp);
+  check_launch("phi1");
+}
+
+template <typename T>
+void inverse_laplacian(Ctx& c, cx<T>* f, int n, double boxlen) {
+  long long cnt = (long long)n * n * (n / 2 + 1);
+  LaunchScope ls(c, "inverse_laplacian", 2.0 * cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_invlap<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, f, n, T(2.0 * M_PI / boxlen));
+  check_launch("inverse_laplacian");
+}
+
+template <typename T>
+void scale(Ctx& c, T* f, long long cnt, double s) {
+  LaunchScope ls(c, "scale", 2.0 * cnt * sizeof(T));
+  hipLaunchKernelGGL(k_scale<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, f, cnt, s);
+  check_launch("scale");
+}
+
+template <typename T>
+void fill(Ctx& c, T* f, long long cnt, T v) {
+  LaunchScope ls(c, "fill", 1.0 * cnt * sizeof(T));
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, f, cnt, v);
+  check_launch("fill");
+}
+
+template <typename T>
+void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
+              const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, int kind, int idim, double vfac,
+              double unit) {
+  long long cnt = (long long)n * n * (n / 2 + 1);
+  int nin = order == 1 ? 1 : order == 2 ? 2 : 5;
+  LaunchScope ls(c, "assemble_kspace", (nin + 1.0) * cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_assemble<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, tmp, phi, phi2, phi3, a3x, a3y, a3z,
+                     n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen);
+  check_launch("assemble");
+}
+
+template <typename T>
+void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind) {
+  long long cnt = (long long)n * n * n;
+  LaunchScope ls(c, "set_particles", (double)cnt * (sizeof(T) + sizeof(double)));
+  hipLaunchKernelGGL(k_set_particles<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, field, out, n, idim, lunit,
+                     kind);
+  check_launch("set_particles");
+}
+
+template <typename T>
+void add_const(Ctx& c, T* f, int n0, int n1, int n2, long long pitch, double v) {
+  long long cnt = (long long)n0 * n1 * n2;
+  hipLaunchKernelGGL(k_add_const<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, f, n0, n1, n2, pitch, v);
+  check_launch("add_const");
+}
+
+template <typename T>
+void sum2(Ctx& c, const T* f, int n0, int n1, int n2, long long pitch, double* out2) {
+  const int nb = 1024;
+  double* part = (double*)c.alloc(sizeof(double) * 2 * nb);
+  hipLaunchKernelGGL(k_sum2<T>, dim3(nb), dim3(256), 0, c.stream, f, n0, n1, n2, pitch, part);
+  check_launch("sum2");
+  std::vector<double> h(2 * nb);
+  MFX_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c.stream));
+  MFX_HIP(hipStreamSynchronize(c.stream));
+  c.release(part);
+  double a = 0, b = 0;
+  for (int i = 0; i < nb; ++i) {
+    a += h[2 * i];
+    b += h[2 * i + 1];
+  }
+  out2[0] = a;
+  out2[1] = b;
+}
+
+template <typename T>
+int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, double* bP, double* beP, double* bc,
+                   int maxbins) {
+  const double kfac = 2.0 * M_PI / boxlen;
+  const int nbins = (int)((kfac * (double)(n / 2)) / kfac);
+  MFX_REQUIRE(nbins <= maxbins, "power spectrum: output arrays too short");
+  double* bins = (double*)c.alloc(sizeof(double) * 4 * nbins);
+  MFX_HIP(hipMemsetAsync(bins, 0, sizeof(double) * 4 * nbins, c.stream));
+  long long cnt = (long long)n * n * (n / 2 + 1);
+  hipLaunchKernelGGL(k_pk<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, f, n, kfac, nbins, bins);
+  check_launch("power_spectrum");
+  std::vector<double> h(4 * nbins);
+  MFX_HIP(hipMemcpyAsync(h.data(), bins, sizeof(double) * 4 * nbins, hipMemcpyDeviceToHost, c.stream));
+  MFX_HIP(hipStreamSynchronize(c.stream));
+  c.release(bins);
+  const double volfac = boxlen * boxlen * boxlen / std::pow(2.0 * M_PI, 3.0);
+  const double fn = 1.0 / std::sqrt((double)n * (double)n * (double)n);
+  const double fftfac = fn * fn;
+  for (int i = 0; i < nbins; ++i) {
+    double P = h[4 * i], eP = h[4 * i + 1], kk = h[4 * i + 2], cn = h[4 * i + 3];
+    if (cn > 0) {
+      kk /= cn;
+      P = P / cn * volfac * fftfac;
+      eP = std::sqrt(eP / cn - P * P) / std::sqrt(cn) * volfac * fftfac;
+    }
+    bk[i] = kk;
+    bP[i] = P;
+    beP[i] = eP;
+    bc[i] = cn;
+  }
+  return nbins;
+}
+
+#define MFX_INST(T)                                                                                               \
+  template void noise_normalise<T>(Ctx&, cx<T>*, int, double, int, int);                                          \
+  template void phi1<T>(Ctx&, cx<T>*, const cx<T>*, int, double, const double*);                                  \
+  template void inverse_laplacian<T>(Ctx&, cx<T>*, int, double);                                                  \
+  template void scale<T>(Ctx&, T*, long long, double);                                                            \
+  template void fill<T>(Ctx&, T*, long long, T);                                                                  \
+  template void assemble<T>(Ctx&, cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*,   \
+                            const cx<T>*, int, double, int, int, int, double, double);                            \
+  template void set_particles<T>(Ctx&, const T*, double*, int, int, double, int);                                 \
+  template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
+  template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \
+  template int power_spectrum<T>(Ctx&, const cx<T>*, int, double, double*, double*, double*, double*, int);
+MFX_INST(float)
+MFX_INST(double)
+#undef MFX_INST
+
+}  // namespace mfx

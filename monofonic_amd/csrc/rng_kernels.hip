@@ -1,0 +1,574 @@
+// rng_kernels.hip -- on-device white-noise generators for gfx950.
+//
+// MUSIC1 (src/plugins/random_music_wnoise_generator.cc:588-627,747-796 and
+// random_music.cc:92-116): one workgroup per 32^3 cube.  The cube's MT19937
+// stream (seed baseseed + icube, GSL seeding rule) is produced 624 words at a
+// time by a parallel twist in LDS; every pair of consecutive uniforms is one
+// attempt of Leva's ratio method (gsl_ran_ugaussian_ratio_method), evaluated
+// for all 312 pairs of a block at once, and the accepted deviates are
+// compacted with a block scan so that they land in the cube in stream order.
+// The global mean is taken with a double-double sum, i.e. the correctly
+// rounded exact sum, and subtracted in double (ctor at :35-66).
+//
+// Threefry (external/panphasia_ho/uniform_rand_threefry4x64.c:254-493): one
+// thread per cell; the cell's counter and the tail-replacement branch are
+// restated bit for bit; Box-Muller in double (high_order_panphasia_routines.c
+// :224-228).
+//
+// NGenIC (src/plugins/random_ngenic.cc:275-345): one thread per (i,j) column
+// running its own RANLUX ranlxd1 stream (exact 48-bit arithmetic on doubles).
+//
+// The acceptance tests compare rounded FP64 expressions; contraction into FMA
+// would change their rounding, so it is disabled in this file.
+#include "ops.hpp"
+
+#include <cmath>
+
+#pragma clang fp contract(off)
+
+namespace mfx {
+
+// ------------------------------------------------------------------ double-double
+struct dd {
+  double hi, lo;
+};
+__device__ __forceinline__ dd dd_add(dd a, double b) {
+  double s = a.hi + b;
+  double bb = s - a.hi;
+  double err = (a.hi - (s - bb)) + (b - bb);
+  double lo = a.lo + err;
+  double hi = s + lo;
+  lo = lo - (hi - s);
+  return dd{hi, lo};
+}
+__device__ __forceinline__ dd dd_add2(dd a, dd b) {
+  dd r = dd_add(a, b.hi);
+  return dd_add(r, b.lo);
+}
+
+// ------------------------------------------------------------------ MT19937
+#define MT_N 624
+#define MT_M 397
+
+__device__ __forceinline__ unsigned mt_mag(unsigned y) { return (y & 1u) ? 0x9908b0dfu : 0u; }
+
+__device__ __forceinline__ unsigned mt_temper(unsigned k) {
+  k ^= (k >> 11);
+  k ^= (k << 7) & 0x9d2c5680u;
+  k ^= (k << 15) & 0xefc60000u;
+  k ^= (k >> 18);
+  return k;
+}
+
+// GSL mt_set: seed 0 -> 4357, Knuth initialisation (one thread)
+__device__ void mt_seed(unsigned* mt, unsigned long seed) {
+  if (seed == 0) seed = 4357;
+  mt[0] = (unsigned)(seed & 0xffffffffUL);
+  for (int i = 1; i < MT_N; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (unsigned)i;
+}
+
+// parallel twist of the 624-word state in LDS by blockDim.x (=256) threads
+__device__ void mt_twist(unsigned* mt) {
+  const int tid = threadIdx.x;
+  unsigned y[3], far[3], old623 = 0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    int kk = tid + 256 * r;
+    y[r] = 0;
+    far[r] = 0;
+    if (kk < MT_N - 1) y[r] = (mt[kk] & 0x80000000u) | (mt[kk + 1] & 0x7fffffffu);
+    if (kk < MT_N - MT_M) far[r] = mt[kk + MT_M];
+  }
+  if (tid == 0) old623 = mt[MT_N - 1];
+  __syncthreads();
+  // kk in [0, 227): uses old words only
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    int kk = tid + 256 * r;
+    if (kk < MT_N - MT_M) mt[kk] = far[r] ^ (y[r] >> 1) ^ mt_mag(y[r]);
+  }
+  __syncthreads();
+  // kk in [227, 454): mt[kk-227] is new
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    int kk = tid + 256 * r;
+    if (kk >= MT_N - MT_M && kk < 2 * (MT_N - MT_M)) mt[kk] = mt[kk - (MT_N - MT_M)] ^ (y[r] >> 1) ^ mt_mag(y[r]);
+  }
+  __syncthreads();
+  // kk in [454, 623) and the wrap-around word 623
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    int kk = tid + 256 * r;
+    if (kk >= 2 * (MT_N - MT_M) && kk < MT_N - 1) mt[kk] = mt[kk - (MT_N - MT_M)] ^ (y[r] >> 1) ^ mt_mag(y[r]);
+  }
+  if (tid == 0) {
+    unsigned yy = (old623 & 0x80000000u) | (mt[0] & 0x7fffffffu);
+    mt[MT_N - 1] = mt[MT_M - 1] ^ (yy >> 1) ^ mt_mag(yy);
+  }
+  __syncthreads();
+}
+
+// one attempt of Leva's ratio method on (U1, U2); returns accept flag and value
+__device__ __forceinline__ bool leva(double U1, double U2, double* out) {
+  const double cs = 0.449871, ct = -0.386595, ca = 0.19600, cb = 0.25472, r1 = 0.27597, r2 = 0.27846;
+  double u = 1 - U1;
+  double v = U2 - 0.5;
+  v *= 1.7156;
+  double x = u - cs;
+  double y = fabs(v) - ct;
+  double Q = x * x + y * (ca * y - cb * x);
+  bool reject = (Q >= r1) && ((Q > r2) || (v * v > -4 * u * u * log(u)));
+  *out = v / u;
+  return !reject;
+}
+
+// exclusive scan over 256 threads (one value each), returns total
+__device__ int block_scan(int v, int* sh, int* total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int t = (tid >= off) ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += t;
+    __syncthreads();
+  }
+  int incl = sh[tid];
+  *total = sh[255];
+  __syncthreads();
+  return incl - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, long baseseed, dd* part) {
+  __shared__ unsigned mt[MT_N];
+  __shared__ int scan[256];
+  __shared__ dd red[256];
+  const int tid = threadIdx.x;
+  const long long icube = blockIdx.x;
+  const int ci = (int)(icube / ((long long)nc * nc)), cj = (int)((icube / nc) % nc), ck = (int)(icube % nc);
+  if (tid == 0) mt_seed(mt, (unsigned long)(baseseed + icube));
+  __syncthreads();
+  const long long cnt = (long long)cs * cs * cs;
+  const long long pitch = n + 2;
+  long long count = 0;
+  dd acc{0.0, 0.0};
+  while (count < cnt) {
+    mt_twist(mt);
+    // pairs p = tid and p = tid + 256 (< 312); pair p uses words 2p, 2p+1
+    int nacc = 0;
+    double val[2];
+    bool ok[2] = {false, false};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      int p = tid + 256 * r;
+      if (p < MT_N / 2) {
+        double U1 = mt_temper(mt[2 * p]) / 4294967296.0;
+        double U2 = mt_temper(mt[2 * p + 1]) / 4294967296.0;
+        ok[r] = leva(U1, U2, &val[r]);
+      }
+    }
+    // stream order is p = 0..311: first half pairs (r = 0) precede r = 1
+    int total0, total1;
+    int off0 = block_scan(ok[0] ? 1 : 0, scan, &total0);
+    int off1 = block_scan(ok[1] ? 1 : 0, scan, &total1);
+    nacc = total0 + total1;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (!ok[r]) continue;
+      long long e = count + (r == 0 ? off0 : total0 + off1);
+      if (e < cnt) {
+        int ii = (int)(e / ((long long)cs * cs)), jj = (int)((e / cs) % cs), kk = (int)(e % cs);
+        long long gi = (long long)ci * cs + ii, gj = (long long)cj * cs + jj, gk = (long long)ck * cs + kk;
+        T sv = T(val[r]);
+        data[(gi * n + gj) * pitch + gk] = sv;
+        acc = dd_add(acc, (double)sv);
+      }
+    }
+    count += nacc;
+    __syncthreads();
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = dd_add2(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  if (tid == 0) part[icube] = red[0];
+}
+
+__global__ void k_dd_mean(const dd* part, long long nparts, double inv_count, double* mean) {
+  __shared__ dd red[256];
+  dd acc{0.0, 0.0};
+  for (long long i = threadIdx.x; i < nparts; i += 256) acc = dd_add2(acc, part[i]);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = dd_add2(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double sum = red[0].hi + red[0].lo;
+    mean[0] = sum * inv_count;
+  }
+}
+
+template <typename T>
+__global__ void k_sub_mean(T* data, int n, const double* mean, int zeromean) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long pitch = n + 2;
+  if (p >= (long long)n * n * pitch) return;
+  int k = (int)(p % pitch);
+  if (k >= n) {
+    data[p] = T(0);
+    return;
+  }
+  if (zeromean) data[p] = T((double)data[p] - mean[0]);
+}
+
+template <typename T>
+void music1_fill(Ctx& c, T* data, int n, long baseseed, int cubesize, int zeromean) {
+  int nc = (int)((double)n / cubesize);
+  if (nc < 1) nc = 1;
+  int cs = cubesize;
+  if (n < cubesize) {
+    nc = 1;
+    cs = n;
+  }
+  MFX_REQUIRE((long long)nc * cs == n, "MUSIC1 fill: grid size must be a multiple of the cube size");
+  const long long ncube = (long long)nc * nc * nc;
+  dd* part = (dd*)c.alloc(sizeof(dd) * ncube);
+  double* mean = (double*)c.alloc(sizeof(double));
+  {
+    LaunchScope ls(c, "music1_cubes", (double)n * n * n * sizeof(T));
+    hipLaunchKernelGGL(k_music1_cubes<T>, dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs, baseseed,
+                       part);
+    check_launch("music1_cubes");
+  }
+  hipLaunchKernelGGL(k_dd_mean, dim3(1), dim3(256), 0, c.stream, part, ncube,
+                     1.0 / (double)((size_t)n * (size_t)n * (size_t)n), mean);
+  check_launch("music1_mean");
+  {
+    const long long tot = (long long)n * n * (n + 2);
+    LaunchScope ls(c, "music1_submean", 2.0 * tot * sizeof(T));
+    hipLaunchKernelGGL(k_sub_mean<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, data, n, mean,
+                       zeromean);
+    check_launch("music1_submean");
+  }
+  MFX_HIP(hipStreamSynchronize(c.stream));
+  c.release(part);
+  c.release(mean);
+}
+
+// raw MT19937 words for stream tests: nseeds streams of `count` words
+__global__ void __launch_bounds__(256) k_mt_raw(unsigned* out, const unsigned long* seeds, int count) {
+  __shared__ unsigned mt[MT_N];
+  if (threadIdx.x == 0) mt_seed(mt, seeds[blockIdx.x]);
+  __syncthreads();
+  for (int base = 0; base < count; base += MT_N) {
+    mt_twist(mt);
+    for (int w = threadIdx.x; w < MT_N; w += 256)
+      if (base + w < count) out[(long long)blockIdx.x * count + base + w] = mt_temper(mt[w]);
+    __syncthreads();
+  }
+}
+
+void mt19937_uniforms(Ctx& c, unsigned* out, const unsigned long* seeds, int nseeds, int count) {
+  hipLaunchKernelGGL(k_mt_raw, dim3(nseeds), dim3(256), 0, c.stream, out, seeds, count);
+  check_launch("mt19937_raw");
+}
+
+// ------------------------------------------------------------------ Threefry-4x64-20
+__device__ __forceinline__ unsigned long long rotl64(unsigned long long x, int r) {
+  return (x << r) | (x >> (64 - r));
+}
+
+__device__ void threefry4x64_20(const unsigned long long* ctr, unsigned long long* X) {
+  // key = 0 (uniform_rand_threefry4x64.c:48): ks = {0,0,0,0, parity}
+  const unsigned long long ks[5] = {0ull, 0ull, 0ull, 0ull, 0x1BD11BDAA9FC1A22ull};
+  const int R[8][2] = {{14, 16}, {52, 57}, {23, 40}, {5, 37}, {25, 33}, {46, 12}, {58, 22}, {32, 32}};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) X[i] = ctr[i] + ks[i];
+#pragma unroll
+  for (int r = 0; r < 20; ++r) {
+    if ((r & 1) == 0) {
+      X[0] += X[1];
+      X[1] = rotl64(X[1], R[r % 8][0]);
+      X[1] ^= X[0];
+      X[2] += X[3];
+      X[3] = rotl64(X[3], R[r % 8][1]);
+      X[3] ^= X[2];
+    } else {
+      X[0] += X[3];
+      X[3] = rotl64(X[3], R[r % 8][0]);
+      X[3] ^= X[0];
+      X[2] += X[1];
+      X[1] = rotl64(X[1], R[r % 8][1]);
+      X[1] ^= X[2];
+    }
+    if ((r & 3) == 3) {
+      const unsigned long long s = (unsigned long long)((r + 1) / 4);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) X[i] += ks[(s + i) % 5];
+      X[3] += s;
+    }
+  }
+}
+
+// first `count` uniforms of a cell (uniform_rand_threefry4x64.c:336-489),
+// p_order = 6 and Nbasis = 84 (pan_matrices_order6.h:4-5)
+__device__ void pan_uniforms(unsigned long long l, unsigned long long j1, unsigned long long j2,
+                             unsigned long long j3, unsigned long long seed, double* u, int count) {
+  const unsigned long long p_order = 6, Nbasis = 84;
+  const double g_scale = 1.0 / 4294967296.0, g_shift = 0.5;
+  const unsigned long long j0 = (p_order << 60) + ((l << 56) >> 4) + ((seed << 32) >> 12);
+  const double br = ((double)4096) * g_scale;
+  for (int blk = 0; blk * 8 < count; ++blk) {
+    unsigned long long ctr[4] = {j0 + (unsigned long long)blk, j1, j2, j3}, X[4];
+    threefry4x64_20(ctr, X);
+    for (int w = 0; w < 8 && blk * 8 + w < count; ++w) {
+      unsigned int oi = (w & 1) ? (unsigned int)((X[w >> 1] << 32) >> 32) : (unsigned int)(X[w >> 1] >> 32);
+      u[blk * 8 + w] = ((double)oi + g_shift) * g_scale;
+    }
+  }
+  for (int i = 0; i < count; i += 2) {
+    if (u[i] < br) {
+      double rep = br;
+      unsigned long long iind = (unsigned long long)i / 8 + 1 * (Nbasis + (unsigned long long)i);
+      int jind = i % 8;
+      unsigned long long ctr[4] = {j0 + iind, j1, j2, j3}, X[4];
+      threefry4x64_20(ctr, X);
+      unsigned int oi[8];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        oi[2 * w] = (unsigned int)(X[w] >> 32);
+        oi[2 * w + 1] = (unsigned int)((X[w] << 32) >> 32);
+      }
+      double nv = ((double)oi[jind] + g_shift) * g_scale;
+      if (nv >= br) {
+        rep *= nv;
+      } else {
+        int counter = 0;
+        while (nv < br && counter < 7) {
+          rep *= br;
+          counter++;
+          jind = (jind + 1) % 8;
+          nv = ((double)oi[jind] + g_shift) * g_scale;
+        }
+        rep *= nv;
+      }
+      u[i] = rep;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_threefry_fill(T* data, int n, unsigned long long l, unsigned long long seed) {
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long pitch = n + 2;
+  if (p >= (long long)n * n * pitch) return;
+  int k = (int)(p % pitch);
+  long long ij = p / pitch;
+  if (k >= n) {
+    data[p] = T(0);
+    return;
+  }
+  int j = (int)(ij % n), i = (int)(ij / n);
+  double u[2];
+  pan_uniforms(l, (unsigned long long)i, (unsigned long long)j, (unsigned long long)k, seed, u, 2);
+  const double pi = 4.0 * atan(1.0);
+  const double two_pi = 2.0 * pi;
+  double radius = sqrt(-2.0 * log(u[0]));
+  double angle = two_pi * u[1];
+  data[p] = T(radius * cos(angle));
+}
+
+template <typename T>
+void threefry_fill(Ctx& c, T* data, int n, unsigned long long seed) {
+  unsigned long long l = 0;
+  while ((1ull << l) < (unsigned long long)n) ++l;
+  MFX_REQUIRE((1ull << l) == (unsigned long long)n, "Threefry fill requires a power-of-two grid");
+  MFX_REQUIRE((seed >> 32) == 0, "Threefry seed must satisfy 0 <= seed < 2^32");
+  const long long tot = (long long)n * n * (n + 2);
+  LaunchScope ls(c, "threefry_fill", (double)tot * sizeof(T));
+  hipLaunchKernelGGL(k_threefry_fill<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, data, n, l,
+                     seed);
+  check_launch("threefry_fill");
+}
+
+__global__ void k_threefry_uniforms(double* out, int ncells, const unsigned long long* cells, unsigned long long seed,
+                                    int count) {
+  int cidx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cidx >= ncells) return;
+  double u[64];
+  const unsigned long long* cl = cells + 4 * cidx;
+  pan_uniforms(cl[0], cl[1], cl[2], cl[3], seed, u, count);
+  for (int i = 0; i < count; ++i) out[(long long)cidx * count + i] = u[i];
+}
+
+void threefry_uniforms(Ctx& c, double* out, int ncells, const unsigned long long* cells, unsigned long long seed,
+                       int count) {
+  MFX_REQUIRE(count <= 64 && count % 2 == 0, "threefry_uniforms: count must be even and <= 64");
+  hipLaunchKernelGGL(k_threefry_uniforms, dim3((ncells + 63) / 64), dim3(64), 0, c.stream, out, ncells, cells, seed,
+                     count);
+  check_launch("threefry_uniforms");
+}
+
+// ------------------------------------------------------------------ RANLUX ranlxd1
+struct Ranlxd {
+  double x[12];
+  double carry;
+  int ir, jr, ir_old, pr;
+};
+
+__host__ __device__ __forceinline__ int rl_next(int i) { return i == 11 ? 0 : i + 1; }
+
+__host__ __device__ void rl_seed(Ranlxd& s, unsigned long seed) {
+  const double one_bit = 1.0 / 281474976710656.0;
+  int xbit[31];
+  if (seed == 0) seed = 1;
+  long sd = (long)seed;
+  int i = (int)(sd & 0xFFFFFFFFUL);
+  for (int k = 0; k < 31; ++k) {
+    xbit[k] = i % 2;
+    i /= 2;
+  }
+  int ibit = 0, jbit = 18;
+  for (int k = 0; k < 12; ++k) {
+    double x = 0;
+    for (int l = 1; l <= 48; ++l) {
+      double y = (double)((xbit[ibit] + 1) % 2);
+      x += x + y;
+      xbit[ibit] = (xbit[ibit] + xbit[jbit]) % 2;
+      ibit = (ibit + 1) % 31;
+      jbit = (jbit + 1) % 31;
+    }
+    s.x[k] = one_bit * x;
+  }
+  s.carry = 0;
+  s.ir = 11;
+  s.jr = 7;
+  s.ir_old = 0;
+  s.pr = 202;
+}
+
+// one carry step on x[ir] -= x[jr] convention of RANLUX
+__host__ __device__ void rl_increment(Ranlxd& s) {
+  const double one_bit = 1.0 / 281474976710656.0;
+  double carry = s.carry;
+  int ir = s.ir, jr = s.jr;
+  // the block form of the reference (12 steps at a time) performs exactly the
+  // same sequence of operations as this single-step loop
+  for (int k = 0; k < s.pr; ++k) {
+    double y1 = s.x[jr] - s.x[ir];
+    double y2 = y1 - carry;
+    if (y2 < 0) {
+      carry = one_bit;
+      y2 += 1;
+    } else {
+      carry = 0;
+    }
+    s.x[ir] = y2;
+    ir = rl_next(ir);
+    jr = rl_next(jr);
+  }
+  s.ir = ir;
+  s.ir_old = ir;
+  s.jr = jr;
+  s.carry = carry;
+}
+
+__host__ __device__ __forceinline__ double rl_uniform(Ranlxd& s) {
+  int ir = s.ir;
+  s.ir = rl_next(ir);
+  if (s.ir == s.ir_old) rl_increment(s);
+  return s.x[s.ir];
+}
+
+// RNG_ngenic ctor (random_ngenic.cc:230-266): seed table from one ranlxd1
+// stream in the reference's octant order, evaluated on the host
+void ngenic_seedtable(unsigned* table, int n, long seed) {
+  Ranlxd s;
+  rl_seed(s, (unsigned long)seed);
+  const size_t N = (size_t)n;
+  for (size_t q = 0; q < N * N; ++q) table[q] = 0u;
+  auto draw = [&]() { return (unsigned)(0x7fffffff * rl_uniform(s)); };
+  for (size_t i = 0; i < N / 2; ++i) {
+    for (size_t j = 0; j < i; j++) table[i * N + j] = draw();
+    for (size_t j = 0; j < i + 1; j++) table[j * N + i] = draw();
+    for (size_t j = 0; j < i; j++) table[(N - 1 - i) * N + j] = draw();
+    for (size_t j = 0; j < i + 1; j++) table[(N - 1 - j) * N + i] = draw();
+    for (size_t j = 0; j < i; j++) table[i * N + (N - 1 - j)] = draw();
+    for (size_t j = 0; j < i + 1; j++) table[j * N + (N - 1 - i)] = draw();
+    for (size_t j = 0; j < i; j++) table[(N - 1 - i) * N + (N - 1 - j)] = draw();
+    for (size_t j = 0; j < i + 1; j++) table[(N - 1 - j) * N + (N - 1 - i)] = draw();
+  }
+}
+
+__global__ void k_ranlxd_raw(double* out, const unsigned* seeds, int count) {
+  Ranlxd s;
+  rl_seed(s, seeds[blockIdx.x * blockDim.x + threadIdx.x]);
+  for (int i = 0; i < count; ++i) out[(long long)(blockIdx.x * blockDim.x + threadIdx.x) * count + i] = rl_uniform(s);
+}
+
+void ranlxd_uniforms(Ctx& c, double* out, const unsigned* seeds, int nseeds, int count) {
+  MFX_REQUIRE(nseeds % 64 == 0, "ranlxd_uniforms: nseeds must be a multiple of 64");
+  hipLaunchKernelGGL(k_ranlxd_raw, dim3(nseeds / 64), dim3(64), 0, c.stream, out, seeds, count);
+  check_launch("ranlxd_raw");
+}
+
+// RNG_ngenic::Fill_Grid, non-distributed branch
+template <typename T>
+__global__ void k_ngenic(cx<T>* g, int n, const unsigned* table) {
+  long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= (long long)n * n) return;
+  const int i = (int)(col / n), j = (int)(col % n);
+  const int nh = n / 2 + 1;
+  const int ii = (i > 0) ? n - i : 0, jj = (j > 0) ? n - j : 0;
+  Ranlxd s;
+  rl_seed(s, table[(long long)i * n + j]);
+  for (int k = 0; k < nh; ++k) {
+    double phase = rl_uniform(s) * 2 * 3.14159265358979323846;
+    double ampl = 0;
+    do {
+      ampl = rl_uniform(s);
+    } while (ampl == 0 || ampl == 1);
+    if (i == n / 2 || j == n / 2 || k == n / 2) continue;
+    if (i == 0 && j == 0 && k == 0) continue;
+    ampl = sqrt(-log(ampl));
+    cx<T> z = mk(T(ampl * cos(phase)), T(ampl * sin(phase)));
+    auto at = [&](int a, int b, int c) -> cx<T>& { return g[((long long)a * n + b) * nh + c]; };
+    if (k > 0) {
+      at(i, j, k) = z;
+    } else {
+      if (i == 0) {
+        if (j < n / 2) {
+          at(i, j, k) = z;
+          at(i, jj, k) = conj(z);
+        }
+      } else if (i < n / 2) {
+        at(i, j, k) = z;
+        at(ii, jj, k) = conj(z);
+      }
+    }
+  }
+}
+
+template <typename T>
+void ngenic_fill(Ctx& c, cx<T>* data, int n, const unsigned* table) {
+  const long long tot = (long long)n * n * (n / 2 + 1);
+  MFX_HIP(hipMemsetAsync(data, 0, sizeof(cx<T>) * tot, c.stream));
+  LaunchScope ls(c, "ngenic_fill", (double)tot * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_ngenic<T>, dim3((unsigned)(((long long)n * n + 63) / 64)), dim3(64), 0, c.stream, data, n,
+                     table);
+  check_launch("ngenic_fill");
+}
+
+template void music1_fill<float>(Ctx&, float*, int, long, int, int);
+template void music1_fill<double>(Ctx&, double*, int, long, int, int);
+template void threefry_fill<float>(Ctx&, float*, int, unsigned long long);
+template void threefry_fill<double>(Ctx&, double*, int, unsigned long long);
+template void ngenic_fill<float>(Ctx&, cx<float>*, int, const unsigned*);
+template void ngenic_fill<double>(Ctx&, cx<double>*, int, const unsigned*);
+
+}  // namespace mfx
