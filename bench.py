@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: 3LPT initial conditions at 512^3 on one MI355X.
+
+Workload (the mixed-precision 3LPT configuration): simple-cubic 512^3 lattice,
+box 1000 Mpc/h, z_start = 24, MUSIC1 white noise, Eisenstein-Hu P(k) with
+Omega_m = 0.3, Omega_b = 0.045, h = 0.7, n_s = 0.96, sigma_8 = 0.81.  FFTs and
+Orszag convolutions (768^3 padded) run in FP32; amplitude table, growth
+factors and the position/velocity assembly run in FP64.
+
+A step is one whole realisation on the device: noise fill -> r2c -> phi(1) ->
+phi(2), phi(3a), phi(3b), A(3) -> growth scaling -> 6 c2r transforms -> FP64
+positions and velocities, all resident in HBM.  value = particles/s = 512^3 /
+(time per step).
+
+Also reported on the same line:
+  roofline      achieved HBM GB/s of the dominant kernel (algorithmic bytes
+                per launch / average launch time from HIP events on the
+                library's stream) against 8 TB/s
+  fft_gbps      algorithmic bytes / time summed over every FFT pass
+  fp64          particles/s of the same workload with FP64 bulk
+  cpu_baseline  the FP64 CPU oracle (OpenMP) on a 128^3 3LPT sample
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "3LPT IC particles/s at 512^3, FP32 vs FP64 on GPU; 3D FFT achieved HBM GB/s"
+HBM_PEAK_GBPS = 8000.0
+FFT_KERNELS = ("fft_", "conv_p", "lpt_")
+
+
+def log(msg):
+    sys.stderr.write(msg + "\n")
+    sys.stderr.flush()
+
+
+def run_gpu(M, ctx, n, precision, steps, warmup, fused, profile_step=True):
+    params = dict(GridRes=n, BoxLength=1000.0, zstart=24.0, LPTorder=3, precision=precision, generator="MUSIC1",
+                  seed=12345, fused=bool(fused))
+    ic = M.ICGenerator(ctx, **params)
+    for w in range(warmup):
+        ic.run()
+        ctx.synchronize()
+        log(f"[bench] {precision} warmup {w + 1}/{warmup} done")
+    ctx.synchronize()
+    ctx.record(0)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ic.run()
+        if s + 1 < steps:
+            ctx.synchronize()
+            log(f"[bench] {precision} step {s + 1}/{steps} done")
+    ctx.record(1)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ctx.elapsed_ms(0, 1)
+    log(f"[bench] {precision} {steps} steps: wall {wall:.4f} s, events {ev_ms / 1e3:.4f} s")
+    stats = None
+    if profile_step:
+        ctx.profile_reset()
+        ctx.profile(True)
+        ic.run()
+        ctx.synchronize()
+        stats = ctx.profile_stats()
+        ctx.profile(False)
+        log(f"[bench] {precision} profiled step done")
+    del ic
+    return wall, ev_ms, stats
+
+
+def roofline(stats):
+    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    name, s = dom
+    avg_ms = s["ms"] / s["launches"]
+    bytes_per = s["bytes"] / s["launches"]
+    achieved = bytes_per / (avg_ms * 1e-3) / 1e9
+    fft = [v for k, v in stats.items() if k.startswith(FFT_KERNELS)]
+    fft_bytes = sum(v["bytes"] for v in fft)
+    fft_ms = sum(v["ms"] for v in fft)
+    return {
+        "bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "launches": s["launches"],
+        "avg_ms": round(avg_ms, 4), "bytes_per_launch": bytes_per,
+    }, (fft_bytes / (fft_ms * 1e-3) / 1e9 if fft_ms > 0 else None)
+
+
+def cpu_baseline(n=128):
+    """FP64 CPU oracle (OpenMP) on an n^3 3LPT sample of the same workload"""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_gpu_lpt import oracle_pipeline
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cos = O.Cosmology()
+    t0 = time.perf_counter()
+    oracle_pipeline(O, n, 1000.0 * n / 512, 24.0, 3, 12345, cos)
+    dt = time.perf_counter() - t0
+    return {"value": round(n ** 3 / dt, 1), "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"3LPT {n}^3 sc, box {1000.0 * n / 512:g} Mpc/h, MUSIC1, FP64 oracle (OpenMP), {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--fused", type=int, default=1, help="1: fused LPT source engine, 0: reference convolve sequence")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-bulk comparison leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the path is single-device by design", file=sys.stderr)
+        sys.exit(2)
+
+    import monofonic_amd as M
+    ctx = M.Context(0)
+    n = args.n
+    wall, ev_ms, stats = run_gpu(M, ctx, n, "float", args.steps, args.warmup, args.fused)
+    ms_per_step = ev_ms / args.steps
+    value = n ** 3 / (ms_per_step * 1e-3)
+    roof, fft_gbps = roofline(stats)
+
+    fp64 = None
+    if not args.no_fp64:
+        w64, e64, st64 = run_gpu(M, ctx, n, "double", max(1, args.steps // 2), 1, args.fused)
+        ms64 = e64 / max(1, args.steps // 2)
+        _, fft64 = roofline(st64)
+        fp64 = {"value": round(n ** 3 / (ms64 * 1e-3), 1), "unit": "particles/s", "ms_per_step": round(ms64, 3),
+                "fft_gbps": round(fft64, 1) if fft64 else None, "fp32_speedup": round(ms64 / ms_per_step, 3)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU oracle baseline ...")
+        cpu = cpu_baseline(args.cpu_n)
+        log(f"[bench] CPU oracle baseline done: {cpu['value']} particles/s")
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3LPT sc {n}^3, box 1000 Mpc/h, z_start 24, MUSIC1 noise, Eisenstein-Hu, "
+                               f"FP32 bulk ({3 * n // 2}^3 padded) + FP64 tables/assembly",
+                   "n": n, "lpt_order": 3, "padded": 3 * n // 2, "fused_lpt": bool(args.fused),
+                   "wall_s": round(wall, 4)},
+        "roofline": roof, "fft_gbps": round(fft_gbps, 1) if fft_gbps else None,
+        "fft_frac": round(fft_gbps / HBM_PEAK_GBPS, 4) if fft_gbps else None,
+        "fp64": fp64, "cpu_baseline": cpu,
+        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                        "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                    for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

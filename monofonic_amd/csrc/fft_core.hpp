@@ -112,6 +112,18 @@ struct BF<8, INV, T> {
   }
 };
 
+// Returns p unchanged but hides its value from the optimiser.  Kernels that
+// run several transforms of the same length would otherwise have the
+// identical twiddle loads of every inlined transform merged and kept live in
+// registers for the whole kernel; passing the table through this barrier at
+// each transform keeps twiddle lifetimes local to that transform.  The asm
+// statement is empty: it emits no instruction.
+template <typename P>
+__device__ __forceinline__ P* opaque(P* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 // ---------------------------------------------------------------- plans
 // Radix schedule per length: EPT must be a multiple of every radix.
 template <int L>
@@ -133,17 +145,17 @@ MFX_PLAN(36, 12, 3, 3, 4)
 MFX_PLAN(48, 12, 3, 4, 4)
 MFX_PLAN(64, 8, 8, 8)
 MFX_PLAN(72, 12, 3, 3, 2, 4)
-MFX_PLAN(96, 24, 3, 4, 8)
+MFX_PLAN(96, 12, 3, 2, 4, 4)
 MFX_PLAN(128, 8, 8, 4, 4)
 MFX_PLAN(144, 12, 3, 3, 4, 4)
-MFX_PLAN(192, 24, 3, 8, 8)
+MFX_PLAN(192, 12, 3, 4, 4, 4)
 MFX_PLAN(256, 8, 8, 8, 4)
-MFX_PLAN(288, 24, 3, 3, 4, 8)
-MFX_PLAN(384, 24, 3, 8, 4, 4)
+MFX_PLAN(288, 12, 3, 3, 2, 4, 4)
+MFX_PLAN(384, 12, 3, 2, 4, 4, 4)
 MFX_PLAN(512, 8, 8, 8, 8)
-MFX_PLAN(768, 24, 3, 8, 8, 4)
+MFX_PLAN(768, 12, 3, 4, 4, 4, 4)
 MFX_PLAN(1024, 8, 8, 8, 4, 4)
-MFX_PLAN(1536, 24, 3, 8, 8, 8)
+MFX_PLAN(1536, 12, 3, 2, 4, 4, 4, 4)
 #undef MFX_PLAN
 
 template <int L>
@@ -197,7 +209,7 @@ __device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds,
 template <typename T, int L, bool INV>
 __device__ __forceinline__ void fft_exec(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t,
                                          const cx<T>* __restrict__ tw) {
-  stage_loop<T, L, INV, 0, 1>(v, lds, ls, t, tw);
+  stage_loop<T, L, INV, 0, 1>(v, lds, ls, t, opaque(tw));
 }
 
 }  // namespace mfx

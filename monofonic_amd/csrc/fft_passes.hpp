@@ -1,0 +1,381 @@
+// fft_passes.hpp -- pass-level building blocks shared by the FFT, Orszag and
+// LPT-engine translation units: launch configuration, line sources/sinks
+// (plain, zero-padded with Fourier multiplier, truncating, output operator),
+// the column kernel over tiles of adjacent columns and the z-row kernels
+// (r2c, c2r, fused c2r-product-r2c).
+#pragma once
+#include "mfx_internal.hpp"
+#include "row_fft.hpp"
+
+#include <cmath>
+#include <string>
+
+namespace mfx {
+
+// ------------------------------------------------------------------ config
+template <typename T, int L>
+struct ColCfg {
+  static constexpr int TPS = PlanInfo<L>::TPS;
+  static constexpr int lim_lds = 65536 / (L * (int)sizeof(cx<T>));
+  static constexpr int lim_thr = 1024 / TPS;
+  static constexpr int lim = (lim_lds < lim_thr ? lim_lds : lim_thr) < 16 ? (lim_lds < lim_thr ? lim_lds : lim_thr) : 16;
+  static constexpr int C = lim >= 16 ? 16 : lim >= 8 ? 8 : lim >= 4 ? 4 : lim >= 2 ? 2 : 1;
+};
+
+
+// ------------------------------------------------------------------ sources / sinks
+template <typename T>
+__device__ __forceinline__ T kval(int i, int n, T kfac, int nyq) {
+  // grid_fft.hh:282-296 (get_k) and 376-385 (gradient, zero at n/2)
+  if (nyq && i == (n >> 1)) return T(0);
+  return (T(i) - T(i > (n >> 1)) * T(n)) * kfac;
+}
+
+template <typename T>
+struct MultDev {
+  int nterms;
+  T cre[2], cim[2];
+  int pw[2][3];
+  int nyq[2];
+  __device__ __forceinline__ cx<T> eval(int i, int j, int k, int n, T kfac) const {
+    T sr = 0, si = 0;
+    const int idx[3] = {i, j, k};
+    for (int t = 0; t < nterms; ++t) {
+      T f = 1;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T kd = kval<T>(idx[d], n, kfac, nyq[t]);
+        for (int p = 0; p < pw[t][d]; ++p) f *= kd;
+      }
+      sr += cre[t] * f;
+      si += cim[t] * f;
+    }
+    return mk(sr, si);
+  }
+};
+
+template <typename T>
+MultDev<T> to_dev(const MultSpec& m) {
+  MultDev<T> d;
+  d.nterms = m.nterms;
+  for (int t = 0; t < 2; ++t) {
+    d.cre[t] = T(m.coef_re[t]);
+    d.cim[t] = T(m.coef_im[t]);
+    d.nyq[t] = m.nyq[t];
+    for (int q = 0; q < 3; ++q) d.pw[t][q] = m.pow[t][q];
+  }
+  return d;
+}
+
+template <typename T>
+struct SrcPlain {
+  const cx<T>* p;
+  long long sb, sp;
+  __device__ __forceinline__ cx<T> load(int b, int col, int pos) const {
+    return p[(long long)b * sb + (long long)pos * sp + col];
+  }
+};
+
+template <typename T>
+struct DstPlain {
+  cx<T>* p;
+  long long sb, sp;
+  T scale;
+  __device__ __forceinline__ cx<T> load_old(int, int, int) const { return mk(T(0), T(0)); }
+  __device__ __forceinline__ void store(int b, int col, int pos, cx<T> v, cx<T>) const {
+    p[(long long)b * sb + (long long)pos * sp + col] = v * scale;
+  }
+};
+
+// line of an n-grid zero-padded to M = 3n/2: position pos <- source index
+// pos (pos <= n/2) or pos - n/2 (pos > n); convolution.hh:569-592
+template <typename T, bool MULT>
+struct SrcPad {
+  const cx<T>* p;
+  long long sb, sp;
+  int n;
+  T scale;
+  T kfac;
+  MultDev<T> m;  // used when MULT: line axis = x (i), b = y (j), col = kz
+  // The load is unconditional (index clamped into the line) and the padding
+  // is applied by a select, so all loads of a thread can be in flight at once.
+  __device__ __forceinline__ cx<T> load(int b, int col, int pos) const {
+    const int nh = n >> 1;
+    const bool pad = (pos > nh) & (pos <= n);
+    const int i = pad ? 0 : (pos <= nh ? pos : pos - nh);
+    cx<T> v = p[(long long)b * sb + (long long)i * sp + col];
+    if (MULT) v = cmul(v, m.eval(i, b, col, n, kfac));
+    v = v * scale;
+    return pad ? mk(T(0), T(0)) : v;
+  }
+};
+
+// keep the n retained outputs of an M-line (convolution.hh:629-634)
+template <typename T>
+struct DstTrunc {
+  cx<T>* p;
+  long long sb, sp;
+  int n;
+  __device__ __forceinline__ cx<T> load_old(int, int, int) const { return mk(T(0), T(0)); }
+  __device__ __forceinline__ void store(int b, int col, int pos, cx<T> v, cx<T>) const {
+    const int nh = n >> 1;
+    int o;
+    if (pos <= nh)
+      o = pos;
+    else if (pos > n)
+      o = pos - nh;
+    else
+      return;
+    p[(long long)b * sb + (long long)o * sp + col] = v;
+  }
+};
+
+// final x pass: truncation, Nyquist zeroing (convolution.hh:637-643), scale
+// and the output operator on the flat k-space array (operators.hh:25-35)
+template <typename T>
+struct DstTruncOp {
+  cx<T>* out;
+  long long sb, sp;  // b = j stride, pos = i stride (complex units)
+  int n;
+  int op;
+  T opfac, scale;
+  int invlap;  // fold apply_InverseLaplacian (grid_fft.hh:440-449)
+  T kfac;
+  // previous output value for the accumulating operators, loaded for every
+  // element of the thread before any store (clamped index, no branch)
+  __device__ __forceinline__ cx<T> load_old(int b, int col, int pos) const {
+    if (op == OP_ASSIGN) return mk(T(0), T(0));
+    const int nh = n >> 1;
+    const bool drop = (pos > nh) & (pos <= n);
+    const int i = drop ? 0 : (pos <= nh ? pos : pos - nh);
+    return out[(long long)i * sp + (long long)b * sb + col];
+  }
+  __device__ __forceinline__ void store(int b, int col, int pos, cx<T> v, cx<T> prev) const {
+    const int nh = n >> 1;
+    int i;
+    if (pos <= nh)
+      i = pos;
+    else if (pos > n)
+      i = pos - nh;
+    else
+      return;
+    const bool nyq = (i == nh) | (b == nh) | (col == nh);
+    cx<T> val = nyq ? mk(T(0), T(0)) : v * scale;
+    const long long idx = (long long)i * sp + (long long)b * sb + col;
+    cx<T> o;
+    if (op == OP_ASSIGN)
+      o = val;
+    else if (op == OP_ADD)
+      o = prev + val;
+    else if (op == OP_SUB)
+      o = prev - val;
+    else
+      o = prev + val * opfac;
+    if (invlap) {
+      T kx = kval<T>(i, n, kfac, 0), ky = kval<T>(b, n, kfac, 0), kz = kval<T>(col, n, kfac, 0);
+      T k2 = kx * kx + ky * ky + kz * kz;
+      o = (i == 0 && b == 0 && col == 0) ? mk(T(0), T(0)) : mk(-o.x / k2, -o.y / k2);
+    }
+    out[idx] = o;
+  }
+};
+
+// ------------------------------------------------------------------ column kernel
+template <typename T, int L, bool INV, class Src, class Dst>
+__global__ void __launch_bounds__((ColCfg<T, L>::C * ColCfg<T, L>::TPS))
+    k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
+  constexpr int C = ColCfg<T, L>::C;
+  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  __shared__ cx<T> lds[L * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const int b = blockIdx.y;
+  const bool act = col < ncols;
+  // lanes past the last column load column ncols-1 and store nothing
+  const int colc = act ? col : ncols - 1;
+  cx<T> v[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
+  fft_exec<T, L, INV>(v, lds + c, C, t, tw);
+  cx<T> old[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) old[e] = dst.load_old(b, colc, t + TPS * e);
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) dst.store(b, col, t + TPS * e, v[e], old[e]);
+  }
+}
+
+template <typename T, int L, bool INV, class Src, class Dst>
+void launch_col_L(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  constexpr int C = ColCfg<T, L>::C;
+  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst>), grid, block, 0, c.stream, s, d, c.twiddles<T>(L), ncols);
+  check_launch(name);
+}
+
+#define MFX_COL_LENGTHS(X) \
+  X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(768) X(1024) X(1536)
+#define MFX_ROW_LENGTHS(X) \
+  X(8) X(12) X(16) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(768)
+
+template <typename T, bool INV, class Src, class Dst>
+void launch_col(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  switch (L) {
+#define MFX_CASE(LL)                                                    \
+  case LL:                                                              \
+    launch_col_L<T, LL, INV, Src, Dst>(c, s, d, ncols, nbatch, name, bytes); \
+    return;
+    MFX_COL_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      throw Error(1, "unsupported transform length " + std::to_string(L));
+  }
+}
+
+// ------------------------------------------------------------------ row kernels (z axis)
+// One workgroup = one wavefront holding RowLanes<K>::RPW rows; the row
+// transforms are the LDS-resident ones of row_fft.hpp.
+template <typename T, int K>
+struct RowBlk {
+  static constexpr int P = RowLanes<K>::P, RPW = RowLanes<K>::RPW, LB = RowLanes<K>::LB;
+  static constexpr int LDS = RPW * 2 * LB;  // complex elements of LDS per workgroup
+};
+
+#define MFX_ROW_PROLOGUE(T, K)                                                      \
+  constexpr int P_ = RowBlk<T, K>::P, LB_ = RowBlk<T, K>::LB;                       \
+  __shared__ cx<T> lds_[RowBlk<T, K>::LDS];                                         \
+  const int lane = threadIdx.x % P_, r_ = threadIdx.x / P_;                         \
+  const long long row = (long long)blockIdx.x * RowBlk<T, K>::RPW + r_;             \
+  const bool act = row < nrows;                                                     \
+  const long long rowc = act ? row : 0;                                             \
+  RowCtx<T, K> rc{lds_ + r_ * 2 * LB_, lds_ + r_ * 2 * LB_ + LB_, lane, twK, twM};
+
+template <typename T, int K>
+__global__ void __launch_bounds__(64)
+    k_row_r2c(const T* __restrict__ src, long long spitch, cx<T>* dst, long long dpitch, int hout,
+              long long nrows, T scale, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+  MFX_ROW_PROLOGUE(T, K)
+  constexpr int E = RowCtx<T, K>::E;
+  const cx<T>* s = reinterpret_cast<const cx<T>*>(src + rowc * spitch);
+  cx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int n = lane + P_ * e;
+    v[e] = s[n < K ? n : K - 1];
+  }
+  rc.r2c_store(v, dst + rowc * dpitch, hout, act, scale);
+}
+
+template <typename T, int K>
+__global__ void __launch_bounds__(64)
+    k_row_c2r(const cx<T>* src, long long spitch, int hin, T* dst, long long dpitch, long long nrows, T scale,
+              const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+  MFX_ROW_PROLOGUE(T, K)
+  constexpr int E = RowCtx<T, K>::E;
+  const cx<T>* buf = rc.c2r(src + rowc * spitch, hin);
+  if (act) {
+    cx<T>* d = reinterpret_cast<cx<T>*>(dst + row * dpitch);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int n = lane + P_ * e;
+      if (n < K) d[n] = buf[n] * scale;
+    }
+    if (lane == 0 && dpitch >= 2 * K + 2) d[K] = mk(T(0), T(0));
+  }
+}
+
+// P3: c2r of two truncated half spectra, pointwise product, r2c, truncation
+template <typename T, int K>
+__global__ void __launch_bounds__(64)
+    k_row_prod(const cx<T>* a, const cx<T>* b, long long spitch, int hin, cx<T>* dst, long long dpitch,
+               int hout, long long nrows, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+  MFX_ROW_PROLOGUE(T, K)
+  constexpr int E = RowCtx<T, K>::E;
+  cx<T> va[E], vb[E];
+  rc.get(va, rc.c2r(a + rowc * spitch, hin));
+  rc.get(vb, rc.c2r(b + rowc * spitch, hin));
+  // packed real product: z[n] = x[2n] + i x[2n+1]
+#pragma unroll
+  for (int e = 0; e < E; ++e) va[e] = mk(va[e].x * vb[e].x, va[e].y * vb[e].y);
+  rc.r2c_store(va, dst + rowc * dpitch, hout, act, T(1));
+}
+
+template <typename T, int K>
+dim3 row_grid(long long nrows) {
+  return dim3((unsigned)((nrows + RowBlk<T, K>::RPW - 1) / RowBlk<T, K>::RPW));
+}
+
+template <typename T, int K>
+void launch_row_r2c_K(Ctx& c, const T* src, long long spitch, cx<T>* dst, long long dpitch, int hout,
+                      long long nrows, T scale, const char* name, double bytes) {
+  const dim3 grid = row_grid<T, K>(nrows);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_row_r2c<T, K>), grid, dim3(64), 0, c.stream, src, spitch, dst, dpitch, hout, nrows, scale,
+                     c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch(name);
+}
+
+template <typename T, int K>
+void launch_row_c2r_K(Ctx& c, const cx<T>* src, long long spitch, int hin, T* dst, long long dpitch,
+                      long long nrows, T scale, const char* name, double bytes) {
+  const dim3 grid = row_grid<T, K>(nrows);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_row_c2r<T, K>), grid, dim3(64), 0, c.stream, src, spitch, hin, dst, dpitch, nrows, scale,
+                     c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch(name);
+}
+
+template <typename T, int K>
+void launch_row_prod_K(Ctx& c, const cx<T>* a, const cx<T>* b, long long spitch, int hin, cx<T>* dst,
+                       long long dpitch, int hout, long long nrows, const char* name, double bytes) {
+  const dim3 grid = row_grid<T, K>(nrows);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_row_prod<T, K>), grid, dim3(64), 0, c.stream, a, b, spitch, hin, dst, dpitch, hout, nrows,
+                     c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch(name);
+}
+
+#define MFX_ROW_SWITCH(K, CALL)                                         \
+  switch (K) {                                                          \
+    MFX_ROW_LENGTHS(CALL)                                               \
+    default:                                                            \
+      throw Error(1, "unsupported row length " + std::to_string(2 * (K))); \
+  }
+
+template <typename T>
+void launch_row_r2c(Ctx& c, int K, const T* src, long long spitch, cx<T>* dst, long long dpitch, int hout,
+                    long long nrows, T scale, const char* name, double bytes) {
+#define MFX_CASE(KK)                                                                              \
+  case KK:                                                                                        \
+    launch_row_r2c_K<T, KK>(c, src, spitch, dst, dpitch, hout, nrows, scale, name, bytes);       \
+    return;
+  MFX_ROW_SWITCH(K, MFX_CASE)
+#undef MFX_CASE
+}
+
+template <typename T>
+void launch_row_c2r(Ctx& c, int K, const cx<T>* src, long long spitch, int hin, T* dst, long long dpitch,
+                    long long nrows, T scale, const char* name, double bytes) {
+#define MFX_CASE(KK)                                                                              \
+  case KK:                                                                                        \
+    launch_row_c2r_K<T, KK>(c, src, spitch, hin, dst, dpitch, nrows, scale, name, bytes);        \
+    return;
+  MFX_ROW_SWITCH(K, MFX_CASE)
+#undef MFX_CASE
+}
+
+template <typename T>
+void launch_row_prod(Ctx& c, int K, const cx<T>* a, const cx<T>* b, long long spitch, int hin, cx<T>* dst,
+                     long long dpitch, int hout, long long nrows, const char* name, double bytes) {
+#define MFX_CASE(KK)                                                                                \
+  case KK:                                                                                          \
+    launch_row_prod_K<T, KK>(c, a, b, spitch, hin, dst, dpitch, hout, nrows, name, bytes);        \
+    return;
+  MFX_ROW_SWITCH(K, MFX_CASE)
+#undef MFX_CASE
+}
+
+
+}  // namespace mfx

@@ -1,18 +1,510 @@
-// lpt_engine.hip -- fused nLPT source engine (placeholder until implemented)
+// lpt_engine.hip -- all nLPT source terms of ic_generator::run
+// (src/ic_generator.cc:464-547) from the Hessian tensors of phi and phi2.
+//
+// The reference evaluates 5 (2LPT) or 33 (3LPT) OrszagConvolver::convolve2
+// calls, each padding, transforming and truncating its two operands.  Every
+// operand is a Hessian component phi_{,ab} = IFFT[(i g_a)(i g_b) phi] of one of
+// only two fields (phi, phi2) or one of four 3a intermediates, so here each
+// field is transformed once into the six padded real-space Hessian rows:
+//
+//   x stage : X_p = IDFT_x[pad(g_x^p F)]               p = 0, 1, 2
+//   y stage : Y0, Yy, Yyy from X_0; Yx, Yxy from X_1; Yxx from X_2
+//   z stage : the kz factor g_z^p and the sign are applied on load, c2r per
+//             row, products formed pointwise in registers, r2c, kz <= n/2
+//   forward : y then x transforms with truncation, Nyquist zeroing and the
+//             inverse Laplacian (grid_fft.hh:440-449) fused into the last pass
+//
+// where g_d is the Nyquist-zeroed gradient wave number of the n-grid index
+// (Grid_FFT::gradient, grid_fft.hh:376-385).  The products are the
+// reference's, regrouped: the truncation (unpad) is linear, so summing the
+// real-space products before one forward transform equals summing the
+// truncated products; the 3a intermediates trunc(phi_xx phi_yy - phi_xy^2),
+// trunc(phi_xy phi_xz), trunc(phi_yz^2), trunc(phi_xz^2) keep the two-stage
+// truncation of OrszagConvolver::convolve3 (convolution.hh:542-549).
+#include "fft_passes.hpp"
 #include "ops.hpp"
 
 namespace mfx {
 
-size_t lpt_workspace_bytes(int n, int prec, int order) { return 16; }
+// ------------------------------------------------------------------ x/y stages
+template <typename T>
+struct OutSet {
+  cx<T>* p[3];
+  int pw[3];
+  int nout;
+};
+
+// one padded source line, up to three outputs IDFT[pad(g^pw * line)]
+template <typename T, int L>
+__global__ void __launch_bounds__((ColCfg<T, L>::C * ColCfg<T, L>::TPS))
+    k_col_pad_multi(const cx<T>* src, long long ssb, long long ssp, int n, T scale, T kfac, OutSet<T> outs,
+                    long long dsb, long long dsp, int ncols, const cx<T>* __restrict__ tw) {
+  constexpr int C = ColCfg<T, L>::C;
+  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  __shared__ cx<T> lds[L * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int col = blockIdx.x * C + c;
+  const int b = blockIdx.y;
+  const bool act = col < ncols;
+  const int colc = act ? col : ncols - 1;
+  const int nh = n >> 1;
+  cx<T> v0[EPT];
+  T g[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int pos = t + TPS * e;
+    const bool pad = (pos > nh) & (pos <= n);
+    const int i = pad ? 0 : (pos <= nh ? pos : pos - nh);
+    // unconditional load (clamped index), padding applied by select
+    const cx<T> x = src[(long long)b * ssb + (long long)i * ssp + colc];
+    g[e] = pad ? T(0) : kval<T>(i, n, kfac, 1);
+    v0[e] = pad ? mk(T(0), T(0)) : x * scale;
+  }
+  for (int o = 0; o < outs.nout; ++o) {
+    cx<T> v[EPT];
+    const int pw = outs.pw[o];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      T f = pw == 0 ? T(1) : (pw == 1 ? g[e] : g[e] * g[e]);
+      v[e] = v0[e] * f;
+    }
+    fft_exec<T, L, true>(v, lds + c, C, t, tw);
+    if (act) {
+      cx<T>* d = outs.p[o];
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) d[(long long)b * dsb + (long long)(t + TPS * e) * dsp + col] = v[e];
+    }
+  }
+}
+
+template <typename T, int L>
+void launch_pad_multi_L(Ctx& c, const cx<T>* src, long long ssb, long long ssp, int n, T scale, T kfac,
+                        const OutSet<T>& outs, long long dsb, long long dsp, int ncols, int nbatch, const char* name,
+                        double bytes) {
+  constexpr int C = ColCfg<T, L>::C;
+  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_col_pad_multi<T, L>), grid, block, 0, c.stream, src, ssb, ssp, n, scale, kfac, outs, dsb,
+                     dsp, ncols, c.twiddles<T>(L));
+  check_launch(name);
+}
+
+template <typename T>
+void launch_pad_multi(Ctx& c, int L, const cx<T>* src, long long ssb, long long ssp, int n, T scale, T kfac,
+                      const OutSet<T>& outs, long long dsb, long long dsp, int ncols, int nbatch, const char* name,
+                      double bytes) {
+  switch (L) {
+#define MFX_CASE(LL)                                                                                          \
+  case LL:                                                                                                    \
+    launch_pad_multi_L<T, LL>(c, src, ssb, ssp, n, scale, kfac, outs, dsb, dsp, ncols, nbatch, name, bytes); \
+    return;
+    MFX_COL_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      throw Error(1, "unsupported padded length " + std::to_string(L));
+  }
+}
+
+// ------------------------------------------------------------------ z stage
+// Y arrays of one field: 0 -> Y0, 1 -> Yx, 2 -> Yy, 3 -> Yxx, 4 -> Yxy, 5 -> Yyy
+template <typename T>
+struct FieldY {
+  const cx<T>* y[6];
+};
+
+// Hessian component (a,b) -> (Y array, kz power)
+__device__ __forceinline__ void hess_src(int a, int b, int& idx, int& pz) {
+  if (a > b) {
+    int tmp = a;
+    a = b;
+    b = tmp;
+  }
+  if (a == 0 && b == 0) { idx = 3; pz = 0; }
+  else if (a == 1 && b == 1) { idx = 5; pz = 0; }
+  else if (a == 2 && b == 2) { idx = 0; pz = 2; }
+  else if (a == 0 && b == 1) { idx = 4; pz = 0; }
+  else if (a == 0 && b == 2) { idx = 1; pz = 1; }
+  else { idx = 2; pz = 1; }
+}
+
+template <typename T>
+struct KzMul {
+  int n, pz;
+  T kfac, sgn;
+  __device__ __forceinline__ T operator()(int k) const {
+    T g = kval<T>(k, n, kfac, 1);
+    return pz == 0 ? sgn : (pz == 1 ? sgn * g : sgn * g * g);
+  }
+};
+
+template <typename T, int K>
+struct ZRow {
+  static constexpr int E = RowCtx<T, K>::E;
+  RowCtx<T, K> rc;
+  bool act;
+  long long off;  // row * pitch
+  int hin, n;
+  T kfac;
+  // real-space Hessian component (a,b) of a field: lane's packed pairs
+  __device__ __forceinline__ void hess(cx<T> (&v)[E], const FieldY<T>& f, int a, int b) const {
+    int idx, pz;
+    hess_src(a, b, idx, pz);
+    rc.get(v, rc.c2r(f.y[idx] + off, hin, KzMul<T>{n, pz, kfac, T(-1)}));
+  }
+  __device__ __forceinline__ void plain(cx<T> (&v)[E], const cx<T>* p) const { rc.get(v, rc.c2r(p + off, hin)); }
+  __device__ __forceinline__ void emit(const cx<T> (&v)[E], cx<T>* p) const {
+    rc.r2c_store(v, p + off, hin, act, T(1));
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> pmul(cx<T> a, cx<T> b) {
+  return mk(a.x * b.x, a.y * b.y);
+}
+
+#define MFX_ZROW_PROLOGUE                                                           \
+  MFX_ROW_PROLOGUE(T, K)                                                            \
+  constexpr int EPT = RowCtx<T, K>::E;                                              \
+  ZRow<T, K> z{rc, act, rowc * pitch, hin, n, kfac};
+
+// stage A: 2LPT source S2 = sum_{i<j} (phi_ii phi_jj - phi_ij^2) (ic_generator.cc:471-475)
+// and, for 3LPT, the 3a intermediates I1 = phi_xx phi_yy - phi_xy^2, I2 = phi_xy phi_xz,
+// I3 = phi_yz^2, I4 = phi_xz^2 (ic_generator.cc:511-515)
+template <typename T, int K, bool THREE>
+__global__ void __launch_bounds__(64)
+    k_lpt_zA(FieldY<T> f, cx<T>* S2, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4, long long pitch, int hin, int n,
+             T kfac, long long nrows, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+  MFX_ZROW_PROLOGUE
+  cx<T> dxx[EPT], dyy[EPT], s2[EPT], w[EPT];
+  z.hess(dxx, f, 0, 0);
+  z.hess(dyy, f, 1, 1);
+  z.hess(w, f, 2, 2);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) s2[e] = pmul(dxx[e], dyy[e]) + pmul(dxx[e] + dyy[e], w[e]);
+  cx<T> oxy[EPT];
+  z.hess(oxy, f, 0, 1);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(oxy[e], oxy[e]);
+  if (THREE) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) w[e] = pmul(dxx[e], dyy[e]) - pmul(oxy[e], oxy[e]);
+    z.emit(w, I1);
+  }
+  z.hess(dxx, f, 0, 2);  // dxx now holds phi_xz
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(dxx[e], dxx[e]);
+  if (THREE) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) w[e] = pmul(oxy[e], dxx[e]);
+    z.emit(w, I2);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) w[e] = pmul(dxx[e], dxx[e]);
+    z.emit(w, I4);
+  }
+  z.hess(dyy, f, 1, 2);  // dyy now holds phi_yz
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(dyy[e], dyy[e]);
+  if (THREE) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) w[e] = pmul(dyy[e], dyy[e]);
+    z.emit(w, I3);
+  }
+  z.emit(s2, S2);
+}
+
+// stage C1: phi3 source = 3a + 3b (ic_generator.cc:511-527), streamed by
+// Hessian component:
+//   3a = I1 phi_zz + 2 I2 phi_yz - I3 phi_xx - I4 phi_yy
+//   3b = -5/7 tr(phi) tr(phi2) + 5/7 sum_i phi_ii phi2_ii + 10/7 sum_{i<j} phi_ij phi2_ij
+template <typename T, int K>
+__global__ void __launch_bounds__(64)
+    k_lpt_zC1(FieldY<T> f1, FieldY<T> f2, const cx<T>* I1, const cx<T>* I2, const cx<T>* I3, const cx<T>* I4,
+              cx<T>* out, long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
+              const cx<T>* __restrict__ twM) {
+  MFX_ZROW_PROLOGUE
+  cx<T> acc[EPT], tr1[EPT], tr2[EPT], a[EPT], b[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = tr1[e] = tr2[e] = mk(T(0), T(0));
+  const cx<T>* partner[6] = {I3, I4, I1, nullptr, nullptr, I2};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    constexpr int ca[6] = {0, 1, 2, 0, 0, 1}, cb[6] = {0, 1, 2, 1, 2, 2};
+    constexpr float pw[6] = {-1.f, -1.f, 1.f, 0.f, 0.f, 2.f};
+    z.hess(a, f1, ca[q], cb[q]);
+    z.hess(b, f2, ca[q], cb[q]);
+    const bool diag = q < 3;
+    const T w = diag ? T(5.0 / 7.0) : T(10.0 / 7.0);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      acc[e] = acc[e] + pmul(a[e], b[e]) * w;
+      if (diag) {
+        tr1[e] = tr1[e] + a[e];
+        tr2[e] = tr2[e] + b[e];
+      }
+    }
+    if (q != 3 && q != 4) {
+      z.plain(b, partner[q]);
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) acc[e] = acc[e] + pmul(a[e], b[e]) * T(pw[q]);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(tr1[e], tr2[e]) * T(5.0 / 7.0);
+  z.emit(acc, out);
+}
+
+// stage C2: A(3) component I (ic_generator.cc:534-545), p = I+1, pp = I+2:
+//   phi2_{I p} phi_{I pp} - phi2_{I pp} phi_{I p} + phi_{p pp}(phi2_{pp} - phi2_{pp pp})
+//   - phi2_{p pp}(phi_{p p} - phi_{pp pp})
+template <typename T, int K, int I>
+__global__ void __launch_bounds__(64)
+    k_lpt_zC2(FieldY<T> f1, FieldY<T> f2, cx<T>* out, long long pitch, int hin, int n, T kfac, long long nrows,
+              const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+  MFX_ZROW_PROLOGUE
+  constexpr int p = (I + 1) % 3, pp = (I + 2) % 3;
+  cx<T> acc[EPT], a[EPT], b[EPT], c[EPT];
+  z.hess(a, f2, I, p);
+  z.hess(b, f1, I, pp);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = pmul(a[e], b[e]);
+  z.hess(a, f2, I, pp);
+  z.hess(b, f1, I, p);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(a[e], b[e]);
+  z.hess(a, f1, p, pp);
+  z.hess(b, f2, p, p);
+  z.hess(c, f2, pp, pp);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] + pmul(a[e], b[e] - c[e]);
+  z.hess(a, f2, p, pp);
+  z.hess(b, f1, p, p);
+  z.hess(c, f1, pp, pp);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(a[e], b[e] - c[e]);
+  z.emit(acc, out);
+}
+
+template <typename T, int K>
+void launch_zA_K(Ctx& c, const FieldY<T>& f, cx<T>* S2, cx<T>* const* I, bool three, long long pitch, int hin, int n,
+                 T kfac, long long nrows, double bytes) {
+  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  LaunchScope ls(c, "lpt_zA", bytes);
+  if (three)
+    hipLaunchKernelGGL((k_lpt_zA<T, K, true>), gd, bd, 0, c.stream, f, S2, I[0], I[1], I[2], I[3], pitch, hin, n,
+                       kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  else
+    hipLaunchKernelGGL((k_lpt_zA<T, K, false>), gd, bd, 0, c.stream, f, S2, S2, S2, S2, S2, pitch, hin, n, kfac,
+                       nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch("lpt_zA");
+}
+
+template <typename T, int K>
+void launch_zC1_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, cx<T>* const* I, cx<T>* out, long long pitch,
+                  int hin, int n, T kfac, long long nrows, double bytes) {
+  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  LaunchScope ls(c, "lpt_zC1", bytes);
+  hipLaunchKernelGGL((k_lpt_zC1<T, K>), gd, bd, 0, c.stream, f1, f2, I[0], I[1], I[2], I[3], out, pitch, hin, n,
+                     kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch("lpt_zC1");
+}
+
+template <typename T, int K>
+void launch_zC2_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, int i, cx<T>* out, long long pitch, int hin,
+                  int n, T kfac, long long nrows, double bytes) {
+  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  LaunchScope ls(c, "lpt_zC2", bytes);
+  if (i == 0)
+    hipLaunchKernelGGL((k_lpt_zC2<T, K, 0>), gd, bd, 0, c.stream, f1, f2, out, pitch, hin, n, kfac, nrows,
+                       c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  else if (i == 1)
+    hipLaunchKernelGGL((k_lpt_zC2<T, K, 1>), gd, bd, 0, c.stream, f1, f2, out, pitch, hin, n, kfac, nrows,
+                       c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  else
+    hipLaunchKernelGGL((k_lpt_zC2<T, K, 2>), gd, bd, 0, c.stream, f1, f2, out, pitch, hin, n, kfac, nrows,
+                       c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch("lpt_zC2");
+}
+
+#define MFX_LPT_KSWITCH(K, CALL)                                          \
+  switch (K) {                                                            \
+    MFX_ROW_LENGTHS(CALL)                                                 \
+    default:                                                              \
+      throw Error(1, "unsupported padded length " + std::to_string(2 * (K))); \
+  }
+
+// ------------------------------------------------------------------ host orchestration
+struct Layout {
+  int n, m, h, hp;
+  size_t t1, t2;  // elements of one x-stage slot (m*n*hp) and one y/z slot (m*m*hp)
+};
+
+static Layout layout(int n) {
+  Layout l;
+  l.n = n;
+  l.m = 3 * n / 2;
+  l.h = n / 2 + 1;
+  l.hp = conv_pitch(n);
+  l.t1 = (size_t)l.m * n * l.hp;
+  l.t2 = (size_t)l.m * l.m * l.hp;
+  return l;
+}
+
+// slots: 3 x-stage (also the y-forward output), 16 y/z, 4 n-grid intermediates
+size_t lpt_workspace_bytes(int n, int prec, int order) {
+  Layout l = layout(n);
+  const size_t csz = 2 * (size_t)prec;
+  const size_t ngrid = (size_t)n * n * (n / 2 + 1);
+  if (order < 3) return csz * (3 * l.t1 + 7 * l.t2);
+  return csz * (3 * l.t1 + 16 * l.t2 + 4 * ngrid);
+}
+
+template <typename T>
+struct Engine {
+  Ctx& c;
+  Layout l;
+  T kfac, s_in, s_out;
+  cx<T>* T1[3];
+  cx<T>* T2[16];
+  cx<T>* NI[4];
+  double cs, bN, bT1, bT2;
+
+  Engine(Ctx& ctx, int n, double boxlen, int order, void* ws) : c(ctx), l(layout(n)) {
+    kfac = T(2.0 * M_PI / boxlen);
+    s_in = T(1.0 / std::sqrt((double)n * n * n));
+    s_out = T(std::sqrt((double)n * n * n) / ((double)l.m * l.m * l.m));
+    cx<T>* p = reinterpret_cast<cx<T>*>(ws);
+    for (auto& q : T1) {
+      q = p;
+      p += l.t1;
+    }
+    const int n2 = order < 3 ? 7 : 16;
+    for (int q = 0; q < 16; ++q) {
+      T2[q] = q < n2 ? p : nullptr;
+      if (q < n2) p += l.t2;
+    }
+    for (auto& q : NI) {
+      q = order < 3 ? nullptr : p;
+      if (order >= 3) p += (size_t)n * n * (n / 2 + 1);
+    }
+    cs = sizeof(cx<T>);
+    bN = cs * (double)n * n * l.h;
+    bT1 = cs * (double)l.m * n * l.h;
+    bT2 = cs * (double)l.m * l.m * l.h;
+  }
+
+  // x and y stages of a field: Hessian rows (six Y arrays) or one plain Y0
+  void inverse(const cx<T>* F, bool hessian, cx<T>* const* Y) {
+    const int n = l.n, m = l.m, hp = l.hp;
+    const long long HN = l.h;
+    OutSet<T> xo{};
+    xo.nout = hessian ? 3 : 1;
+    for (int q = 0; q < xo.nout; ++q) {
+      xo.p[q] = T1[q];
+      xo.pw[q] = q;
+    }
+    launch_pad_multi<T>(c, m, F, HN, (long long)n * HN, n, s_in, kfac, xo, hp, (long long)n * hp, l.h, n,
+                        "lpt_x_inv", bN + xo.nout * bT1);
+    // y stage: line along j of T1 [x][j][kz], batch x, output [x][y][kz]
+    auto ystage = [&](int src, std::initializer_list<std::pair<int, int>> outs) {
+      OutSet<T> yo{};
+      yo.nout = 0;
+      for (auto& o : outs) {
+        yo.p[yo.nout] = Y[o.first];
+        yo.pw[yo.nout] = o.second;
+        ++yo.nout;
+      }
+      launch_pad_multi<T>(c, m, T1[src], (long long)n * hp, hp, n, T(1), kfac, yo, (long long)m * hp, hp, l.h, m,
+                          "lpt_y_inv", bT1 + yo.nout * bT2);
+    };
+    if (hessian) {
+      ystage(0, {{0, 0}, {2, 1}, {5, 2}});  // Y0, Yy, Yyy
+      ystage(1, {{1, 0}, {4, 1}});          // Yx, Yxy
+      ystage(2, {{3, 0}});                  // Yxx
+    } else {
+      ystage(0, {{0, 0}});
+    }
+  }
+
+  // y and x forward transforms of a z-stage output into an n-grid
+  void forward(cx<T>* Z, cx<T>* out, bool invlap) {
+    const int n = l.n, m = l.m, hp = l.hp;
+    const long long HN = l.h;
+    launch_col<T, false>(c, m, SrcPlain<T>{Z, (long long)m * hp, hp}, DstTrunc<T>{T1[0], (long long)n * hp, hp, n},
+                         l.h, m, "lpt_y_fwd", bT2 + bT1);
+    DstTruncOp<T> d{out, HN, (long long)n * HN, n, OP_ASSIGN, T(0), s_out, invlap ? 1 : 0, kfac};
+    launch_col<T, false>(c, m, SrcPlain<T>{T1[0], hp, (long long)n * hp}, d, l.h, n, "lpt_x_fwd", bT1 + bN);
+  }
+
+  FieldY<T> fy(int base) const {
+    FieldY<T> f;
+    for (int q = 0; q < 6; ++q) f.y[q] = T2[base + q];
+    return f;
+  }
+};
 
 template <typename T>
 void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x, cx<T>* a3y, cx<T>* a3z, int n,
                  double boxlen, int order, void* ws) {
-  throw Error(1, "fused LPT engine not available");
+  MFX_REQUIRE(n % 2 == 0, "Orszag convolver requires an even grid size");
+  Engine<T> E(c, n, boxlen, order, ws);
+  const Layout& l = E.l;
+  const int K = l.m / 2;
+  const long long nrows = (long long)l.m * l.m;
+  const bool three = order > 2;
+  // phi: Hessian rows into T2[0..5]
+  E.inverse(phi, true, E.T2);
+  const FieldY<T> fphi = E.fy(0);
+  // stage A: S2 -> T2[6]; I1..I4 -> T2[7..10]
+  cx<T>* Iz[4] = {E.T2[7], E.T2[8], E.T2[9], E.T2[10]};
+  {
+    const double bytes = 6 * E.bT2 + (three ? 5 : 1) * E.bT2;
+#define MFX_CASE(KK)                                                                                        \
+  case KK:                                                                                                  \
+    launch_zA_K<T, KK>(c, fphi, E.T2[6], Iz, three, l.hp, l.h, n, E.kfac, nrows, bytes); \
+    break;
+    MFX_LPT_KSWITCH(K, MFX_CASE)
+#undef MFX_CASE
+  }
+  E.forward(E.T2[6], phi2, true);  // phi2 = -S2/k^2 (ic_generator.cc:488)
+  if (!three) return;
+  for (int q = 0; q < 4; ++q) E.forward(Iz[q], E.NI[q], false);
+  // phi2 Hessian rows -> T2[6..11]; I1..I4 rows -> T2[12..15]
+  E.inverse(phi2, true, E.T2 + 6);
+  const FieldY<T> fphi2 = E.fy(6);
+  cx<T>* Ir[4];
+  for (int q = 0; q < 4; ++q) {
+    Ir[q] = E.T2[12 + q];
+    cx<T>* Y[6] = {Ir[q], nullptr, nullptr, nullptr, nullptr, nullptr};
+    E.inverse(E.NI[q], false, Y);
+  }
+  // stage C1: phi3 source over I1's rows
+  {
+    const double bytes = 16 * E.bT2 + E.bT2;
+#define MFX_CASE(KK)                                                                                  \
+  case KK:                                                                                            \
+    launch_zC1_K<T, KK>(c, fphi, fphi2, Ir, Ir[0], l.hp, l.h, n, E.kfac, nrows, bytes); \
+    break;
+    MFX_LPT_KSWITCH(K, MFX_CASE)
+#undef MFX_CASE
+  }
+  E.forward(Ir[0], phi3, true);
+  // stage C2: A3 components over I2..I4's rows
+  cx<T>* A3[3] = {a3x, a3y, a3z};
+  for (int i = 0; i < 3; ++i) {
+    const double bytes = 10 * E.bT2 + E.bT2;
+#define MFX_CASE(KK)                                                                              \
+  case KK:                                                                                        \
+    launch_zC2_K<T, KK>(c, fphi, fphi2, i, Ir[1 + i], l.hp, l.h, n, E.kfac, nrows, bytes); \
+    break;
+    MFX_LPT_KSWITCH(K, MFX_CASE)
+#undef MFX_CASE
+    E.forward(Ir[1 + i], A3[i], true);
+  }
 }
 
 template void lpt_sources<float>(Ctx&, const cx<float>*, cx<float>*, cx<float>*, cx<float>*, cx<float>*, cx<float>*,
                                  int, double, int, void*);
 template void lpt_sources<double>(Ctx&, const cx<double>*, cx<double>*, cx<double>*, cx<double>*, cx<double>*,
                                   cx<double>*, int, double, int, void*);
+
 }  // namespace mfx

@@ -76,10 +76,27 @@ double orc_gauss_ratio(orc_mt_t *s) {
  *   subtraction at :781-793 passes the OpenMP reduction variable `sum`, which
  *   is private and zero-initialised inside that parallel loop, so it subtracts
  *   0; it is therefore a no-op and is not restated.
- *   zeromean (:44-65): subtract the global mean (summed in (i,j,k) order here;
- *   the reference's OpenMP reduction order depends on the thread count).
+ *   zeromean (:44-65): subtract the global mean (a double-double sum, see
+ *   dd_add below).
  *   Fill_Grid (random_music.cc:92-116) copies cell (i,j,k) of the cube
  *   structure into relem(i,j,k). */
+/* Knuth TwoSum accumulation (double-double): the reference sums the field in
+ * an OpenMP reduction whose order depends on the thread count
+ * (random_music_wnoise_generator.cc:48-55); the mean is defined here as the
+ * correctly rounded exact sum divided by n^3, which every summation order
+ * approximates to a few ulp. */
+typedef struct { double hi, lo; } dd_t;
+static dd_t dd_add(dd_t a, double b) {
+  double s = a.hi + b;
+  double bb = s - a.hi;
+  double err = (a.hi - (s - bb)) + (b - bb);
+  double lo = a.lo + err;
+  double hi = s + lo;
+  lo = lo - (hi - s);
+  dd_t r = {hi, lo};
+  return r;
+}
+
 static void music1_cube_values(long baseseed, int nc, int cs, size_t icube, double *vals) {
   orc_mt_t st;
   long cubeseed = baseseed + (long)icube;
@@ -112,10 +129,11 @@ void orc_music1_fill(double *data, int n, long baseseed, int cubesize) {
     }
     free(vals);
   }
-  double mean = 0.0;
+  dd_t acc = {0.0, 0.0};
   for (size_t i = 0; i < (size_t)n; ++i)
     for (size_t j = 0; j < (size_t)n; ++j)
-      for (size_t k = 0; k < (size_t)n; ++k) mean += data[(i * n + j) * np + k];
+      for (size_t k = 0; k < (size_t)n; ++k) acc = dd_add(acc, data[(i * n + j) * np + k]);
+  double mean = acc.hi + acc.lo;
   mean *= 1.0 / (double)((size_t)n * n * n);
   for (size_t i = 0; i < (size_t)n; ++i)
     for (size_t j = 0; j < (size_t)n; ++j) {
@@ -148,10 +166,11 @@ void orc_music1_fill_f32(float *data, int n, long baseseed, int cubesize) {
     }
     free(vals);
   }
-  double mean = 0.0;
+  dd_t acc = {0.0, 0.0};
   for (size_t i = 0; i < (size_t)n; ++i)
     for (size_t j = 0; j < (size_t)n; ++j)
-      for (size_t k = 0; k < (size_t)n; ++k) mean += (double)data[(i * n + j) * np + k];
+      for (size_t k = 0; k < (size_t)n; ++k) acc = dd_add(acc, (double)data[(i * n + j) * np + k]);
+  double mean = acc.hi + acc.lo;
   mean *= 1.0 / (double)((size_t)n * n * n);
   for (size_t i = 0; i < (size_t)n; ++i)
     for (size_t j = 0; j < (size_t)n; ++j) {

@@ -1,0 +1,205 @@
+"""nLPT source terms and the full IC pipeline on the GPU against the oracle's
+restatement of ic_generator::run (src/ic_generator.cc:249-944)."""
+import numpy as np
+import pytest
+
+from conftest import rel_max, rel_rms
+from helpers import random_kspace_field
+
+pytestmark = pytest.mark.gpu
+
+BOX = 100.0
+
+
+def gpu_lpt_reference_sequence(M, conv, phi, phi2, phi3, A3, order):
+    """The reference's convolve call sequence (ic_generator.cc:471-545), through
+    the drop-in OrszagConvolver interface."""
+    op = M.op
+    conv.convolve_SumOfHessians(phi, (0, 0), phi, (1, 1), (2, 2), op.assign_to(phi2))
+    conv.convolve_Hessians(phi, (1, 1), phi, (2, 2), op.add_to(phi2))
+    conv.convolve_Hessians(phi, (0, 1), phi, (0, 1), op.subtract_from(phi2))
+    conv.convolve_Hessians(phi, (0, 2), phi, (0, 2), op.subtract_from(phi2))
+    conv.convolve_Hessians(phi, (1, 2), phi, (1, 2), op.subtract_from(phi2))
+    phi2.apply_InverseLaplacian()
+    if order < 3:
+        return
+    conv.convolve_Hessians(phi, (0, 0), phi, (1, 1), phi, (2, 2), op.assign_to(phi3))
+    conv.convolve_Hessians(phi, (0, 1), phi, (0, 2), phi, (1, 2), op.multiply_add_to(phi3, 2.0))
+    conv.convolve_Hessians(phi, (1, 2), phi, (1, 2), phi, (0, 0), op.subtract_from(phi3))
+    conv.convolve_Hessians(phi, (0, 2), phi, (0, 2), phi, (1, 1), op.subtract_from(phi3))
+    conv.convolve_Hessians(phi, (0, 1), phi, (0, 1), phi, (2, 2), op.subtract_from(phi3))
+    conv.convolve_SumOfHessians(phi, (0, 0), phi2, (1, 1), (2, 2), op.multiply_add_to(phi3, -5.0 / 7.0))
+    conv.convolve_SumOfHessians(phi, (1, 1), phi2, (2, 2), (0, 0), op.multiply_add_to(phi3, -5.0 / 7.0))
+    conv.convolve_SumOfHessians(phi, (2, 2), phi2, (0, 0), (1, 1), op.multiply_add_to(phi3, -5.0 / 7.0))
+    conv.convolve_Hessians(phi, (0, 1), phi2, (0, 1), op.multiply_add_to(phi3, +10.0 / 7.0))
+    conv.convolve_Hessians(phi, (0, 2), phi2, (0, 2), op.multiply_add_to(phi3, +10.0 / 7.0))
+    conv.convolve_Hessians(phi, (1, 2), phi2, (1, 2), op.multiply_add_to(phi3, +10.0 / 7.0))
+    phi3.apply_InverseLaplacian()
+    for idim in range(3):
+        p, pp = (idim + 1) % 3, (idim + 2) % 3
+        conv.convolve_Hessians(phi2, (idim, p), phi, (idim, pp), op.assign_to(A3[idim]))
+        conv.convolve_Hessians(phi2, (idim, pp), phi, (idim, p), op.subtract_from(A3[idim]))
+        conv.convolve_DifferenceOfHessians(phi, (p, pp), phi2, (p, p), (pp, pp), op.add_to(A3[idim]))
+        conv.convolve_DifferenceOfHessians(phi2, (p, pp), phi, (p, p), (pp, pp), op.subtract_from(A3[idim]))
+        A3[idim].apply_InverseLaplacian()
+
+
+def _kgrid(M, ctx, n, prec):
+    return M.Grid_FFT(ctx, n, BOX, prec, initialspace=M.kspace_id)
+
+
+@pytest.mark.parametrize("n,order", [(32, 3), (64, 2)])
+def test_lpt_sequence_vs_oracle(ctx, orc, n, order):
+    import monofonic_amd as M
+    print(f"lpt sequence n={n} order={order}", flush=True)
+    phi_h = random_kspace_field(orc, n, 5, slope=-3.0)
+    phi = _kgrid(M, ctx, n, "double")
+    phi.upload(phi_h)
+    phi2 = _kgrid(M, ctx, n, "double")
+    phi3 = _kgrid(M, ctx, n, "double")
+    A3 = [_kgrid(M, ctx, n, "double") for _ in range(3)]
+    conv = M.OrszagConvolver(ctx, n, BOX, "double")
+    gpu_lpt_reference_sequence(M, conv, phi, phi2, phi3, A3, order)
+    r2, r3, rA = orc.lpt_terms(phi_h, BOX, order)
+    assert rel_rms(phi2.kdata(), orc.kview(r2)) < 1e-12
+    assert rel_max(phi2.kdata(), orc.kview(r2)) < 1e-10
+    if order > 2:
+        assert rel_rms(phi3.kdata(), orc.kview(r3)) < 1e-12
+        assert rel_max(phi3.kdata(), orc.kview(r3)) < 1e-10
+        for d in range(3):
+            assert rel_rms(A3[d].kdata(), orc.kview(rA[d])) < 1e-12
+            assert rel_max(A3[d].kdata(), orc.kview(rA[d])) < 1e-10
+
+
+def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos):
+    """ic_generator::run restated on the CPU: MUSIC1 noise, phi(1..3), A(3),
+    growth scaling and FP64 sc-lattice positions/velocities"""
+    a = 1.0 / (1.0 + zstart)
+    D = cos.Dplus(a)
+    vfac = cos.vfact(a)
+    volfac = (boxlen / n / 2.0 / np.pi) ** 1.5
+    wn = orc.music1_fill(n, seed)
+    orc.fft_r2c(wn)
+    orc.noise_normalise(wn, volfac)
+    phi = np.zeros_like(wn)
+    orc.phi1(phi, wn, boxlen, cos.amplitude_table(n, boxlen))
+    phi2 = phi3 = None
+    A3 = (None, None, None)
+    if order > 1:
+        phi2, phi3, A3 = orc.lpt_terms(phi, boxlen, order)
+    orc.scale(phi, -D)
+    if order > 1:
+        orc.scale(phi2, -3.0 / 7.0 * D * D)
+    if order > 2:
+        orc.scale(phi3, 1.0 / 3.0 * D ** 3)
+        for x in A3:
+            orc.scale(x, 1.0 / 7.0 * D ** 3)
+    out = []
+    for kind in range(2):
+        for idim in range(3):
+            f = orc.assemble_component(phi, phi2, phi3, A3, boxlen, order, kind, idim, vfac, 1.0)
+            if kind == 0:
+                out.append(orc.set_positions(f, idim, 1.0).reshape(n, n, n))
+            else:
+                out.append(f[:, :, :n].copy())
+    return out, D, vfac
+
+
+def lattice_q(n, idim):
+    q = np.arange(n) / n
+    sh = [1, 1, 1]
+    sh[idim] = n
+    return np.broadcast_to(q.reshape(sh), (n, n, n))
+
+
+@pytest.mark.parametrize("n,order,zstart", [(32, 3, 24.0), (64, 2, 50.0)])
+def test_icgen_fp64_matches_oracle(ctx, orc, n, order, zstart):
+    import monofonic_amd as M
+    print(f"icgen fp64 n={n} order={order}", flush=True)
+    ic = M.ICGenerator(ctx, n, BOX, zstart, LPTorder=order, precision="double", generator="MUSIC1", seed=4711,
+                       fused=False)
+    ic.run()
+    cos = orc.Cosmology()
+    ref, D, vfac = oracle_pipeline(orc, n, BOX, zstart, order, 4711, cos)
+    dg, vg = ic.scalars()
+    assert abs(dg / D - 1) < 1e-12 and abs(vg / vfac - 1) < 1e-12
+    for w in range(6):
+        got = ic.particles(w)
+        if w < 3:
+            psi_g = got - lattice_q(n, w)
+            psi_r = ref[w] - lattice_q(n, w)
+        else:
+            psi_g, psi_r = got, ref[w]
+        # tolerance relative to the field RMS (sigma_Psi)
+        assert rel_rms(psi_g, psi_r) < 1e-11, w
+        assert rel_max(psi_g, psi_r) < 1e-10, w
+
+
+def test_icgen_fp32_bulk_vs_fp64(ctx):
+    """FP32 FFT/convolution bulk with FP64 tables and assembly stays within
+    1e-5 RMS of the FP64 path (displacements relative to sigma_Psi)."""
+    import monofonic_amd as M
+    n = 64
+    res = {}
+    for prec in ("double", "float"):
+        ic = M.ICGenerator(ctx, n, BOX, 24.0, LPTorder=3, precision=prec, generator="MUSIC1", seed=99, fused=True)
+        ic.run()
+        res[prec] = [ic.particles(w) for w in range(6)]
+    for w in range(6):
+        a, b = res["float"][w], res["double"][w]
+        if w < 3:
+            a = a - lattice_q(n, w)
+            b = b - lattice_q(n, w)
+        assert rel_rms(a, b) < 1e-5, w
+
+
+@pytest.mark.parametrize("n,order", [(32, 3), (48, 3), (64, 2)])
+def test_fused_lpt_engine_vs_oracle(ctx, orc, n, order):
+    """mfx_lpt_sources (Hessian-tensor engine) equals the reference's 5/33
+    convolve calls, restated by the oracle"""
+    import monofonic_amd as M
+    print(f"fused lpt n={n} order={order}", flush=True)
+    phi_h = random_kspace_field(orc, n, 8, slope=-3.0)
+    phi = _kgrid(M, ctx, n, "double")
+    phi.upload(phi_h)
+    phi2, phi3 = _kgrid(M, ctx, n, "double"), _kgrid(M, ctx, n, "double")
+    A3 = [_kgrid(M, ctx, n, "double") for _ in range(3)]
+    conv = M.OrszagConvolver(ctx, n, BOX, "double")
+    conv.lpt_sources(phi, phi2, phi3, A3, order)
+    r2, r3, rA = orc.lpt_terms(phi_h, BOX, order)
+    pairs = [(phi2, r2)] + ([(phi3, r3)] + list(zip(A3, rA)) if order > 2 else [])
+    for g, r in pairs:
+        assert rel_rms(g.kdata(), orc.kview(r)) < 1e-12
+        assert rel_max(g.kdata(), orc.kview(r)) < 1e-10
+
+
+def test_fused_fp32_vs_reference_sequence_fp32(ctx, orc):
+    import monofonic_amd as M
+    n = 64
+    phi_h = random_kspace_field(orc, n, 9, slope=-3.0)
+    res = {}
+    for fused in (True, False):
+        phi = _kgrid(M, ctx, n, "float")
+        phi.upload(phi_h.astype(np.float32))
+        out = [_kgrid(M, ctx, n, "float") for _ in range(5)]
+        conv = M.OrszagConvolver(ctx, n, BOX, "float")
+        if fused:
+            conv.lpt_sources(phi, out[0], out[1], out[2:], 3)
+        else:
+            gpu_lpt_reference_sequence(M, conv, phi, out[0], out[1], out[2:], 3)
+        res[fused] = [g.kdata() for g in out]
+    for a, b in zip(res[True], res[False]):
+        assert rel_rms(a, b) < 1e-5
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_icgen_fused_matches_oracle(ctx, orc, fused):
+    import monofonic_amd as M
+    n = 32
+    ic = M.ICGenerator(ctx, n, BOX, 24.0, LPTorder=3, precision="double", generator="MUSIC1", seed=31, fused=fused)
+    ic.run()
+    ref, _, _ = oracle_pipeline(orc, n, BOX, 24.0, 3, 31, orc.Cosmology())
+    for w in range(6):
+        a = ic.particles(w) - (lattice_q(n, w) if w < 3 else 0)
+        b = ref[w] - (lattice_q(n, w) if w < 3 else 0)
+        assert rel_rms(a, b) < 1e-11, w

@@ -1,0 +1,129 @@
+"""Pins for the oracle's RNGs (known-answer tests) and cosmology (scipy)."""
+import numpy as np
+import pytest
+import scipy.integrate as si
+import scipy.stats as ss
+
+# Threefry4x64-20 KATs held by the reference (external/panphasia_ho/uniform_rand_threefry4x64.c:94-147)
+THREEFRY_KAT = [
+    ([0] * 4, [0] * 4, [0x09218ebde6c85537, 0x55941f5266d86105, 0x4bd25e16282434dc, 0xee29ec846bd2e40b]),
+    ([2 ** 64 - 1] * 4, [2 ** 64 - 1] * 4,
+     [0x29c24097942bba1b, 0x0371bbfb0f6f4e11, 0x3c231ffa33f83a1c, 0xcd29113fde32d168]),
+    ([0x243f6a8885a308d3, 0x13198a2e03707344, 0xa4093822299f31d0, 0x082efa98ec4e6c89],
+     [0x452821e638d01377, 0xbe5466cf34e90c6c, 0xbe5466cf34e90c6c, 0xc0ac29b7c97c50dd],
+     [0xa7e8fde591651bd9, 0xbaafd0c30138319b, 0x84a5c1a729e685b9, 0x901d406ccebc1ba4]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,res", THREEFRY_KAT)
+def test_threefry_kat(orc, ctr, key, res):
+    assert orc.threefry4x64_20(ctr, key) == res
+
+
+def test_mt19937_known_answers(orc):
+    # C++ standard [rand.predef]: 10000th output of std::mt19937 (seed 5489)
+    assert orc.mt19937(5489, 10000)[-1] == 4123659995
+    # GSL's own regression value for gsl_rng_mt19937, seed 4357, 1000th output
+    assert orc.mt19937(4357, 1000)[-1] == 1186927261
+    # GSL seeding: seed 0 is replaced by 4357
+    assert np.array_equal(orc.mt19937(0, 50), orc.mt19937(4357, 50))
+
+
+def test_ranlxd1_known_answer(orc):
+    # GSL's regression value for gsl_rng_ranlxd1, seed 1, 10000th output of
+    # gsl_rng_get (= uniform * 2^32 truncated)
+    u = orc.ranlxd1(1, 10000)
+    assert int(u[-1] * 4294967296.0) == 1998227290
+    # 48-bit exact outputs
+    assert np.all(u * 2.0 ** 48 == np.floor(u * 2.0 ** 48))
+
+
+def test_ratio_method_distribution(orc):
+    g = orc.gauss_ratio(1234, 200000)
+    assert abs(g.mean()) < 0.01
+    assert abs(g.std() - 1) < 0.01
+    assert ss.kstest(g, "norm").pvalue > 1e-3
+
+
+def test_music1_fill_properties(orc):
+    n = 32
+    a = orc.music1_fill(n, 77)
+    x = a[:, :, :n]
+    assert abs(x.mean()) < 1e-15
+    assert abs(x.std() - 1) < 0.02
+    assert np.all(a[:, :, n:] == 0)
+
+
+def test_music1_cube_stream(orc):
+    """cube 0 of a 32^3 grid is the ratio-method stream of MT19937(seed) minus
+    the global mean"""
+    n = 32
+    a = orc.music1_fill(n, 9)
+    raw = orc.gauss_ratio(9, n ** 3)
+    mean = raw.mean()
+    assert np.max(np.abs(a[:, :, :n].ravel() - (raw - mean))) < 1e-15
+
+
+def test_ngenic_hermitian(orc):
+    n = 16
+    a = orc.ngenic_fill(n, orc.ngenic_seedtable(n, 5))
+    k = orc.kview(a)
+    assert k[0, 0, 0] == 0
+    assert np.all(k[n // 2] == 0) and np.all(k[:, n // 2] == 0) and np.all(k[:, :, n // 2] == 0)
+    P = k[:, :, 0]
+    idx = (-np.arange(n)) % n
+    Q = np.conj(P[idx][:, idx])
+    mask = np.ones((n, n), bool)
+    mask[n // 2, :] = mask[:, n // 2] = False
+    assert np.array_equal(P[mask], Q[mask])
+    assert np.allclose(np.abs(k[1:3, 1:3, 1:3]) ** 0, 1)
+
+
+# ------------------------------------------------------------------ cosmology
+def test_eisenstein_hu_limits(orc):
+    c = orc.Cosmology()
+    assert abs(c.transfer(1e-6) - 1) < 1e-4
+    assert c.transfer(1.0) < 0.01
+
+
+def test_sigma8_normalisation_vs_scipy(orc):
+    c = orc.Cosmology()
+
+    def integrand(lk):
+        k = np.exp(lk)
+        x = 8 * k
+        w = 1 - 0.1 * x * x if x < 1e-3 else 3 * (np.sin(x) - x * np.cos(x)) / x ** 3
+        return k * k * w * w * k ** 0.96 * c.transfer(k) ** 2 * k
+
+    pts = np.linspace(np.log(1e-4), np.log(1e4), 400)
+    s = sum(si.quad(integrand, pts[i], pts[i + 1], epsabs=0, epsrel=1e-12, limit=200)[0] for i in range(len(pts) - 1))
+    s8 = np.sqrt(4 * np.pi * s)
+    assert abs(c.pnorm - 0.81 ** 2 / s8 ** 2) < 1e-8 * c.pnorm
+
+
+def test_growth_vs_scipy(orc):
+    c = orc.Cosmology()
+    Om = 0.3
+
+    def rhs(lna, y):
+        a = np.exp(lna)
+        H = c.H(a)
+        D, dD = y  # dD = dD/dlna
+        # D'' + (2 + dlnH/dlna) D' = 1.5 Om H0^2/(a^3 H^2) D
+        eps = 1e-6
+        dlnH = (np.log(c.H(a * (1 + eps))) - np.log(c.H(a * (1 - eps)))) / (2 * eps)
+        return [dD, -(2 + dlnH) * dD + 1.5 * Om * (70.0 ** 2) / (a ** 3 * H ** 2) * D]
+
+    a0 = 1e-6
+    sol = si.solve_ivp(rhs, [np.log(a0), 0.0], [a0, a0], rtol=1e-11, atol=1e-20, dense_output=True)
+    D = lambda a: sol.sol(np.log(a))[0]
+    for a in (1 / 51, 1 / 25, 0.5):
+        assert abs(c.Dplus(a) - D(a) / D(1.0)) < 2e-6 * c.Dplus(a)
+
+
+def test_growth_eds_limit(orc):
+    """Omega_m = 1 without radiation: D+ = a"""
+    c = orc.Cosmology(Omega_m=1.0, Omega_b=0.045, zero_radiation=True)
+    for a in (0.01, 0.1, 0.5):
+        assert abs(c.Dplus(a) - a) < 1e-7 * a
+        assert abs(c.f(a) - 1) < 1e-7
