@@ -110,6 +110,25 @@ struct SrcPad {
   }
 };
 
+// zero-padded line times g(i)^pw with g the Nyquist-zeroed gradient wave
+// number of the source index i (one Hessian factor per transformed axis)
+template <typename T>
+struct SrcPadPow {
+  const cx<T>* p;
+  long long sb, sp;
+  int n, pw;
+  T scale, kfac;
+  __device__ __forceinline__ cx<T> load(int b, int col, int pos) const {
+    const int nh = n >> 1;
+    const bool pad = (pos > nh) & (pos <= n);
+    const int i = pad ? 0 : (pos <= nh ? pos : pos - nh);
+    const cx<T> x = p[(long long)b * sb + (long long)i * sp + col];
+    const T g = kval<T>(i, n, kfac, 1);
+    const T f = pw == 0 ? scale : (pw == 1 ? scale * g : scale * g * g);
+    return pad ? mk(T(0), T(0)) : x * f;
+  }
+};
+
 // keep the n retained outputs of an M-line (convolution.hh:629-634)
 template <typename T>
 struct DstTrunc {
@@ -181,6 +200,23 @@ struct DstTruncOp {
 };
 
 // ------------------------------------------------------------------ column kernel
+// Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.  The
+// linear workgroup id is remapped so that every XCD processes one contiguous
+// range of (batch, tile) work items: neighbouring column tiles, which share
+// 128-byte lines, then run on the same XCD close in time.  gridDim.x is the
+// number of tiles, gridDim.y the batch.
+__device__ __forceinline__ void xcd_tile(int& tile, int& b) {
+  constexpr int NX = 8;
+  const int ntiles = gridDim.x;
+  const long long total = (long long)gridDim.x * gridDim.y;
+  const long long id = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+  const long long per = total / NX, rem = total % NX;
+  const long long x = id % NX, local = id / NX;
+  const long long w = (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+  tile = (int)(w % ntiles);
+  b = (int)(w / ntiles);
+}
+
 template <typename T, int L, bool INV, class Src, class Dst>
 __global__ void __launch_bounds__((ColCfg<T, L>::C * ColCfg<T, L>::TPS))
     k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
@@ -188,8 +224,9 @@ __global__ void __launch_bounds__((ColCfg<T, L>::C * ColCfg<T, L>::TPS))
   constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
   __shared__ cx<T> lds[L * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
-  const int b = blockIdx.y;
+  int tile, b;
+  xcd_tile(tile, b);
+  const int col = tile * C + c;
   const bool act = col < ncols;
   // lanes past the last column load column ncols-1 and store nothing
   const int colc = act ? col : ncols - 1;

@@ -396,25 +396,18 @@ struct Engine {
   void inverse(const cx<T>* F, bool hessian, cx<T>* const* Y) {
     const int n = l.n, m = l.m, hp = l.hp;
     const long long HN = l.h;
-    OutSet<T> xo{};
-    xo.nout = hessian ? 3 : 1;
-    for (int q = 0; q < xo.nout; ++q) {
-      xo.p[q] = T1[q];
-      xo.pw[q] = q;
-    }
-    launch_pad_multi<T>(c, m, F, HN, (long long)n * HN, n, s_in, kfac, xo, hp, (long long)n * hp, l.h, n,
-                        "lpt_x_inv", bN + xo.nout * bT1);
+    // x stage: line along i of the n-grid [i][j][kz], batch j, output [x][j][kz];
+    // one launch per power of g_x (each re-reads the n-grid source, which is
+    // cheaper than holding three transforms in registers)
+    const int nx = hessian ? 3 : 1;
+    for (int p = 0; p < nx; ++p)
+      launch_col<T, true>(c, m, SrcPadPow<T>{F, HN, (long long)n * HN, n, p, s_in, kfac},
+                          DstPlain<T>{T1[p], hp, (long long)n * hp, T(1)}, l.h, n, "lpt_x_inv", bN + bT1);
     // y stage: line along j of T1 [x][j][kz], batch x, output [x][y][kz]
     auto ystage = [&](int src, std::initializer_list<std::pair<int, int>> outs) {
-      OutSet<T> yo{};
-      yo.nout = 0;
-      for (auto& o : outs) {
-        yo.p[yo.nout] = Y[o.first];
-        yo.pw[yo.nout] = o.second;
-        ++yo.nout;
-      }
-      launch_pad_multi<T>(c, m, T1[src], (long long)n * hp, hp, n, T(1), kfac, yo, (long long)m * hp, hp, l.h, m,
-                          "lpt_y_inv", bT1 + yo.nout * bT2);
+      for (auto& o : outs)
+        launch_col<T, true>(c, m, SrcPadPow<T>{T1[src], (long long)n * hp, hp, n, o.second, T(1), kfac},
+                            DstPlain<T>{Y[o.first], (long long)m * hp, hp, T(1)}, l.h, m, "lpt_y_inv", bT1 + bT2);
     };
     if (hessian) {
       ystage(0, {{0, 0}, {2, 1}, {5, 2}});  // Y0, Yy, Yyy

@@ -141,14 +141,25 @@ struct RowCtx {
   // row z[n] = x[2n] + i x[2n+1], n < K.  Unnormalised.
   template <class Mul>
   __device__ __forceinline__ cx<T>* c2r(const cx<T>* s, int hin, Mul mul) const {
-    // unconditional loads from clamped indices: all of a lane's loads in flight
     cx<T> x[E + 1];
+    load_raw(x, s, hin);
+    return c2r_raw(x, hin, mul);
+  }
+
+  // the lane's entries k = lane + P*e (e <= E) of a half-spectrum row, loaded
+  // unconditionally from clamped indices so that all loads are in flight
+  __device__ __forceinline__ void load_raw(cx<T> (&x)[E + 1], const cx<T>* s, int hin) const {
 #pragma unroll
     for (int e = 0; e <= E; ++e) {
       const int k = lane + P * e;
       const int kc = k < hin ? k : hin - 1;
       x[e] = s[kc];
     }
+  }
+
+  // c2r of a row whose entries were fetched by load_raw
+  template <class Mul>
+  __device__ __forceinline__ cx<T>* c2r_raw(const cx<T> (&x)[E + 1], int hin, Mul mul) const {
     wave_sync();  // the previous transform's result buffer may still be read
 #pragma unroll
     for (int e = 0; e <= E; ++e) {
