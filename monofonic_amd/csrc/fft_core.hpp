@@ -112,16 +112,28 @@ struct BF<8, INV, T> {
   }
 };
 
-// Returns p unchanged but hides its value from the optimiser.  Kernels that
-// run several transforms of the same length would otherwise have the
-// identical twiddle loads of every inlined transform merged and kept live in
-// registers for the whole kernel; passing the table through this barrier at
-// each transform keeps twiddle lifetimes local to that transform.  The asm
-// statement is empty: it emits no instruction.
-template <typename P>
-__device__ __forceinline__ P* opaque(P* p) {
-  asm volatile("" : "+s"(p));
-  return p;
+// A twiddle table in global memory, typed with the global address space so
+// that its reads are global (not flat) loads: a flat load counts against the
+// LDS counter too, and every LDS wait of the transform would then also wait
+// for the twiddle fetch.
+template <typename T>
+using gcx = __attribute__((address_space(1))) const cx<T>;
+
+// Returns the table address unchanged but hides its value from the
+// optimiser.  Kernels that run several transforms of the same length would
+// otherwise have the identical twiddle loads of every inlined transform
+// merged and kept live in registers for the whole kernel; passing the table
+// through this barrier at each transform keeps twiddle lifetimes local to
+// that transform.  The asm statement is empty: it emits no instruction.
+template <typename T>
+__device__ __forceinline__ gcx<T>* global_table(const cx<T>* p) {
+  gcx<T>* q = (gcx<T>*)p;
+  asm volatile("" : "+s"(q));
+  return q;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> ld(gcx<T>* p, int i) {
+  return mk(p[i].x, p[i].y);
 }
 
 // ---------------------------------------------------------------- plans
@@ -166,8 +178,7 @@ struct PlanInfo {
 
 // one Stockham stage: radix R, NS = product of the radices already applied
 template <typename T, int L, int EPT, int R, int NS, bool INV>
-__device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ lds, int ls, int t,
-                                          const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ lds, int ls, int t, gcx<T>* tw) {
   constexpr int TPS = L / EPT;
   constexpr int NB = EPT / R;
   static_assert(EPT % R == 0, "radix must divide EPT");
@@ -180,7 +191,7 @@ __device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ l
     const int k = j % NS;
     if (NS > 1) {
 #pragma unroll
-      for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], tw[q * k * (L / (NS * R))]);
+      for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], ld(tw, q * k * (L / (NS * R))));
     }
     BF<R, INV, T>::run(a);
     const int base = (j / NS) * NS * R + k;
@@ -194,8 +205,7 @@ __device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ l
 }
 
 template <typename T, int L, bool INV, int S, int NS>
-__device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t,
-                                           const cx<T>* tw) {
+__device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t, gcx<T>* tw) {
   if constexpr (S < Plan<L>::NR) {
     constexpr int R = Plan<L>::R[S];
     fft_stage<T, L, Plan<L>::EPT, R, NS, INV>(v, lds, ls, t, tw);
@@ -209,7 +219,7 @@ __device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds,
 template <typename T, int L, bool INV>
 __device__ __forceinline__ void fft_exec(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t,
                                          const cx<T>* __restrict__ tw) {
-  stage_loop<T, L, INV, 0, 1>(v, lds, ls, t, opaque(tw));
+  stage_loop<T, L, INV, 0, 1>(v, lds, ls, t, global_table(tw));
 }
 
 }  // namespace mfx

@@ -280,14 +280,24 @@ struct RowBlk {
   static constexpr int LDS = RPW * 2 * LB;  // complex elements of LDS per workgroup
 };
 
+// The twiddle tables (K entries each) are copied to LDS once per workgroup:
+// the transforms then read them with LDS loads, which leaves the vector
+// memory counter to the row data alone, so that a row fetched ahead of its
+// use is not waited for by every twiddle access.
 #define MFX_ROW_PROLOGUE(T, K)                                                      \
   constexpr int P_ = RowBlk<T, K>::P, LB_ = RowBlk<T, K>::LB;                       \
   __shared__ cx<T> lds_[RowBlk<T, K>::LDS];                                         \
+  __shared__ cx<T> tw_[2 * (K)];                                                    \
+  for (int i_ = threadIdx.x; i_ < (K); i_ += 64) {                                  \
+    tw_[i_] = twK[i_];                                                              \
+    tw_[(K) + i_] = twM[i_];                                                        \
+  }                                                                                 \
+  __syncthreads();                                                                  \
   const int lane = threadIdx.x % P_, r_ = threadIdx.x / P_;                         \
   const long long row = (long long)blockIdx.x * RowBlk<T, K>::RPW + r_;             \
   const bool act = row < nrows;                                                     \
   const long long rowc = act ? row : 0;                                             \
-  RowCtx<T, K> rc{lds_ + r_ * 2 * LB_, lds_ + r_ * 2 * LB_ + LB_, lane, twK, twM};
+  RowCtx<T, K> rc{lds_ + r_ * 2 * LB_, lds_ + r_ * 2 * LB_ + LB_, lane, tw_, tw_ + (K)};
 
 template <typename T, int K>
 __global__ void __launch_bounds__(64)
@@ -295,13 +305,10 @@ __global__ void __launch_bounds__(64)
               long long nrows, T scale, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
   constexpr int E = RowCtx<T, K>::E;
-  const cx<T>* s = reinterpret_cast<const cx<T>*>(src + rowc * spitch);
+  const __amdgpu_buffer_rsrc_t r = row_rsrc(src + rowc * spitch, K * sizeof(cx<T>));
   cx<T> v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const int n = lane + P_ * e;
-    v[e] = s[n < K ? n : K - 1];
-  }
+  for (int e = 0; e < E; ++e) v[e] = buf_load<T>(r, (unsigned)(lane + P_ * e) * sizeof(cx<T>));
   rc.r2c_store(v, dst + rowc * dpitch, hout, act, scale);
 }
 
@@ -312,14 +319,14 @@ __global__ void __launch_bounds__(64)
   MFX_ROW_PROLOGUE(T, K)
   constexpr int E = RowCtx<T, K>::E;
   const cx<T>* buf = rc.c2r(src + rowc * spitch, hin);
-  if (act) {
-    cx<T>* d = reinterpret_cast<cx<T>*>(dst + row * dpitch);
+  // K packed pairs, and the two padding reals of an in-place row set to zero;
+  // an inactive row stores nothing (zero-length resource)
+  const unsigned nst = act ? (unsigned)(dpitch >= 2 * K + 2 ? K + 1 : K) : 0u;
+  const __amdgpu_buffer_rsrc_t r = row_rsrc(dst + rowc * dpitch, nst * sizeof(cx<T>));
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int n = lane + P_ * e;
-      if (n < K) d[n] = buf[n] * scale;
-    }
-    if (lane == 0 && dpitch >= 2 * K + 2) d[K] = mk(T(0), T(0));
+  for (int e = 0; e <= E; ++e) {
+    const int n = lane + P_ * e;
+    if (n <= K) buf_store<T>(n < K ? buf[n] * scale : mk(T(0), T(0)), r, (unsigned)n * sizeof(cx<T>));
   }
 }
 

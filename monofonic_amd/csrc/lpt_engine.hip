@@ -24,6 +24,8 @@
 #include "fft_passes.hpp"
 #include "ops.hpp"
 
+#include <type_traits>
+
 namespace mfx {
 
 // ------------------------------------------------------------------ x/y stages
@@ -112,20 +114,17 @@ struct FieldY {
   const cx<T>* y[6];
 };
 
-// Hessian component (a,b) -> (Y array, kz power)
-__device__ __forceinline__ void hess_src(int a, int b, int& idx, int& pz) {
-  if (a > b) {
-    int tmp = a;
-    a = b;
-    b = tmp;
-  }
-  if (a == 0 && b == 0) { idx = 3; pz = 0; }
-  else if (a == 1 && b == 1) { idx = 5; pz = 0; }
-  else if (a == 2 && b == 2) { idx = 0; pz = 2; }
-  else if (a == 0 && b == 1) { idx = 4; pz = 0; }
-  else if (a == 0 && b == 2) { idx = 1; pz = 1; }
-  else { idx = 2; pz = 1; }
+// Hessian component (a,b) -> Y array and kz power
+constexpr int hess_idx(int a, int b) {
+  return (a > b) ? hess_idx(b, a)
+                 : (a == 0 && b == 0) ? 3
+                 : (a == 1 && b == 1) ? 5
+                 : (a == 2 && b == 2) ? 0
+                 : (a == 0 && b == 1) ? 4
+                 : (a == 0 && b == 2) ? 1
+                                      : 2;
 }
+constexpr int hess_pz(int a, int b) { return (a == 2) + (b == 2); }
 
 template <typename T>
 struct KzMul {
@@ -137,21 +136,49 @@ struct KzMul {
   }
 };
 
+// compile-time loop: f(std::integral_constant<int, O>) for O = B .. N-1
+template <int B, int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < N) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, N>(f);
+  }
+}
+
+// One z row of the fused stages.  Every operand row is fetched into registers
+// (fetch) one step before it is transformed, so that its HBM latency overlaps
+// the previous transform.
 template <typename T, int K>
 struct ZRow {
   static constexpr int E = RowCtx<T, K>::E;
+  using Raw = cx<T>[E + 1];
   RowCtx<T, K> rc;
   bool act;
   long long off;  // row * pitch
   int hin, n;
   T kfac;
-  // real-space Hessian component (a,b) of a field: lane's packed pairs
-  __device__ __forceinline__ void hess(cx<T> (&v)[E], const FieldY<T>& f, int a, int b) const {
-    int idx, pz;
-    hess_src(a, b, idx, pz);
-    rc.get(v, rc.c2r(f.y[idx] + off, hin, KzMul<T>{n, pz, kfac, T(-1)}));
+  __device__ __forceinline__ void fetch(Raw& x, const cx<T>* p) const { rc.load_raw(x, p + off, hin); }
+  template <int A, int B>
+  __device__ __forceinline__ void fetch(Raw& x, const FieldY<T>& f) const {
+    fetch(x, f.y[hess_idx(A, B)]);
   }
-  __device__ __forceinline__ void plain(cx<T> (&v)[E], const cx<T>* p) const { rc.get(v, rc.c2r(p + off, hin)); }
+  __device__ __forceinline__ KzMul<T> kz(int pz) const { return KzMul<T>{n, pz, kfac, T(-1)}; }
+  // real-space Hessian component (A,B) of a field from its fetched Y row:
+  // lane's packed pairs
+  template <int A, int B>
+  __device__ __forceinline__ void hess(cx<T> (&v)[E], const Raw& x) const {
+    const KzMul<T> m = kz(hess_pz(A, B));
+    rc.get(v, rc.c2r_from([&](int e, int k) { return x[e] * m(k); }, hin));
+  }
+  // Hessian components (A,B) - (C,D) of one field, combined before the c2r
+  template <int A, int B, int C, int D>
+  __device__ __forceinline__ void hess_diff(cx<T> (&v)[E], const Raw& x, const Raw& y) const {
+    const KzMul<T> m1 = kz(hess_pz(A, B)), m2 = kz(hess_pz(C, D));
+    rc.get(v, rc.c2r_from([&](int e, int k) { return x[e] * m1(k) - y[e] * m2(k); }, hin));
+  }
+  __device__ __forceinline__ void plain(cx<T> (&v)[E], const Raw& x) const {
+    rc.get(v, rc.c2r_from([&](int e, int) { return x[e]; }, hin));
+  }
   __device__ __forceinline__ void emit(const cx<T> (&v)[E], cx<T>* p) const {
     rc.r2c_store(v, p + off, hin, act, T(1));
   }
@@ -165,6 +192,7 @@ __device__ __forceinline__ cx<T> pmul(cx<T> a, cx<T> b) {
 #define MFX_ZROW_PROLOGUE                                                           \
   MFX_ROW_PROLOGUE(T, K)                                                            \
   constexpr int EPT = RowCtx<T, K>::E;                                              \
+  using Raw = typename ZRow<T, K>::Raw;                                             \
   ZRow<T, K> z{rc, act, rowc * pitch, hin, n, kfac};
 
 // stage A: 2LPT source S2 = sum_{i<j} (phi_ii phi_jj - phi_ij^2) (ic_generator.cc:471-475)
@@ -175,14 +203,20 @@ __global__ void __launch_bounds__(64)
     k_lpt_zA(FieldY<T> f, cx<T>* S2, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4, long long pitch, int hin, int n,
              T kfac, long long nrows, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ZROW_PROLOGUE
-  cx<T> dxx[EPT], dyy[EPT], s2[EPT], w[EPT];
-  z.hess(dxx, f, 0, 0);
-  z.hess(dyy, f, 1, 1);
-  z.hess(w, f, 2, 2);
+  Raw r0, r1;
+  cx<T> dxx[EPT], dyy[EPT], s2[EPT], w[EPT], oxy[EPT];
+  z.template fetch<0, 0>(r0, f);
+  z.template fetch<1, 1>(r1, f);
+  z.template hess<0, 0>(dxx, r0);
+  z.template fetch<2, 2>(r0, f);
+  z.template hess<1, 1>(dyy, r1);
+  z.template fetch<0, 1>(r1, f);
+  z.template hess<2, 2>(w, r0);
+  z.template fetch<0, 2>(r0, f);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = pmul(dxx[e], dyy[e]) + pmul(dxx[e] + dyy[e], w[e]);
-  cx<T> oxy[EPT];
-  z.hess(oxy, f, 0, 1);
+  z.template hess<0, 1>(oxy, r1);
+  z.template fetch<1, 2>(r1, f);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(oxy[e], oxy[e]);
   if (THREE) {
@@ -190,7 +224,7 @@ __global__ void __launch_bounds__(64)
     for (int e = 0; e < EPT; ++e) w[e] = pmul(dxx[e], dyy[e]) - pmul(oxy[e], oxy[e]);
     z.emit(w, I1);
   }
-  z.hess(dxx, f, 0, 2);  // dxx now holds phi_xz
+  z.template hess<0, 2>(dxx, r0);  // dxx now holds phi_xz
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(dxx[e], dxx[e]);
   if (THREE) {
@@ -201,7 +235,7 @@ __global__ void __launch_bounds__(64)
     for (int e = 0; e < EPT; ++e) w[e] = pmul(dxx[e], dxx[e]);
     z.emit(w, I4);
   }
-  z.hess(dyy, f, 1, 2);  // dyy now holds phi_yz
+  z.template hess<1, 2>(dyy, r1);  // dyy now holds phi_yz
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(dyy[e], dyy[e]);
   if (THREE) {
@@ -216,38 +250,63 @@ __global__ void __launch_bounds__(64)
 // Hessian component:
 //   3a = I1 phi_zz + 2 I2 phi_yz - I3 phi_xx - I4 phi_yy
 //   3b = -5/7 tr(phi) tr(phi2) + 5/7 sum_i phi_ii phi2_ii + 10/7 sum_{i<j} phi_ij phi2_ij
+// Operand order: per component q, phi_q (kind 0), phi2_q (kind 1) and, where
+// 3a has a term with phi_q, the intermediate paired with it (kind 2).
+namespace c1 {
+constexpr int NOPS = 16;
+constexpr int kind[NOPS] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 0, 1, 0, 1, 2};
+constexpr int ca[NOPS] = {0, 0, 0, 1, 1, 0, 2, 2, 0, 0, 0, 0, 0, 1, 1, 0};
+constexpr int cb[NOPS] = {0, 0, 0, 1, 1, 0, 2, 2, 0, 1, 1, 2, 2, 2, 2, 0};
+// kind 2: which intermediate (0..3 = I1..I4) and its 3a weight
+constexpr int part[NOPS] = {2, 2, 2, 3, 3, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1};
+constexpr int pw[NOPS] = {0, 0, -1, 0, 0, -1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 2};
+}  // namespace c1
+
 template <typename T, int K>
 __global__ void __launch_bounds__(64)
     k_lpt_zC1(FieldY<T> f1, FieldY<T> f2, const cx<T>* I1, const cx<T>* I2, const cx<T>* I3, const cx<T>* I4,
               cx<T>* out, long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
               const cx<T>* __restrict__ twM) {
   MFX_ZROW_PROLOGUE
-  cx<T> acc[EPT], tr1[EPT], tr2[EPT], a[EPT], b[EPT];
+  const cx<T>* Ip[4] = {I1, I2, I3, I4};
+  cx<T> acc[EPT], tr1[EPT], tr2[EPT], a[EPT], v[EPT];
+  Raw rb[2];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) acc[e] = tr1[e] = tr2[e] = mk(T(0), T(0));
-  const cx<T>* partner[6] = {I3, I4, I1, nullptr, nullptr, I2};
+  auto fetch_op = [&](auto oc) {
+    constexpr int o = decltype(oc)::value;
+    if constexpr (c1::kind[o] == 0)
+      z.template fetch<c1::ca[o], c1::cb[o]>(rb[o & 1], f1);
+    else if constexpr (c1::kind[o] == 1)
+      z.template fetch<c1::ca[o], c1::cb[o]>(rb[o & 1], f2);
+    else
+      z.fetch(rb[o & 1], Ip[c1::part[o]]);
+  };
+  fetch_op(std::integral_constant<int, 0>{});
+  static_for<0, c1::NOPS>([&](auto oc) {
+    constexpr int o = decltype(oc)::value;
+    if constexpr (o + 1 < c1::NOPS) fetch_op(std::integral_constant<int, o + 1>{});
+    constexpr bool diag = c1::ca[o] == c1::cb[o];
+    if constexpr (c1::kind[o] == 0) {
+      z.template hess<c1::ca[o], c1::cb[o]>(a, rb[o & 1]);
+      if constexpr (diag) {
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    constexpr int ca[6] = {0, 1, 2, 0, 0, 1}, cb[6] = {0, 1, 2, 1, 2, 2};
-    constexpr float pw[6] = {-1.f, -1.f, 1.f, 0.f, 0.f, 2.f};
-    z.hess(a, f1, ca[q], cb[q]);
-    z.hess(b, f2, ca[q], cb[q]);
-    const bool diag = q < 3;
-    const T w = diag ? T(5.0 / 7.0) : T(10.0 / 7.0);
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      acc[e] = acc[e] + pmul(a[e], b[e]) * w;
-      if (diag) {
-        tr1[e] = tr1[e] + a[e];
-        tr2[e] = tr2[e] + b[e];
+        for (int e = 0; e < EPT; ++e) tr1[e] = tr1[e] + a[e];
       }
-    }
-    if (q != 3 && q != 4) {
-      z.plain(b, partner[q]);
+    } else if constexpr (c1::kind[o] == 1) {
+      z.template hess<c1::ca[o], c1::cb[o]>(v, rb[o & 1]);
+      const T w = diag ? T(5.0 / 7.0) : T(10.0 / 7.0);
 #pragma unroll
-      for (int e = 0; e < EPT; ++e) acc[e] = acc[e] + pmul(a[e], b[e]) * T(pw[q]);
+      for (int e = 0; e < EPT; ++e) {
+        acc[e] = acc[e] + pmul(a[e], v[e]) * w;
+        if constexpr (diag) tr2[e] = tr2[e] + v[e];
+      }
+    } else {
+      z.plain(v, rb[o & 1]);
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) acc[e] = acc[e] + pmul(a[e], v[e]) * T(c1::pw[o]);
     }
-  }
+  });
 #pragma unroll
   for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(tr1[e], tr2[e]) * T(5.0 / 7.0);
   z.emit(acc, out);
@@ -256,31 +315,42 @@ __global__ void __launch_bounds__(64)
 // stage C2: A(3) component I (ic_generator.cc:534-545), p = I+1, pp = I+2:
 //   phi2_{I p} phi_{I pp} - phi2_{I pp} phi_{I p} + phi_{p pp}(phi2_{pp} - phi2_{pp pp})
 //   - phi2_{p pp}(phi_{p p} - phi_{pp pp})
+// The two differences are formed in k space, one c2r each.  Operand rows
+// rotate through three fetch buffers.
 template <typename T, int K, int I>
 __global__ void __launch_bounds__(64)
     k_lpt_zC2(FieldY<T> f1, FieldY<T> f2, cx<T>* out, long long pitch, int hin, int n, T kfac, long long nrows,
               const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ZROW_PROLOGUE
   constexpr int p = (I + 1) % 3, pp = (I + 2) % 3;
-  cx<T> acc[EPT], a[EPT], b[EPT], c[EPT];
-  z.hess(a, f2, I, p);
-  z.hess(b, f1, I, pp);
+  cx<T> acc[EPT], a[EPT], v[EPT];
+  Raw r0, r1, r2;
+  z.template fetch<I, p>(r0, f2);
+  z.template fetch<I, pp>(r1, f1);
+  z.template hess<I, p>(a, r0);
+  z.template fetch<I, pp>(r2, f2);
+  z.template hess<I, pp>(v, r1);
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) acc[e] = pmul(a[e], b[e]);
-  z.hess(a, f2, I, pp);
-  z.hess(b, f1, I, p);
+  for (int e = 0; e < EPT; ++e) acc[e] = pmul(a[e], v[e]);
+  z.template fetch<I, p>(r0, f1);
+  z.template hess<I, pp>(a, r2);
+  z.template fetch<p, pp>(r1, f1);
+  z.template hess<I, p>(v, r0);
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(a[e], b[e]);
-  z.hess(a, f1, p, pp);
-  z.hess(b, f2, p, p);
-  z.hess(c, f2, pp, pp);
+  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(a[e], v[e]);
+  z.template fetch<p, p>(r2, f2);
+  z.template fetch<pp, pp>(r0, f2);
+  z.template hess<p, pp>(a, r1);
+  z.template fetch<p, pp>(r1, f2);
+  z.template hess_diff<p, p, pp, pp>(v, r2, r0);
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] + pmul(a[e], b[e] - c[e]);
-  z.hess(a, f2, p, pp);
-  z.hess(b, f1, p, p);
-  z.hess(c, f1, pp, pp);
+  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] + pmul(a[e], v[e]);
+  z.template fetch<p, p>(r2, f1);
+  z.template fetch<pp, pp>(r0, f1);
+  z.template hess<p, pp>(a, r1);
+  z.template hess_diff<p, p, pp, pp>(v, r2, r0);
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(a[e], b[e] - c[e]);
+  for (int e = 0; e < EPT; ++e) acc[e] = acc[e] - pmul(a[e], v[e]);
   z.emit(acc, out);
 }
 

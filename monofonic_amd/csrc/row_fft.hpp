@@ -14,6 +14,8 @@
 #pragma once
 #include "fft_core.hpp"
 
+#include <type_traits>
+
 namespace mfx {
 
 // radix 16 as 4 x 4 with the internal twiddles exp(-+2 pi i n2 k1 / 16)
@@ -97,7 +99,6 @@ __device__ __forceinline__ cx<T>* row_stages(cx<T>* in, cx<T>* out, int lane, co
     constexpr int R = RowPlan<K>::R[S];
     constexpr int NBF = K / R;
     constexpr int P = RowLanes<K>::P;
-    const cx<T>* twp = opaque(tw);
     for (int j = lane; j < NBF; j += P) {
       cx<T> a[R];
 #pragma unroll
@@ -105,7 +106,7 @@ __device__ __forceinline__ cx<T>* row_stages(cx<T>* in, cx<T>* out, int lane, co
       const int k = j % NS;
       if (NS > 1) {
 #pragma unroll
-        for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], twp[q * k * (K / (NS * R))]);
+        for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], tw[q * k * (K / (NS * R))]);
       }
       BF<R, INV, T>::run(a);
       const int base = (j / NS) * NS * R + k;
@@ -126,6 +127,37 @@ __device__ __forceinline__ cx<T>* row_fft(cx<T>* A, cx<T>* B, int lane, const cx
   return row_stages<T, K, INV, 0, 1>(A, B, lane, tw);
 }
 
+// Row I/O through raw buffer instructions.  A row is described by a buffer
+// resource (base address and length in bytes, held in scalar registers) and
+// each lane addresses its elements by one 32-bit offset register plus
+// immediate offsets.  The hardware range check returns zero for loads at or
+// beyond the length and drops such stores, which implements the zero
+// padding of a half spectrum (entries k >= hin) and the truncation of an
+// output row (k >= hout) without per-element selects.
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, unsigned bytes) {
+  // word 3 = 0x00020000: 32-bit data format, the raw-buffer setting of gfx9-family GPUs
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(cx<T>) == 8)
+    return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(cx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_store(cx<T> v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(cx<T>) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, off, 0, 0);
+}
+
 // Row context: lane, LDS buffers and twiddles of one row
 template <typename T, int K>
 struct RowCtx {
@@ -133,8 +165,8 @@ struct RowCtx {
   cx<T>* A;
   cx<T>* B;
   int lane;
-  const cx<T>* twK;  // exp(-2 pi i n/K)
-  const cx<T>* twM;  // exp(-2 pi i n/(2K))
+  const cx<T>* twK;  // exp(-2 pi i n/K), n < K, in LDS
+  const cx<T>* twM;  // exp(-2 pi i n/(2K)), n < K, in LDS
 
   // Load the half spectrum X[0..K] (entries k >= hin are zero, entry k is
   // scaled by mul(k)) and c2r it; returns the buffer holding the packed real
@@ -143,33 +175,30 @@ struct RowCtx {
   __device__ __forceinline__ cx<T>* c2r(const cx<T>* s, int hin, Mul mul) const {
     cx<T> x[E + 1];
     load_raw(x, s, hin);
-    return c2r_raw(x, hin, mul);
+    return c2r_from([&](int e, int k) { return x[e] * mul(k); }, hin);
   }
 
-  // the lane's entries k = lane + P*e (e <= E) of a half-spectrum row, loaded
-  // unconditionally from clamped indices so that all loads are in flight
+  // the lane's entries k = lane + P*e (e <= E) of a half-spectrum row whose
+  // first hin entries are stored; entries k >= hin load as zero
   __device__ __forceinline__ void load_raw(cx<T> (&x)[E + 1], const cx<T>* s, int hin) const {
+    const __amdgpu_buffer_rsrc_t r = row_rsrc(s, (unsigned)hin * sizeof(cx<T>));
 #pragma unroll
-    for (int e = 0; e <= E; ++e) {
-      const int k = lane + P * e;
-      const int kc = k < hin ? k : hin - 1;
-      x[e] = s[kc];
-    }
+    for (int e = 0; e <= E; ++e) x[e] = buf_load<T>(r, (unsigned)(lane + P * e) * sizeof(cx<T>));
   }
 
-  // c2r of a row whose entries were fetched by load_raw
-  template <class Mul>
-  __device__ __forceinline__ cx<T>* c2r_raw(const cx<T> (&x)[E + 1], int hin, Mul mul) const {
+  // c2r of the half spectrum whose entry k = lane + P*e is val(e, k) (val
+  // returns zero for the entries k >= hin that load_raw fetched as zero)
+  template <class Val>
+  __device__ __forceinline__ cx<T>* c2r_from(Val val, int hin) const {
     wave_sync();  // the previous transform's result buffer may still be read
 #pragma unroll
     for (int e = 0; e <= E; ++e) {
       const int k = lane + P * e;
-      if (k <= K) B[k] = (k < hin) ? x[e] * mul(k) : mk(T(0), T(0));
+      if (k <= K) B[k] = val(e, k);
     }
     wave_sync();
     // Z[k] = (X[k] + conj X[K-k]) + i (X[k] - conj X[K-k]) e^{+2 pi i k/(2K)};
     // the imaginary parts of X[0] and X[K] do not enter
-    const cx<T>* tm = opaque(twM);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = lane + P * e;
@@ -181,7 +210,7 @@ struct RowCtx {
           Bk.y = T(0);
         }
         cx<T> Ev = Ak + Bk;
-        cx<T> O = twmul<true>(Ak - Bk, tm[k]);
+        cx<T> O = twmul<true>(Ak - Bk, twM[k]);
         A[k] = mk(Ev.x - O.y, Ev.y + O.x);
       }
     }
@@ -216,24 +245,24 @@ struct RowCtx {
     }
     wave_sync();
     cx<T>* Z = row_fft<T, K, false>(A, B, lane, twK);
-    const cx<T>* tm = opaque(twM);
-    if (act) {
+    // entries k >= hout (all of them for an inactive row) are dropped by the
+    // range check of the row's buffer resource
+    const __amdgpu_buffer_rsrc_t r = row_rsrc(d, act ? (unsigned)hout * sizeof(cx<T>) : 0u);
 #pragma unroll
-      for (int e = 0; e <= E; ++e) {
-        const int k = lane + P * e;
-        if (k < hout && k <= K) {
-          cx<T> X;
-          if (k == K) {
-            X = mk(Z[0].x - Z[0].y, T(0));
-          } else {
-            // X[k] = (Z[k] + conj Z[K-k])/2 - i/2 e^{-2 pi i k/(2K)} (Z[k] - conj Z[K-k])
-            cx<T> Zk = Z[k], Zc = conj(Z[(K - k) % K]);
-            cx<T> a = (Zk + Zc) * T(0.5);
-            cx<T> dd = cmul((Zk - Zc) * T(0.5), tm[k]);
-            X = mk(a.x + dd.y, a.y - dd.x);
-          }
-          d[k] = X * scale;
+    for (int e = 0; e <= E; ++e) {
+      const int k = lane + P * e;
+      if (k <= K) {
+        cx<T> X;
+        if (k == K) {
+          X = mk(Z[0].x - Z[0].y, T(0));
+        } else {
+          // X[k] = (Z[k] + conj Z[K-k])/2 - i/2 e^{-2 pi i k/(2K)} (Z[k] - conj Z[K-k])
+          cx<T> Zk = Z[k], Zc = conj(Z[(K - k) % K]);
+          cx<T> a = (Zk + Zc) * T(0.5);
+          cx<T> dd = cmul((Zk - Zc) * T(0.5), twM[k]);
+          X = mk(a.x + dd.y, a.y - dd.x);
         }
+        buf_store<T>(X * scale, r, (unsigned)k * sizeof(cx<T>));
       }
     }
     wave_sync();
