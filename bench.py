@@ -15,10 +15,14 @@ positions and velocities, all resident in HBM.  value = particles/s = 512^3 /
 Also reported on the same line:
   roofline      achieved HBM GB/s of the dominant kernel (algorithmic bytes
                 per launch / average launch time from HIP events on the
-                library's stream) against 8 TB/s
+                library's stream) against 8 TB/s; traffic = its calibrated
+                rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch from the
+                summary committed under profiles/ for this workload
   fft_gbps      algorithmic bytes / time summed over every FFT pass
   fp64          particles/s of the same workload with FP64 bulk
-  cpu_baseline  the FP64 CPU oracle (OpenMP) on a 128^3 3LPT sample
+  accuracy      per output field, FP32 bulk vs FP64 bulk: RMS and max error
+                relative to the field RMS, and the P(k) ratio deviation
+  cpu_baseline  the FP64 CPU oracle (OpenMP) on a 256^3 3LPT sample (box 500)
 """
 import argparse
 import json
@@ -69,8 +73,7 @@ def run_gpu(M, ctx, n, precision, steps, warmup, fused, profile_step=True):
         stats = ctx.profile_stats()
         ctx.profile(False)
         log(f"[bench] {precision} profiled step done")
-    del ic
-    return wall, ev_ms, stats
+    return wall, ev_ms, stats, ic
 
 
 def roofline(stats):
@@ -89,7 +92,66 @@ def roofline(stats):
     }, (fft_bytes / (fft_ms * 1e-3) / 1e9 if fft_ms > 0 else None)
 
 
-def cpu_baseline(n=128):
+def profiled_traffic(kernel, workload):
+    """calibrated HBM bytes per launch of `kernel` on `workload`, from the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries committed under profiles/"""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith((".md", ".txt")):
+            continue
+        for line in open(os.path.join(d, f)):
+            w = line.split()
+            if len(w) == 4 and w[0] == "traffic" and w[1] == kernel and w[2] == workload:
+                return float(w[3])
+    return None
+
+
+def accuracy(M, ctx, ic32, ic64, n, boxlen):
+    """per output field: RMS and max |FP32 - FP64| relative to the FP64 field's
+    RMS (sigma_Psi for displacements), and the largest deviation of the P(k)
+    ratio FP32/FP64 over 0.01 < k < k_Nyquist (P(k) from Compute_PowerSpectrum
+    of each field)"""
+    import numpy as np
+    names = ["psi_x", "psi_y", "psi_z", "v_x", "v_y", "v_z"]
+    q1 = np.arange(n, dtype=np.float64) / n
+    kny = np.pi * n / boxlen
+    out = {}
+    for w in range(6):
+        a = ic32.particles(w)
+        b = ic64.particles(w)
+        if w < 3:  # positions q + Psi: compare the displacements Psi
+            sh = [1, 1, 1]
+            sh[w] = n
+            a -= q1.reshape(sh)
+            b -= q1.reshape(sh)
+        sig = float(np.sqrt(np.mean(b * b)))
+        d = a - b
+        rms = float(np.sqrt(np.mean(d * d))) / sig
+        mx = float(np.max(np.abs(d))) / sig
+        del d
+        pk = []
+        for f in (a, b):
+            g = M.Grid_FFT(ctx, n, boxlen, "double")
+            buf = np.zeros((n, n, n + 2))
+            buf[:, :, :n] = f
+            g.upload(buf)
+            del buf
+            g.FourierTransformForward()
+            k, P, _, _ = g.Compute_PowerSpectrum()
+            pk.append(P)
+            g.reset()
+        sel = (k > 0.01) & (k < kny) & (pk[1] > 0)
+        dev = float(np.max(np.abs(pk[0][sel] / pk[1][sel] - 1)))
+        out[names[w]] = {"rms_rel": float(f"{rms:.3e}"), "max_rel": float(f"{mx:.3e}"),
+                         "pk_ratio_dev": float(f"{dev:.3e}")}
+        del a, b
+        log(f"[bench] accuracy {names[w]}: rms {rms:.2e} max {mx:.2e} P(k) {dev:.2e}")
+    return out
+
+
+def cpu_baseline(n=256):
     """FP64 CPU oracle (OpenMP) on an n^3 3LPT sample of the same workload"""
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -112,7 +174,8 @@ def main():
     ap.add_argument("--fused", type=int, default=1, help="1: fused LPT source engine, 0: reference convolve sequence")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-bulk comparison leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the per-field FP32 vs FP64 error report")
+    ap.add_argument("--cpu-n", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -122,18 +185,32 @@ def main():
     import monofonic_amd as M
     ctx = M.Context(0)
     n = args.n
-    wall, ev_ms, stats = run_gpu(M, ctx, n, "float", args.steps, args.warmup, args.fused)
+    wall, ev_ms, stats, ic32 = run_gpu(M, ctx, n, "float", args.steps, args.warmup, args.fused)
     ms_per_step = ev_ms / args.steps
     value = n ** 3 / (ms_per_step * 1e-3)
     roof, fft_gbps = roofline(stats)
+    workload_id = f"3lpt-{n}-f32" + ("" if args.fused else "-seq")
+    tr = profiled_traffic(roof["kernel"], workload_id)
+    roof["traffic"] = round(tr) if tr else None
 
     fp64 = None
+    acc = None
     if not args.no_fp64:
-        w64, e64, st64 = run_gpu(M, ctx, n, "double", max(1, args.steps // 2), 1, args.fused)
+        w64, e64, st64, ic64 = run_gpu(M, ctx, n, "double", max(1, args.steps // 2), 1, args.fused)
         ms64 = e64 / max(1, args.steps // 2)
         _, fft64 = roofline(st64)
         fp64 = {"value": round(n ** 3 / (ms64 * 1e-3), 1), "unit": "particles/s", "ms_per_step": round(ms64, 3),
                 "fft_gbps": round(fft64, 1) if fft64 else None, "fp32_speedup": round(ms64 / ms_per_step, 3)}
+        if not args.no_accuracy:
+            fields = accuracy(M, ctx, ic32, ic64, n, 1000.0)
+            acc = {"reference": "FP64-bulk GPU path, same seed (it matches the FP64 CPU oracle to 1e-11 rms at "
+                                "32^3-64^3 in the parity tests)",
+                   "normalisation": "rms_rel, max_rel: |FP32 - FP64| / RMS of the FP64 field (sigma_Psi for Psi)",
+                   "fields": fields,
+                   "max_rms_rel": max(v["rms_rel"] for v in fields.values()),
+                   "max_pk_ratio_dev": max(v["pk_ratio_dev"] for v in fields.values())}
+        del ic64
+    del ic32
 
     cpu = None
     if not args.no_cpu:
@@ -151,7 +228,7 @@ def main():
                    "wall_s": round(wall, 4)},
         "roofline": roof, "fft_gbps": round(fft_gbps, 1) if fft_gbps else None,
         "fft_frac": round(fft_gbps / HBM_PEAK_GBPS, 4) if fft_gbps else None,
-        "fp64": fp64, "cpu_baseline": cpu,
+        "fp64": fp64, "accuracy": acc, "cpu_baseline": cpu,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                     for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])},

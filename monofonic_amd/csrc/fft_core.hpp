@@ -176,9 +176,37 @@ struct PlanInfo {
   static constexpr int TPS = L / EPT;
 };
 
-// one Stockham stage: radix R, NS = product of the radices already applied
+// product of the radices of the stages before stage S
+template <int L, int S>
+struct StageNS {
+  static constexpr int value = S == 0 ? 1 : Plan<L>::R[S - 1] * StageNS<L, (S > 0 ? S - 1 : 0)>::value;
+};
+template <int L>
+struct StageNS<L, 0> {
+  static constexpr int value = 1;
+};
+
+// Twiddles of stage S for thread t: w[m*(R-1) + q-1] = tw[q k L/(NS R)] for
+// butterfly m (k = (t + TPS m) mod NS) and q = 1..R-1; none for stage 0.
+template <typename T, int L, int S>
+__device__ __forceinline__ void stage_twiddles(cx<T> (&w)[Plan<L>::EPT], gcx<T>* tw, int t) {
+  if constexpr (S > 0 && S < Plan<L>::NR) {
+    constexpr int EPT = Plan<L>::EPT, TPS = L / EPT, R = Plan<L>::R[S], NB = EPT / R;
+    constexpr int NS = StageNS<L, S>::value;
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      const int k = (t + TPS * m) % NS;
+#pragma unroll
+      for (int q = 1; q < R; ++q) w[m * (R - 1) + q - 1] = ld(tw, q * k * (L / (NS * R)));
+    }
+  }
+}
+
+// one Stockham stage: radix R, NS = product of the radices already applied,
+// w = this stage's twiddles (stage_twiddles)
 template <typename T, int L, int EPT, int R, int NS, bool INV>
-__device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ lds, int ls, int t, gcx<T>* tw) {
+__device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ lds, int ls, int t,
+                                          const cx<T> (&w)[EPT]) {
   constexpr int TPS = L / EPT;
   constexpr int NB = EPT / R;
   static_assert(EPT % R == 0, "radix must divide EPT");
@@ -191,7 +219,7 @@ __device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ l
     const int k = j % NS;
     if (NS > 1) {
 #pragma unroll
-      for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], ld(tw, q * k * (L / (NS * R))));
+      for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], w[m * (R - 1) + q - 1]);
     }
     BF<R, INV, T>::run(a);
     const int base = (j / NS) * NS * R + k;
@@ -204,12 +232,18 @@ __device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ l
   __syncthreads();
 }
 
-template <typename T, int L, bool INV, int S, int NS>
-__device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t, gcx<T>* tw) {
+// Stage S runs with its twiddles already in registers (wcur) and first issues
+// the loads of stage S+1's twiddles, whose latency then overlaps stage S's
+// arithmetic and LDS exchange.
+template <typename T, int L, bool INV, int S>
+__device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t, gcx<T>* tw,
+                                           const cx<T> (&wcur)[Plan<L>::EPT]) {
   if constexpr (S < Plan<L>::NR) {
     constexpr int R = Plan<L>::R[S];
-    fft_stage<T, L, Plan<L>::EPT, R, NS, INV>(v, lds, ls, t, tw);
-    stage_loop<T, L, INV, S + 1, NS * R>(v, lds, ls, t, tw);
+    cx<T> wnext[Plan<L>::EPT];
+    stage_twiddles<T, L, S + 1>(wnext, tw, t);
+    fft_stage<T, L, Plan<L>::EPT, R, StageNS<L, S>::value, INV>(v, lds, ls, t, wcur);
+    stage_loop<T, L, INV, S + 1>(v, lds, ls, t, tw, wnext);
   }
 }
 
@@ -219,7 +253,8 @@ __device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds,
 template <typename T, int L, bool INV>
 __device__ __forceinline__ void fft_exec(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t,
                                          const cx<T>* __restrict__ tw) {
-  stage_loop<T, L, INV, 0, 1>(v, lds, ls, t, global_table(tw));
+  cx<T> w0[Plan<L>::EPT];
+  stage_loop<T, L, INV, 0>(v, lds, ls, t, global_table(tw), w0);
 }
 
 }  // namespace mfx

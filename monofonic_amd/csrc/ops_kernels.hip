@@ -207,6 +207,10 @@ __global__ void k_sum2(const T* f, int n0, int n1, int n2, long long pitch, doub
 // power spectrum bins (grid_fft.cc:1133-1210): per-bin sums of |z|^2, |z|^4, k, count
 template <typename T>
 __global__ void k_pk(const cx<T>* f, int n, double kfac, int nbins, double* bins) {
+  // Modes with |n| an integer (e.g. (9,12,0), |n| = 15) lie exactly on a bin
+  // edge, and their bin depends on the rounding of |k|: |k|^2 is summed
+  // without fused multiply-adds, as the reference's host loop does.
+#pragma clang fp contract(off)
   const int nh = n / 2 + 1;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)n * n * nh) return;

@@ -1,0 +1,99 @@
+"""k-space operators of Grid_FFT / ic_generator::run and Compute_PowerSpectrum
+on the GPU against the oracle (grid_fft.hh:392-466,844-887; grid_fft.cc:1133-1210;
+ic_generator.cc:341-346,450-457)."""
+import numpy as np
+import pytest
+
+from conftest import rel_max, rel_rms
+from helpers import random_kspace_field, random_real_grid
+
+pytestmark = pytest.mark.gpu
+
+BOX = 100.0
+
+
+def _kgrid(M, ctx, n, prec, a):
+    g = M.Grid_FFT(ctx, n, BOX, prec, initialspace=M.kspace_id)
+    g.upload(a)
+    return g
+
+
+@pytest.mark.parametrize("n,prec,tol", [(32, "double", 1e-14), (48, "double", 1e-14), (32, "float", 1e-6)])
+def test_inverse_laplacian_matches_oracle(ctx, orc, n, prec, tol):
+    import monofonic_amd as M
+    print(f"inverse laplacian n={n} {prec}", flush=True)
+    a = random_kspace_field(orc, n, 21)
+    g = _kgrid(M, ctx, n, prec, a)
+    g.apply_InverseLaplacian()
+    r = a.copy()
+    orc.inverse_laplacian(r, BOX)
+    assert rel_rms(g.kdata(), orc.kview(r)) < tol
+    assert rel_max(g.kdata(), orc.kview(r)) < 10 * tol
+
+
+@pytest.mark.parametrize("n", [16, 24])
+def test_zero_dc_scale_stats(ctx, orc, n):
+    import monofonic_amd as M
+    a = random_real_grid(n, 3)
+    a[:, :, :n] += 0.25
+    g = M.Grid_FFT(ctx, n, BOX, "double")
+    g.upload(a)
+    x = a[:, :, :n]
+    m, s = g.mean(), g.std()
+    assert abs(m - x.mean()) < 1e-13
+    assert abs(s - x.std()) < 1e-12
+    g *= 3.0
+    assert np.allclose(g.rdata(), 3.0 * x, rtol=0, atol=1e-13)
+    g /= 3.0
+    g.FourierTransformForward()
+    k = g.kdata()
+    assert abs(k[0, 0, 0]) > 1.0
+    g.zero_DC_mode()
+    k2 = g.kdata()
+    assert k2[0, 0, 0] == 0
+    assert np.array_equal(k2.ravel()[1:], k.ravel()[1:])
+
+
+@pytest.mark.parametrize("fixing,inversion", [(False, False), (True, False), (False, True), (True, True)])
+def test_noise_normalise_matches_oracle(ctx, orc, fixing, inversion):
+    import monofonic_amd as M
+    n = 32
+    a = random_real_grid(n, 8)
+    orc.fft_r2c(a)
+    volfac = (BOX / n / 2.0 / np.pi) ** 1.5
+    g = _kgrid(M, ctx, n, "double", a)
+    M.noise_normalise(g, volfac, fixing, inversion)
+    r = a.copy()
+    orc.noise_normalise(r, volfac, fixing, inversion)
+    assert rel_max(g.kdata(), orc.kview(r)) < 1e-14
+
+
+@pytest.mark.parametrize("n,prec,tol", [(32, "double", 1e-14), (64, "float", 1e-6)])
+def test_phi1_matches_oracle(ctx, orc, n, prec, tol):
+    import monofonic_amd as M
+    a = random_real_grid(n, 9)
+    orc.fft_r2c(a)
+    table = orc.Cosmology().amplitude_table(n, BOX)
+    wn = _kgrid(M, ctx, n, prec, a)
+    phi = M.Grid_FFT(ctx, n, BOX, prec, initialspace=M.kspace_id)
+    M.phi1(phi, wn, table)
+    r = np.zeros_like(a)
+    orc.phi1(r, a, BOX, table)
+    assert rel_rms(phi.kdata(), orc.kview(r)) < tol
+    assert rel_max(phi.kdata(), orc.kview(r)) < 10 * tol
+
+
+@pytest.mark.parametrize("n,prec,tol", [(32, "double", 1e-13), (64, "double", 1e-13), (64, "float", 1e-5)])
+def test_power_spectrum_matches_oracle(ctx, orc, n, prec, tol):
+    import monofonic_amd as M
+    print(f"power spectrum n={n} {prec}", flush=True)
+    a = random_kspace_field(orc, n, 13, slope=-1.0)
+    g = _kgrid(M, ctx, n, prec, a)
+    k, P, eP, c = g.Compute_PowerSpectrum()
+    rk, rP, reP, rc = orc.power_spectrum(a, BOX)
+    assert len(k) == len(rk)
+    assert np.array_equal(c, rc)
+    sel = rc > 0
+    assert np.max(np.abs(k[sel] / rk[sel] - 1)) < tol
+    assert np.max(np.abs(P[sel] / rP[sel] - 1)) < tol
+    assert np.max(np.abs(eP[sel] - reP[sel]) / rP[sel]) < 100 * tol
