@@ -161,6 +161,31 @@ int mfx_lpt_sources(mfx_conv *cv, mfx_grid *phi, mfx_grid *phi2, mfx_grid *phi3,
  * (src/ic_generator.cc:821-860, 877-919); tmp is left in k space */
 int mfx_assemble_kspace(mfx_grid *tmp, mfx_grid *phi, mfx_grid *phi2, mfx_grid *phi3, mfx_grid *A3x,
                         mfx_grid *A3y, mfx_grid *A3z, int order, int kind, int idim, double vfac, double unit);
+/* ------------------------------------------------------------ PLT corrections
+ * particle::lattice_gradient for the sc lattice (include/particle_plt.hh:48-585).
+ * mfx_plt_create runs init_D (:68-515) on an ngrid^3 mesh of the unit box (the
+ * reference uses 64) on the device: Ewald-summed dynamical matrix, FFT, largest
+ * eigenpair per mode, spherical projection and growth-rate factor. */
+typedef struct mfx_plt mfx_plt;
+int mfx_plt_create(mfx_ctx *c, int ngrid, mfx_plt **out);
+int mfx_plt_destroy(mfx_plt *p);
+/* tables, 4 x ngrid x ngrid x (ngrid/2+1) doubles: D_r (= 1), D_theta, D_phi,
+ * growth-rate factor (the D_xx_, D_yy_, D_zz_, D_xy_ grids after init_D) */
+int mfx_plt_tables(const mfx_plt *p, double *host, size_t count);
+/* real parts of the k-space dynamical matrix, 6 x ngrid x ngrid x (ngrid/2+1):
+ * xx, xy, xz, yy, yz, zz, unnormalised transform (:286-313) -- test hook */
+int mfx_plt_dmatrix(const mfx_plt *p, double *host, size_t count);
+/* lattice_gradient::gradient (imaginary parts, 3 per mode) and vfac_corr
+ * (:551-585) at `count` k-space indices ijk (3 per mode) of an n-grid of side
+ * boxlen -- test hook */
+int mfx_plt_gradient(const mfx_plt *p, size_t n, double boxlen, const long long *ijk, int count, double *g,
+                     double *vfac);
+/* mfx_assemble_kspace with the PLT gradient and, for velocities, its growth-
+ * rate factor (ic_generator.cc:821-919 in ENABLE_PLT builds) */
+int mfx_assemble_kspace_plt(mfx_grid *tmp, mfx_grid *phi, mfx_grid *phi2, mfx_grid *phi3, mfx_grid *A3x,
+                            mfx_grid *A3y, mfx_grid *A3z, int order, int kind, int idim, double vfac, double unit,
+                            const mfx_plt *plt);
+
 /* set_positions / set_velocities for the sc lattice, 64-bit reals
  * (include/particle_generator.hh:499-543, 597-638): out_dev is a device
  * array of n^3 doubles, particle order (i,j,k); kind 0 positions, 1 velocities */
@@ -216,6 +241,8 @@ typedef struct {
   double H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb;
   int zero_radiation;
   int use_fused_lpt;    /* 1: mfx_lpt_sources, 0: the reference's convolve call sequence */
+  int use_plt;          /* 1: PLT-corrected assembly (ENABLE_PLT builds, ic_generator.cc:366-370) */
+  int plt_ngrid;        /* PLT mesh; 0 selects the reference's 64 */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;

@@ -25,7 +25,8 @@ EXPORTS = [
     "mfx_grid_power_spectrum", "mfx_noise_normalise", "mfx_phi1", "mfx_conv_create", "mfx_conv_destroy",
     "mfx_conv2", "mfx_conv3", "mfx_conv_gradients", "mfx_conv_gradient_and_hessian", "mfx_conv_hessians",
     "mfx_conv_hessians3", "mfx_conv_sum_of_hessians", "mfx_conv_difference_of_hessians",
-    "mfx_conv_multiply_field", "mfx_lpt_sources", "mfx_assemble_kspace", "mfx_set_particles",
+    "mfx_conv_multiply_field", "mfx_lpt_sources", "mfx_plt_create", "mfx_plt_destroy", "mfx_plt_tables",
+    "mfx_plt_dmatrix", "mfx_plt_gradient", "mfx_assemble_kspace_plt", "mfx_assemble_kspace", "mfx_set_particles",
     "mfx_rng_music1_fill", "mfx_rng_ngenic_fill", "mfx_rng_threefry_fill", "mfx_rng_mt19937_raw",
     "mfx_rng_ranlxd1_raw", "mfx_rng_threefry_uniforms", "mfx_cosmo_create", "mfx_cosmo_destroy",
     "mfx_cosmo_query", "mfx_cosmo_amplitude_table", "mfx_icgen_create", "mfx_icgen_destroy", "mfx_icgen_run",
@@ -49,7 +50,8 @@ class IcgenParams(ctypes.Structure):
                 ("seed", ctypes.c_long), ("fixing", ctypes.c_int), ("inversion", ctypes.c_int),
                 ("H0", ctypes.c_double), ("Omega_m", ctypes.c_double), ("Omega_b", ctypes.c_double),
                 ("n_s", ctypes.c_double), ("sigma_8", ctypes.c_double), ("Tcmb", ctypes.c_double),
-                ("zero_radiation", ctypes.c_int), ("use_fused_lpt", ctypes.c_int)]
+                ("zero_radiation", ctypes.c_int), ("use_fused_lpt", ctypes.c_int), ("use_plt", ctypes.c_int),
+                ("plt_ngrid", ctypes.c_int)]
 
 
 def load():
@@ -100,6 +102,10 @@ def load():
         "mfx_conv_multiply_field": [vp, vp, vp, vp, i32, dbl],
         "mfx_lpt_sources": [vp, vp, vp, vp, vp, vp, vp, i32],
         "mfx_assemble_kspace": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, dbl, dbl],
+        "mfx_assemble_kspace_plt": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, dbl, dbl, vp],
+        "mfx_plt_create": [vp, i32, pp], "mfx_plt_destroy": [vp],
+        "mfx_plt_tables": [vp, vp, sz], "mfx_plt_dmatrix": [vp, vp, sz],
+        "mfx_plt_gradient": [vp, sz, dbl, vp, i32, vp, vp],
         "mfx_set_particles": [vp, vp, i32, dbl, i32],
         "mfx_rng_music1_fill": [vp, ctypes.c_long, i32],
         "mfx_rng_ngenic_fill": [vp, ctypes.c_long],

@@ -419,10 +419,62 @@ def phi1(phi, wn, amp_table):
     call("mfx_phi1", phi.handle, wn.handle, t.ctypes.data_as(ctypes.c_void_p), t.size)
 
 
-def assemble_kspace(tmp, phi, phi2, phi3, A3, order, kind, idim, vfac, unit=1.0):
+def assemble_kspace(tmp, phi, phi2, phi3, A3, order, kind, idim, vfac, unit=1.0, plt=None):
+    """ic_generator.cc:821-919 for one component; with `plt` (a PLT object)
+    the gradient is particle::lattice_gradient (ENABLE_PLT builds)"""
     h = lambda g: g.handle if g is not None else None
-    call("mfx_assemble_kspace", tmp.handle, phi.handle, h(phi2), h(phi3), h(A3[0]), h(A3[1]), h(A3[2]), order,
-         kind, idim, float(vfac), float(unit))
+    if plt is None:
+        call("mfx_assemble_kspace", tmp.handle, phi.handle, h(phi2), h(phi3), h(A3[0]), h(A3[1]), h(A3[2]), order,
+             kind, idim, float(vfac), float(unit))
+    else:
+        call("mfx_assemble_kspace_plt", tmp.handle, phi.handle, h(phi2), h(phi3), h(A3[0]), h(A3[1]), h(A3[2]),
+             order, kind, idim, float(vfac), float(unit), plt.handle)
+
+
+class PLT:
+    """particle::lattice_gradient (include/particle_plt.hh:48-585) for the sc
+    lattice: init_D runs on the device at construction"""
+
+    def __init__(self, ctx, ngrid=64):
+        self.ctx = ctx
+        self.ng = int(ngrid)
+        self._h = ctypes.c_void_p()
+        call("mfx_plt_create", ctx.handle, self.ng, ctypes.byref(self._h))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("mfx_plt_destroy", self._h)
+        except Exception:
+            pass
+
+    def tables(self):
+        """(4, ng, ng, ng/2+1): D_r, D_theta, D_phi, growth-rate factor"""
+        nh = self.ng // 2 + 1
+        out = np.zeros((4, self.ng, self.ng, nh))
+        call("mfx_plt_tables", self._h, out.ctypes.data_as(ctypes.c_void_p), out.size)
+        return out
+
+    def dmatrix(self):
+        """(6, ng, ng, ng/2+1) real parts of the unnormalised k-space D_xx..D_zz"""
+        nh = self.ng // 2 + 1
+        out = np.zeros((6, self.ng, self.ng, nh))
+        call("mfx_plt_dmatrix", self._h, out.ctypes.data_as(ctypes.c_void_p), out.size)
+        return out
+
+    def gradient(self, n, boxlen, ijk):
+        """gradient (imaginary parts) and vfac_corr at modes ijk (m x 3) of an n-grid"""
+        ijk = np.ascontiguousarray(ijk, dtype=np.int64).reshape(-1, 3)
+        m = ijk.shape[0]
+        g = np.zeros((m, 3))
+        v = np.zeros(m)
+        p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        call("mfx_plt_gradient", self._h, int(n), float(boxlen), p(ijk), m, p(g), p(v))
+        return g, v
 
 
 # ------------------------------------------------------------------ LPT driver (src/ic_generator.cc:90-971)
@@ -434,7 +486,7 @@ class ICGenerator:
 
     def __init__(self, ctx, GridRes, BoxLength, zstart, LPTorder=3, precision="double", generator="MUSIC1",
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
-                 sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True):
+                 sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64):
         self.ctx = ctx
         self.n = int(GridRes)
         p = _lib.IcgenParams()
@@ -443,6 +495,7 @@ class ICGenerator:
         p.fixing, p.inversion = int(DoFixing), int(DoInversion)
         p.H0, p.Omega_m, p.Omega_b, p.n_s, p.sigma_8, p.Tcmb = H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb
         p.zero_radiation, p.use_fused_lpt = int(ZeroRadiation), int(fused)
+        p.use_plt, p.plt_ngrid = int(PLT), int(plt_ngrid)
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 

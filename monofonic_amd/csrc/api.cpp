@@ -12,6 +12,7 @@
 #include "../../include/mfx.h"
 #include "cosmology.hpp"
 #include "ops.hpp"
+#include "plt.hpp"
 
 using namespace mfx;
 
@@ -831,34 +832,21 @@ int mfx_lpt_sources(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3,
 }
 
 // ------------------------------------------------------------------ assembly
+static void assemble_impl(mfx_grid* tmp, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3x,
+                          mfx_grid* A3y, mfx_grid* A3z, int order, int kind, int idim, double vfac, double unit,
+                          const mfx_plt* plt);
+
 int mfx_assemble_kspace(mfx_grid* tmp, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3x, mfx_grid* A3y,
                         mfx_grid* A3z, int order, int kind, int idim, double vfac, double unit) {
+  return guard([&] { assemble_impl(tmp, phi, phi2, phi3, A3x, A3y, A3z, order, kind, idim, vfac, unit, nullptr); });
+}
+
+int mfx_assemble_kspace_plt(mfx_grid* tmp, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3x,
+                            mfx_grid* A3y, mfx_grid* A3z, int order, int kind, int idim, double vfac, double unit,
+                            const mfx_plt* plt) {
   return guard([&] {
-    require_grid(tmp);
-    require_grid(phi);
-    require_cubic(tmp);
-    same_shape(tmp, phi);
-    MFX_REQUIRE(order >= 1 && order <= 3, "order must be 1, 2 or 3");
-    MFX_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (displacement) or 1 (velocity)");
-    MFX_REQUIRE(idim >= 0 && idim < 3, "idim must be 0, 1 or 2");
-    mfx_grid* in[5] = {phi2, phi3, A3x, A3y, A3z};
-    const int nin = order == 1 ? 0 : order == 2 ? 1 : 5;
-    for (int q = 0; q < nin; ++q) {
-      require_grid(in[q]);
-      same_shape(tmp, in[q]);
-    }
-    grid_forward(phi, true);
-    for (int q = 0; q < nin; ++q) grid_forward(in[q], true);
-    tmp->space = MFX_KSPACE;
-    auto P = [&](mfx_grid* g, int q) { return (q < nin && g) ? g->d : phi->d; };
-    if (tmp->prec == MFX_FLOAT)
-      assemble<float>(tmp->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2, 0),
-                      (const cx<float>*)P(phi3, 1), (const cx<float>*)P(A3x, 2), (const cx<float>*)P(A3y, 3),
-                      (const cx<float>*)P(A3z, 4), (int)tmp->n[0], tmp->L[0], order, kind, idim, vfac, unit);
-    else
-      assemble<double>(tmp->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2, 0),
-                       (const cx<double>*)P(phi3, 1), (const cx<double>*)P(A3x, 2), (const cx<double>*)P(A3y, 3),
-                       (const cx<double>*)P(A3z, 4), (int)tmp->n[0], tmp->L[0], order, kind, idim, vfac, unit);
+    MFX_REQUIRE(plt, "null PLT tables");
+    assemble_impl(tmp, phi, phi2, phi3, A3x, A3y, A3z, order, kind, idim, vfac, unit, plt);
   });
 }
 
@@ -1047,6 +1035,132 @@ int mfx_cosmo_amplitude_table(const mfx_cosmo* cc, size_t n, double boxlen, doub
 
 }  // extern "C"
 
+// ------------------------------------------------------------------ PLT
+struct mfx_plt {
+  mfx_ctx* ctx;
+  int ng;
+  double* dyn;  // six k-space components of the dynamical matrix
+  double* tab;  // D_theta, D_phi, growth-rate factor
+  long long nk() const { return (long long)ng * ng * (ng / 2 + 1); }
+  long long nt() const { return (long long)ng * ng * (ng + 2); }
+};
+
+static void assemble_impl(mfx_grid* tmp, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3x,
+                          mfx_grid* A3y, mfx_grid* A3z, int order, int kind, int idim, double vfac, double unit,
+                          const mfx_plt* plt) {
+  require_grid(tmp);
+  require_grid(phi);
+  require_cubic(tmp);
+  same_shape(tmp, phi);
+  MFX_REQUIRE(order >= 1 && order <= 3, "order must be 1, 2 or 3");
+  MFX_REQUIRE(kind == 0 || kind == 1, "kind must be 0 (displacement) or 1 (velocity)");
+  MFX_REQUIRE(idim >= 0 && idim < 3, "idim must be 0, 1 or 2");
+  MFX_REQUIRE(!plt || plt->ctx == tmp->ctx, "PLT tables belong to another context");
+  mfx_grid* in[5] = {phi2, phi3, A3x, A3y, A3z};
+  const int nin = order == 1 ? 0 : order == 2 ? 1 : 5;
+  for (int q = 0; q < nin; ++q) {
+    require_grid(in[q]);
+    same_shape(tmp, in[q]);
+  }
+  grid_forward(phi, true);
+  for (int q = 0; q < nin; ++q) grid_forward(in[q], true);
+  tmp->space = MFX_KSPACE;
+  auto P = [&](mfx_grid* g, int q) { return (q < nin && g) ? g->d : phi->d; };
+  const double* pt = plt ? plt->tab : nullptr;
+  const int png = plt ? plt->ng : 0;
+  if (tmp->prec == MFX_FLOAT)
+    assemble<float>(tmp->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2, 0),
+                    (const cx<float>*)P(phi3, 1), (const cx<float>*)P(A3x, 2), (const cx<float>*)P(A3y, 3),
+                    (const cx<float>*)P(A3z, 4), (int)tmp->n[0], tmp->L[0], order, kind, idim, vfac, unit, pt, png);
+  else
+    assemble<double>(tmp->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2, 0),
+                     (const cx<double>*)P(phi3, 1), (const cx<double>*)P(A3x, 2), (const cx<double>*)P(A3y, 3),
+                     (const cx<double>*)P(A3z, 4), (int)tmp->n[0], tmp->L[0], order, kind, idim, vfac, unit, pt, png);
+}
+
+static void plt_free(mfx_plt* p) {
+  if (p->dyn) p->ctx->c.release(p->dyn);
+  if (p->tab) p->ctx->c.release(p->tab);
+}
+
+extern "C" {
+
+int mfx_plt_create(mfx_ctx* c, int ngrid, mfx_plt** out) {
+  return guard([&] {
+    MFX_REQUIRE(c && out, "null argument");
+    MFX_REQUIRE(ngrid >= 8 && ngrid <= 256 && ngrid % 2 == 0, "PLT mesh size must be even, 8..256");
+    auto* p = new mfx_plt{c, ngrid, nullptr, nullptr};
+    try {
+      p->dyn = (double*)c->c.alloc(sizeof(double) * 6 * p->nt());
+      p->tab = (double*)c->c.alloc(sizeof(double) * 3 * p->nk());
+      plt_init(c->c, ngrid, p->dyn, p->tab);
+      MFX_HIP(hipStreamSynchronize(c->c.stream));
+    } catch (...) {
+      plt_free(p);
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+int mfx_plt_destroy(mfx_plt* p) {
+  return guard([&] {
+    if (!p) return;
+    MFX_HIP(hipStreamSynchronize(p->ctx->c.stream));
+    plt_free(p);
+    delete p;
+  });
+}
+
+int mfx_plt_tables(const mfx_plt* p, double* host, size_t count) {
+  return guard([&] {
+    MFX_REQUIRE(p && host, "null argument");
+    const long long nk = p->nk();
+    MFX_REQUIRE(count == (size_t)(4 * nk), "PLT tables hold 4 x ng x ng x (ng/2+1) values");
+    for (long long q = 0; q < nk; ++q) host[q] = 1.0;
+    MFX_HIP(hipMemcpyAsync(host + nk, p->tab, sizeof(double) * 3 * nk, hipMemcpyDeviceToHost, p->ctx->c.stream));
+    MFX_HIP(hipStreamSynchronize(p->ctx->c.stream));
+  });
+}
+
+int mfx_plt_dmatrix(const mfx_plt* p, double* host, size_t count) {
+  return guard([&] {
+    MFX_REQUIRE(p && host, "null argument");
+    const long long nk = p->nk(), nt = p->nt();
+    MFX_REQUIRE(count == (size_t)(6 * nk), "the dynamical matrix has 6 x ng x ng x (ng/2+1) real parts");
+    std::vector<double> buf(6 * nt);
+    MFX_HIP(hipMemcpyAsync(buf.data(), p->dyn, sizeof(double) * 6 * nt, hipMemcpyDeviceToHost, p->ctx->c.stream));
+    MFX_HIP(hipStreamSynchronize(p->ctx->c.stream));
+    for (int comp = 0; comp < 6; ++comp)
+      for (long long q = 0; q < nk; ++q) host[comp * nk + q] = buf[comp * nt + 2 * q];
+  });
+}
+
+int mfx_plt_gradient(const mfx_plt* p, size_t n, double boxlen, const long long* ijk, int count, double* g,
+                     double* vfac) {
+  return guard([&] {
+    MFX_REQUIRE(p && ijk && g && vfac && count >= 0, "invalid argument");
+    MFX_REQUIRE(n >= 2 && boxlen > 0, "invalid grid");
+    for (int t = 0; t < 3 * count; ++t)
+      MFX_REQUIRE(ijk[t] >= 0 && ijk[t] < (long long)n && ((t % 3) < 2 || ijk[t] <= (long long)(n / 2)),
+                  "mode index outside the n-grid half spectrum");
+    if (count == 0) return;
+    Ctx& c = p->ctx->c;
+    long long* d_ijk = (long long*)c.alloc(sizeof(long long) * 3 * count);
+    double* d_g = (double*)c.alloc(sizeof(double) * 4 * count);
+    MFX_HIP(hipMemcpyAsync(d_ijk, ijk, sizeof(long long) * 3 * count, hipMemcpyHostToDevice, c.stream));
+    plt_gradients(c, PltView{p->tab, p->ng}, (int)n, boxlen, d_ijk, count, d_g, d_g + 3 * count);
+    MFX_HIP(hipMemcpyAsync(g, d_g, sizeof(double) * 3 * count, hipMemcpyDeviceToHost, c.stream));
+    MFX_HIP(hipMemcpyAsync(vfac, d_g + 3 * count, sizeof(double) * count, hipMemcpyDeviceToHost, c.stream));
+    MFX_HIP(hipStreamSynchronize(c.stream));
+    c.release(d_ijk);
+    c.release(d_g);
+  });
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------------ LPT driver
 struct mfx_icgen {
   mfx_ctx* ctx;
@@ -1056,6 +1170,7 @@ struct mfx_icgen {
   mfx_conv* conv = nullptr;
   double* part[6] = {};
   double* amp = nullptr;
+  mfx_plt* plt = nullptr;
   double dplus = 0, vfact = 0;
 };
 
@@ -1072,6 +1187,7 @@ static void icgen_free(mfx_icgen* ic) {
       p = nullptr;
     }
   if (ic->amp) ic->ctx->c.release(ic->amp);
+  if (ic->plt) mfx_plt_destroy(ic->plt);
 }
 
 extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen** out) {
@@ -1106,6 +1222,11 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);
       ic->amp = (double*)c->c.alloc(sizeof(double) * nt);
       MFX_HIP(hipMemcpy(ic->amp, t.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+      // PLT tables are computed once, outside the timed realisations
+      if (p->use_plt) {
+        int st = mfx_plt_create(c, p->plt_ngrid > 0 ? p->plt_ngrid : 64, &ic->plt);
+        if (st) throw Error(st, g_err);
+      }
       // twiddle tables are created before the first timed run
       if (p->precision == MFX_FLOAT) {
         c->c.twiddles<float>((int)p->n);
@@ -1185,12 +1306,13 @@ extern "C" int mfx_icgen_run(mfx_icgen* ic) {
         if (tmp->prec == MFX_FLOAT)
           assemble<float>(ic->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2),
                           (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
-                          (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact, 1.0);
+                          (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact, 1.0,
+                          ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
         else
           assemble<double>(ic->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2),
                            (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]), (const cx<double>*)P(A3[1]),
                            (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact,
-                           1.0);
+                           1.0, ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
         grid_backward(tmp, true);
         if (tmp->prec == MFX_FLOAT)
           set_particles<float>(ic->ctx->c, (const float*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0, kind);

@@ -8,6 +8,7 @@
 // and velocity factors, position assembly) are applied in double regardless of
 // the storage precision T of the bulk fields.
 #include "ops.hpp"
+#include "plt.hpp"
 
 #include <cmath>
 
@@ -87,12 +88,14 @@ __global__ void k_fill(T* f, long long cnt, T v) {
 
 // ------------------------------------------------------------------ assembly
 // ic_generator.cc:821-860 (kind 0, displacement) and 877-919 (kind 1,
-// velocity) with op::fourier_gradient (operators.hh:38-61, vfac_corr = 1),
-// followed by zero_DC_mode.  Potentials are already growth-scaled.
+// velocity), followed by zero_DC_mode.  Potentials are already growth-scaled.
+// The gradient is op::fourier_gradient (operators.hh:38-61, vfac_corr = 1),
+// or with PLT tables (plt.tab != nullptr) particle::lattice_gradient, whose
+// growth-rate factor then multiplies the velocities (ENABLE_PLT builds).
 template <typename T>
 __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
                            const cx<T>* a3y, const cx<T>* a3z, int n, double k0, int order, int kind, int idim,
-                           double vfac, double unit_over_L) {
+                           double vfac, double unit_over_L, PltView plt, double boxlen) {
   const int nh = n / 2 + 1;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)n * n * nh) return;
@@ -100,9 +103,13 @@ __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, cons
   long long ij = idx / nh;
   int j = (int)(ij % n), i = (int)(ij / n);
   const int ijk[3] = {i, j, k};
-  double g[3];
+  double g[3], vc = 1.0;
+  if (plt.tab) {
+    plt_gradient(plt, n, boxlen, i, j, k, g, vc);
+  } else {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) g[d] = gd_of(ijk[d], n, 1.0) * k0;
+    for (int d = 0; d < 3; ++d) g[d] = gd_of(ijk[d], n, 1.0) * k0;
+  }
   double pr, pi;
   if (kind == 0) {
     pr = phi[idx].x;
@@ -140,6 +147,10 @@ __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, cons
     }
     tr += cr;
     ti += ci;
+  }
+  if (plt.tab && kind == 1) {
+    tr *= vc;
+    ti *= vc;
   }
   tr *= unit_over_L;
   ti *= unit_over_L;
@@ -277,12 +288,13 @@ void fill(Ctx& c, T* f, long long cnt, T v) {
 template <typename T>
 void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
               const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, int kind, int idim, double vfac,
-              double unit) {
+              double unit, const double* plt_tab, int plt_ng) {
   long long cnt = (long long)n * n * (n / 2 + 1);
   int nin = order == 1 ? 1 : order == 2 ? 2 : 5;
-  LaunchScope ls(c, "assemble_kspace", (nin + 1.0) * cnt * sizeof(cx<T>));
+  LaunchScope ls(c, plt_tab ? "assemble_kspace_plt" : "assemble_kspace", (nin + 1.0) * cnt * sizeof(cx<T>));
   hipLaunchKernelGGL(k_assemble<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, tmp, phi, phi2, phi3, a3x, a3y, a3z,
-                     n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen);
+                     n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen, PltView{plt_tab, plt_ng},
+                     boxlen);
   check_launch("assemble");
 }
 
@@ -361,7 +373,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
   template void scale<T>(Ctx&, T*, long long, double);                                                            \
   template void fill<T>(Ctx&, T*, long long, T);                                                                  \
   template void assemble<T>(Ctx&, cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*,   \
-                            const cx<T>*, int, double, int, int, int, double, double);                            \
+                            const cx<T>*, int, double, int, int, int, double, double, const double*, int);        \
   template void set_particles<T>(Ctx&, const T*, double*, int, int, double, int);                                 \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \

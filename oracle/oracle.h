@@ -156,6 +156,28 @@ double orc_cosmo_H(const orc_cosmo_t *c, double a);
 double orc_cosmo_amplitude(const orc_cosmo_t *c, double k);
 double orc_cosmo_sigma8_unnorm(const orc_cosmo_t *c);
 
+/* ---------- PLT corrections, sc lattice (include/particle_plt.hh:48-585) -- */
+/* real-space dynamical matrix on ng^3 sites incl. the r = 0 background
+ * (:161-284); out = 6 padded real grids xx, xy, xz, yy, yz, zz */
+void orc_plt_dynamical_matrix(int ng, double *out);
+/* largest eigenpair of a symmetric 3x3 matrix; eval ascending */
+void orc_sym3_eigen(const double A[3][3], double eval[3], double evec_max[3]);
+/* the same, with the eigenvector of a degenerate largest pair defined as the
+ * projection of kdir onto its eigenspace */
+void orc_plt_top_eigen(const double A[3][3], const double kdir[3], double eval[3], double v[3]);
+/* align_with_k with a fixed sign for transverse eigenvectors */
+void orc_plt_align(double v[3], const double kv[3]);
+/* init_D (:68-515): tabs = 4 tables over ng x ng x (ng/2+1): D_r, D_theta,
+ * D_phi, growth-rate correction; mu1 (optional) largest eigenvalue */
+void orc_plt_tables(int ng, double *tabs, double *mu1);
+/* lattice_gradient::gradient (imaginary parts) and vfac_corr (:551-585) */
+void orc_plt_gradient(const double *tabs, int ng, int n, double boxlen, size_t i, size_t j, size_t k, double g[3],
+                      double *vfac);
+/* assembly with the PLT gradient (ic_generator.cc:821-919, ENABLE_PLT) */
+void orc_assemble_component_plt(const double *phi, const double *phi2, const double *phi3, const double *A3x,
+                                const double *A3y, const double *A3z, const double *tabs, int ng, double *out, int n,
+                                double boxlen, int order, int kind, int idim, double vfac, double unit);
+
 #ifdef __cplusplus
 }
 #endif

@@ -308,3 +308,47 @@ class Cosmology:
         for n2 in range(1, nmax):
             t[n2] = self.amplitude(kfac * np.sqrt(n2))
         return t
+
+
+# ---------------------------------------------------------------- PLT (particle_plt.hh)
+def plt_dynamical_matrix(ng):
+    """6 padded real grids D_xx, D_xy, D_xz, D_yy, D_yz, D_zz on ng^3 sites"""
+    out = np.zeros((6, ng, ng, ng + 2))
+    lib().orc_plt_dynamical_matrix(ctypes.c_int(ng), _p(out))
+    return out
+
+
+def sym3_eigen(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    ev, vec = np.zeros(3), np.zeros(3)
+    lib().orc_sym3_eigen(_p(A), _p(ev), _p(vec))
+    return ev, vec
+
+
+def plt_tables(ng=64, with_mu=False):
+    """(4, ng, ng, ng/2+1): D_r, D_theta, D_phi, growth-rate correction"""
+    tabs = np.zeros((4, ng, ng, ng // 2 + 1))
+    mu = np.zeros((ng, ng, ng // 2 + 1)) if with_mu else None
+    lib().orc_plt_tables(ctypes.c_int(ng), _p(tabs), _p(mu) if with_mu else ctypes.c_void_p(0))
+    return (tabs, mu) if with_mu else tabs
+
+
+def plt_gradient(tabs, n, boxlen, i, j, k):
+    ng = tabs.shape[1]
+    g = np.zeros(3)
+    vf = ctypes.c_double()
+    lib().orc_plt_gradient(_p(tabs), ctypes.c_int(ng), ctypes.c_int(n), ctypes.c_double(boxlen),
+                           ctypes.c_size_t(i), ctypes.c_size_t(j), ctypes.c_size_t(k), _p(g), ctypes.byref(vf))
+    return g, vf.value
+
+
+def assemble_component_plt(phi, phi2, phi3, A3, tabs, boxlen, order, kind, idim, vfac, unit=1.0):
+    n = phi.shape[0]
+    out = np.zeros(real_shape(n))
+    nul = ctypes.c_void_p(0)
+    g = lambda a: _p(a) if a is not None else nul
+    lib().orc_assemble_component_plt(g(phi), g(phi2), g(phi3), g(A3[0]), g(A3[1]), g(A3[2]), _p(tabs),
+                                     ctypes.c_int(tabs.shape[1]), _p(out), ctypes.c_int(n), ctypes.c_double(boxlen),
+                                     ctypes.c_int(order), ctypes.c_int(kind), ctypes.c_int(idim),
+                                     ctypes.c_double(vfac), ctypes.c_double(unit))
+    return out

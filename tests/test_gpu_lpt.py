@@ -71,9 +71,10 @@ def test_lpt_sequence_vs_oracle(ctx, orc, n, order):
             assert rel_max(A3[d].kdata(), orc.kview(rA[d])) < 1e-10
 
 
-def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos):
+def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None):
     """ic_generator::run restated on the CPU: MUSIC1 noise, phi(1..3), A(3),
-    growth scaling and FP64 sc-lattice positions/velocities"""
+    growth scaling and FP64 sc-lattice positions/velocities; with plt_tabs the
+    assembly uses the PLT gradient (ENABLE_PLT builds)"""
     a = 1.0 / (1.0 + zstart)
     D = cos.Dplus(a)
     vfac = cos.vfact(a)
@@ -97,7 +98,10 @@ def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos):
     out = []
     for kind in range(2):
         for idim in range(3):
-            f = orc.assemble_component(phi, phi2, phi3, A3, boxlen, order, kind, idim, vfac, 1.0)
+            if plt_tabs is None:
+                f = orc.assemble_component(phi, phi2, phi3, A3, boxlen, order, kind, idim, vfac, 1.0)
+            else:
+                f = orc.assemble_component_plt(phi, phi2, phi3, A3, plt_tabs, boxlen, order, kind, idim, vfac, 1.0)
             if kind == 0:
                 out.append(orc.set_positions(f, idim, 1.0).reshape(n, n, n))
             else:
