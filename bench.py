@@ -43,9 +43,10 @@ def log(msg):
     sys.stderr.flush()
 
 
-def run_gpu(M, ctx, n, precision, steps, warmup, fused, profile_step=True):
-    params = dict(GridRes=n, BoxLength=1000.0, zstart=24.0, LPTorder=3, precision=precision, generator="MUSIC1",
-                  seed=12345, fused=bool(fused))
+def run_gpu(M, ctx, n, precision, steps, warmup, fused, profile_step=True, generator="MUSIC1", plt=False,
+            boxlen=1000.0):
+    params = dict(GridRes=n, BoxLength=boxlen, zstart=24.0, LPTorder=3, precision=precision, generator=generator,
+                  seed=12345, fused=bool(fused), PLT=bool(plt))
     ic = M.ICGenerator(ctx, **params)
     for w in range(warmup):
         ic.run()
@@ -176,6 +177,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the per-field FP32 vs FP64 error report")
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--generator", default="MUSIC1", choices=["MUSIC1", "NGENIC", "PANPHASIA_THREEFRY"],
+                    help="white-noise plugin (the 1024^3 configuration uses PANPHASIA_THREEFRY)")
+    ap.add_argument("--plt", action="store_true", help="PLT-corrected assembly (the 1024^3 configuration)")
+    ap.add_argument("--box", type=float, default=None, help="box length in Mpc/h (default 1000 * n / 512)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -185,24 +190,27 @@ def main():
     import monofonic_amd as M
     ctx = M.Context(0)
     n = args.n
-    wall, ev_ms, stats, ic32 = run_gpu(M, ctx, n, "float", args.steps, args.warmup, args.fused)
+    box = args.box if args.box else 1000.0 * n / 512
+    extra = dict(generator=args.generator, plt=args.plt, boxlen=box)
+    wall, ev_ms, stats, ic32 = run_gpu(M, ctx, n, "float", args.steps, args.warmup, args.fused, **extra)
     ms_per_step = ev_ms / args.steps
     value = n ** 3 / (ms_per_step * 1e-3)
     roof, fft_gbps = roofline(stats)
-    workload_id = f"3lpt-{n}-f32" + ("" if args.fused else "-seq")
+    workload_id = f"3lpt-{n}-f32" + ("" if args.fused else "-seq") + ("-plt" if args.plt else "") + \
+        ("" if args.generator == "MUSIC1" else "-" + args.generator.lower())
     tr = profiled_traffic(roof["kernel"], workload_id)
     roof["traffic"] = round(tr) if tr else None
 
     fp64 = None
     acc = None
     if not args.no_fp64:
-        w64, e64, st64, ic64 = run_gpu(M, ctx, n, "double", max(1, args.steps // 2), 1, args.fused)
+        w64, e64, st64, ic64 = run_gpu(M, ctx, n, "double", max(1, args.steps // 2), 1, args.fused, **extra)
         ms64 = e64 / max(1, args.steps // 2)
         _, fft64 = roofline(st64)
         fp64 = {"value": round(n ** 3 / (ms64 * 1e-3), 1), "unit": "particles/s", "ms_per_step": round(ms64, 3),
                 "fft_gbps": round(fft64, 1) if fft64 else None, "fp32_speedup": round(ms64 / ms_per_step, 3)}
         if not args.no_accuracy:
-            fields = accuracy(M, ctx, ic32, ic64, n, 1000.0)
+            fields = accuracy(M, ctx, ic32, ic64, n, box)
             acc = {"reference": "FP64-bulk GPU path, same seed (it matches the FP64 CPU oracle to 1e-11 rms at "
                                 "32^3-64^3 in the parity tests)",
                    "normalisation": "rms_rel, max_rel: |FP32 - FP64| / RMS of the FP64 field (sigma_Psi for Psi)",
@@ -222,10 +230,11 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"3LPT sc {n}^3, box 1000 Mpc/h, z_start 24, MUSIC1 noise, Eisenstein-Hu, "
-                               f"FP32 bulk ({3 * n // 2}^3 padded) + FP64 tables/assembly",
+        "config": {"workload": f"3LPT sc {n}^3, box {box:g} Mpc/h, z_start 24, {args.generator} noise, "
+                               f"Eisenstein-Hu, FP32 bulk ({3 * n // 2}^3 padded) + FP64 tables/assembly"
+                               + (", PLT corrections" if args.plt else ""),
                    "n": n, "lpt_order": 3, "padded": 3 * n // 2, "fused_lpt": bool(args.fused),
-                   "wall_s": round(wall, 4)},
+                   "generator": args.generator, "plt": bool(args.plt), "wall_s": round(wall, 4)},
         "roofline": roof, "fft_gbps": round(fft_gbps, 1) if fft_gbps else None,
         "fft_frac": round(fft_gbps / HBM_PEAK_GBPS, 4) if fft_gbps else None,
         "fp64": fp64, "accuracy": acc, "cpu_baseline": cpu,

@@ -251,7 +251,9 @@ int mfx_icgen_destroy(mfx_icgen *ic);
 /* one full realisation; results stay on the device */
 int mfx_icgen_run(mfx_icgen *ic);
 /* device pointers of the FP64 particle arrays: what 0..2 positions x,y,z,
- * 3..5 velocities x,y,z (n^3 doubles each, order (i,j,k)) */
+ * 3..5 velocities x,y,z (n^3 doubles each, order (i,j,k)).  With the fused
+ * LPT engine they share device memory with its workspace: valid until the
+ * next mfx_icgen_run */
 int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
 /* grids of the realisation: 0 wnoise, 1 phi, 2 phi2, 3 phi3, 4..6 A3, 7 tmp */
 int mfx_icgen_grid(mfx_icgen *ic, int which, mfx_grid **g);

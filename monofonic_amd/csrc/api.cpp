@@ -527,13 +527,9 @@ int mfx_conv_create(mfx_ctx* c, size_t n, double L, int precision, mfx_conv** ou
     cv->n = (int)n;
     cv->L = L;
     cv->prec = precision;
+    // the workspaces of the two-operand chain and of the fused LPT engine are
+    // allocated on first use: a caller of one never pays for the other
     cv->ws_bytes = conv_workspace_bytes((int)n, precision);
-    try {
-      cv->ws = c->c.alloc(cv->ws_bytes);
-    } catch (...) {
-      delete cv;
-      throw;
-    }
     *out = cv;
   });
 }
@@ -590,6 +586,7 @@ static void conv2_impl(mfx_conv* cv, mfx_grid* a, const MultSpec& ma, mfx_grid* 
   MFX_REQUIRE(out->space == MFX_KSPACE || op == MFX_OP_ASSIGN, "output grid of an accumulating op must be in k space");
   out->space = MFX_KSPACE;
   Ctx& c = cv->ctx->c;
+  if (!cv->ws) cv->ws = c.alloc(cv->ws_bytes);
   if (cv->prec == MFX_FLOAT)
     conv2<float>(c, (const cx<float>*)a->d, ma, (const cx<float>*)b->d, mb, (cx<float>*)out->d, op, opfac, cv->n,
                  cv->L, cv->ws);
@@ -788,7 +785,8 @@ static void lpt_sequence(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* 
   }
 }
 
-static void lpt_fused(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3[3], int order) {
+// workspace of the fused engine for LPT order `order` (grown on demand)
+static void lpt_reserve(mfx_conv* cv, int order) {
   const size_t need = lpt_workspace_bytes(cv->n, cv->prec, order);
   if (cv->lpt_bytes < need) {
     MFX_HIP(hipStreamSynchronize(cv->ctx->c.stream));
@@ -798,6 +796,10 @@ static void lpt_fused(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi
     cv->lpt_ws = cv->ctx->c.alloc(need);
     cv->lpt_bytes = need;
   }
+}
+
+static void lpt_fused(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3[3], int order) {
+  lpt_reserve(cv, order);
   grid_forward(phi, true);
   phi2->space = MFX_KSPACE;
   if (order > 2) {
@@ -1171,6 +1173,7 @@ struct mfx_icgen {
   double* part[6] = {};
   double* amp = nullptr;
   mfx_plt* plt = nullptr;
+  bool part_in_ws = false;  // particle arrays live in the LPT workspace
   double dplus = 0, vfact = 0;
 };
 
@@ -1181,11 +1184,10 @@ static void icgen_free(mfx_icgen* ic) {
       gg = nullptr;
     }
   if (ic->conv) mfx_conv_destroy(ic->conv);
-  for (auto*& p : ic->part)
-    if (p) {
-      ic->ctx->c.release(p);
-      p = nullptr;
-    }
+  for (auto*& p : ic->part) {
+    if (p && !ic->part_in_ws) ic->ctx->c.release(p);
+    p = nullptr;
+  }
   if (ic->amp) ic->ctx->c.release(ic->amp);
   if (ic->plt) mfx_plt_destroy(ic->plt);
 }
@@ -1216,7 +1218,20 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
         int st = mfx_conv_create(c, p->n, p->boxlen, p->precision, &ic->conv);
         if (st) throw Error(st, g_err);
       }
-      for (auto*& pp : ic->part) pp = (double*)c->c.alloc(sizeof(double) * p->n * p->n * p->n);
+      // The FP64 particle arrays are written after the LPT sources are
+      // complete, so with the fused engine they share its workspace (at 1024^3
+      // FP32 this keeps the realisation within one GPU's HBM); they stay valid
+      // until the next run.
+      const size_t npart = p->n * p->n * p->n;
+      if (ic->conv && p->use_fused_lpt) {
+        lpt_reserve(ic->conv, p->lpt_order);
+        if (ic->conv->lpt_bytes >= 6 * npart * sizeof(double)) {
+          ic->part_in_ws = true;
+          for (int w = 0; w < 6; ++w) ic->part[w] = (double*)ic->conv->lpt_ws + w * npart;
+        }
+      }
+      if (!ic->part_in_ws)
+        for (auto*& pp : ic->part) pp = (double*)c->c.alloc(sizeof(double) * npart);
       const size_t nt = 3 * (p->n / 2) * (p->n / 2) + 1;
       std::vector<double> t(nt);
       amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);

@@ -65,3 +65,27 @@ def test_plane_wave_unitary(ctx):
     assert abs(k[a, b, c] - amp) < 1e-9
     k[a, b, c] = 0
     assert np.max(np.abs(k)) < 1e-9
+
+
+@pytest.mark.parametrize("shape", [(1536, 16, 1536), (16, 1536, 1536), (1024, 16, 1024), (288, 24, 384)])
+@pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("float", 3e-6)])
+def test_r2c_long_axes_vs_numpy(ctx, shape, prec, tol):
+    """the column (1536, 1024) and row (768, 512 complex) lengths that only the
+    1024^3 configuration uses, on thin non-cubic grids, against numpy's rfftn
+    (pocketfft) with the unitary normalisation"""
+    import monofonic_amd as M
+    n0, n1, n2 = shape
+    print(f"r2c {shape} {prec}", flush=True)
+    rng = np.random.default_rng(n0 + n1 + n2)
+    x = rng.standard_normal((n0, n1, n2))
+    a = np.zeros((n0, n1, n2 + 2))
+    a[:, :, :n2] = x
+    g = M.Grid_FFT(ctx, shape, (100.0, 100.0, 100.0), prec)
+    g.upload(a)
+    g.FourierTransformForward()
+    ref = np.fft.rfftn(x) / np.sqrt(n0 * n1 * n2)
+    got = g.kdata()
+    assert rel_rms(got, ref) < tol
+    assert rel_max(got, ref) < 20 * tol
+    g.FourierTransformBackward()
+    assert rel_max(g.rdata(), x) < 20 * tol

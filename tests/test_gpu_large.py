@@ -1,0 +1,72 @@
+"""The 1024^3 configuration (1536^3 padded, FP32 bulk) through a property that
+does not depend on the grid size: for a band-limited potential whose products
+stay below a third of the Nyquist wave number, the Orszag 3/2 products are
+exact, so phi2, phi3 and A3 are the same continuous fields at every
+resolution.  The 1024^3 fields sampled at every 16th point must equal the
+oracle's 64^3 FP64 fields (ic_generator.cc:464-547)."""
+import numpy as np
+import pytest
+
+BOX = 300.0
+# (mode (mx, my, mz), amplitude): sums of up to three modes stay within
+# n/3 = 21 of the 64^3 grid in every component
+MODES = [((1, 2, 0), 1.0), ((0, 1, 3), 0.7), ((2, 0, 1), -0.5), ((1, -1, 2), 0.4)]
+
+
+def kspace_phi(n, dtype):
+    """k-space grid (padded layout) of phi(x) = sum_m a_m cos(2 pi m.x / n),
+    unitary normalisation: coefficient a sqrt(n^3)/2 at m and -m"""
+    a = np.zeros((n, n, n + 2), dtype=dtype)
+    k = a.view(np.complex128 if dtype == np.float64 else np.complex64).reshape(n, n, n // 2 + 1)
+    s = np.sqrt(float(n) ** 3) / 2
+    for (mx, my, mz), amp in MODES:
+        if mz > 0:
+            k[mx % n, my % n, mz] += amp * s
+        elif mz < 0:
+            k[(-mx) % n, (-my) % n, -mz] += amp * s
+        else:  # both +m and -m lie in the stored kz = 0 plane
+            k[mx % n, my % n, 0] += amp * s
+            k[(-mx) % n, (-my) % n, 0] += amp * s
+    return a
+
+
+def test_kspace_phi_is_the_cosine_sum(orc):
+    n = 16
+    a = kspace_phi(n, np.float64)
+    orc.fft_c2r(a)
+    x = np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    f = sum(amp * np.cos(2 * np.pi * (m[0] * X + m[1] * Y + m[2] * Z) / n) for m, amp in MODES)
+    assert np.max(np.abs(a[:, :, :n] - f)) < 1e-13
+
+
+@pytest.mark.gpu
+def test_lpt_sources_1024_band_limited(ctx, orc):
+    import monofonic_amd as M
+    n, ns, step = 1024, 64, 16
+    print("band-limited 3LPT sources at 1024^3 FP32 (1536^3 padded)", flush=True)
+    phi = M.Grid_FFT(ctx, n, BOX, "float", initialspace=M.kspace_id)
+    phi.upload(kspace_phi(n, np.float32))
+    out = [M.Grid_FFT(ctx, n, BOX, "float", initialspace=M.kspace_id) for _ in range(5)]
+    conv = M.OrszagConvolver(ctx, n, BOX, "float")
+    conv.lpt_sources(phi, out[0], out[1], out[2:], 3)
+    ctx.synchronize()
+    del conv
+    print("  sources done", flush=True)
+    r2, r3, rA = orc.lpt_terms(kspace_phi(ns, np.float64), BOX, 3)
+    refs = [r2, r3] + list(rA)
+    names = ["phi2", "phi3", "A3x", "A3y", "A3z"]
+    for g, r, name in zip(out, refs, names):
+        g.FourierTransformBackward()
+        got = g.rdata()[::step, ::step, ::step].astype(np.float64)
+        g.reset()
+        ref = r.copy()
+        orc.fft_c2r(ref)
+        ref = ref[:, :, :ns]
+        scale = np.sqrt(np.mean(ref ** 2))
+        err = np.sqrt(np.mean((got - ref) ** 2)) / scale
+        mx = np.max(np.abs(got - ref)) / scale
+        print(f"  {name}: rms {err:.2e} max {mx:.2e} (field rms {scale:.3e})", flush=True)
+        assert scale > 0
+        assert err < 2e-5, name
+        assert mx < 2e-4, name
