@@ -248,8 +248,19 @@ typedef struct {
 typedef struct mfx_icgen mfx_icgen;
 int mfx_icgen_create(mfx_ctx *c, const mfx_icgen_params *p, mfx_icgen **out);
 int mfx_icgen_destroy(mfx_icgen *ic);
-/* one full realisation; results stay on the device */
+/* one full realisation with the seed of the parameters; results stay on the
+ * device */
 int mfx_icgen_run(mfx_icgen *ic);
+/* Graph replay of realisations (no counterpart in the reference, which runs
+ * one realisation per process).  mfx_icgen_set_seed sets the white-noise seed
+ * of the next realisation and starts its host part (for NGENIC the seed table,
+ * computed by a worker thread); mfx_icgen_run or mfx_icgen_launch copies it to
+ * the device.  mfx_icgen_capture runs one realisation and captures the device
+ * sequence into a HIP graph; mfx_icgen_launch replays it on the context's
+ * stream.  Replays give the same results as mfx_icgen_run with that seed. */
+int mfx_icgen_set_seed(mfx_icgen *ic, long seed);
+int mfx_icgen_capture(mfx_icgen *ic);
+int mfx_icgen_launch(mfx_icgen *ic);
 /* device pointers of the FP64 particle arrays: what 0..2 positions x,y,z,
  * 3..5 velocities x,y,z (n^3 doubles each, order (i,j,k)).  With the fused
  * LPT engine they share device memory with its workspace: valid until the

@@ -30,6 +30,7 @@ EXPORTS = [
     "mfx_rng_music1_fill", "mfx_rng_ngenic_fill", "mfx_rng_threefry_fill", "mfx_rng_mt19937_raw",
     "mfx_rng_ranlxd1_raw", "mfx_rng_threefry_uniforms", "mfx_cosmo_create", "mfx_cosmo_destroy",
     "mfx_cosmo_query", "mfx_cosmo_amplitude_table", "mfx_icgen_create", "mfx_icgen_destroy", "mfx_icgen_run",
+    "mfx_icgen_set_seed", "mfx_icgen_capture", "mfx_icgen_launch",
     "mfx_icgen_particles", "mfx_icgen_grid", "mfx_icgen_scalars",
 ]
 
@@ -118,6 +119,7 @@ def load():
         "mfx_cosmo_query": [vp, i32, dbl, ctypes.POINTER(dbl)],
         "mfx_cosmo_amplitude_table": [vp, sz, dbl, vp, sz],
         "mfx_icgen_create": [vp, vp, pp], "mfx_icgen_destroy": [vp], "mfx_icgen_run": [vp],
+        "mfx_icgen_set_seed": [vp, ctypes.c_long], "mfx_icgen_capture": [vp], "mfx_icgen_launch": [vp],
         "mfx_icgen_particles": [vp, i32, pp], "mfx_icgen_grid": [vp, i32, pp],
         "mfx_icgen_scalars": [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)],
     }

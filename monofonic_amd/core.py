@@ -509,6 +509,18 @@ class ICGenerator:
     def run(self):
         call("mfx_icgen_run", self._h)
 
+    def set_seed(self, seed):
+        """stage the noise seed of the next realisation (host part of the fill)"""
+        call("mfx_icgen_set_seed", self._h, int(seed))
+
+    def capture(self):
+        """run once and capture the device sequence of a realisation as a HIP graph"""
+        call("mfx_icgen_capture", self._h)
+
+    def launch(self):
+        """replay the captured realisation for the staged seed (asynchronous)"""
+        call("mfx_icgen_launch", self._h)
+
     def scalars(self):
         d, v = ctypes.c_double(), ctypes.c_double()
         call("mfx_icgen_scalars", self._h, ctypes.byref(d), ctypes.byref(v))

@@ -2,9 +2,11 @@
 // counterpart of ic_generator::run for the sc lattice.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <memory>
 #include <string>
 #include <vector>
@@ -90,6 +92,9 @@ void Ctx::release(void* p) {
 struct mfx_ctx {
   Ctx c;
   hipEvent_t ev[16] = {};
+  // grids, convolvers, generators and PLT tables created on the context:
+  // they use its stream, so the context outlives them
+  std::atomic<int> children{0};
 };
 
 struct mfx_grid {
@@ -143,6 +148,9 @@ int mfx_ctx_create(int device, mfx_ctx** out) {
 int mfx_ctx_destroy(mfx_ctx* c) {
   return guard([&] {
     if (!c) return;
+    MFX_REQUIRE(c->children.load() == 0,
+                "context still has " + std::to_string(c->children.load()) +
+                    " grids/convolvers/generators/PLT tables; destroy them first");
     (void)hipSetDevice(c->c.device);
     (void)hipStreamSynchronize(c->c.stream);
     for (auto& e : c->ev) (void)hipEventDestroy(e);
@@ -281,6 +289,7 @@ int mfx_grid_create(mfx_ctx* c, size_t n0, size_t n1, size_t n2, double L0, doub
       delete g;
       throw;
     }
+    c->children++;
     *out = g;
   });
 }
@@ -290,6 +299,7 @@ int mfx_grid_destroy(mfx_grid* g) {
     if (!g) return;
     MFX_HIP(hipStreamSynchronize(g->ctx->c.stream));
     g->ctx->c.release(g->d);
+    g->ctx->children--;
     delete g;
   });
 }
@@ -530,6 +540,7 @@ int mfx_conv_create(mfx_ctx* c, size_t n, double L, int precision, mfx_conv** ou
     // the workspaces of the two-operand chain and of the fused LPT engine are
     // allocated on first use: a caller of one never pays for the other
     cv->ws_bytes = conv_workspace_bytes((int)n, precision);
+    c->children++;
     *out = cv;
   });
 }
@@ -544,6 +555,7 @@ int mfx_conv_destroy(mfx_conv* cv) {
       cv->ctx->c.release(cv->fbuf->d);
       delete cv->fbuf;
     }
+    cv->ctx->children--;
     delete cv;
   });
 }
@@ -869,63 +881,85 @@ int mfx_set_particles(mfx_grid* field, double* out_dev, int idim, double lunit, 
 // ------------------------------------------------------------------ RNG
 static bool is_pow2(size_t x) { return x && !(x & (x - 1)); }
 
-static void music1_impl(mfx_grid* g, long seed, int cubesize) {
+// Validation of the noise seeds (random_music.cc:70-74,186-190;
+// uniform_rand_threefry4x64.c seeding)
+static void check_music1(const mfx_grid* g, long seed, int cubesize) {
   require_cubic(g);
-  // random_music.cc:70-74 and 186-190
   MFX_REQUIRE(is_pow2(g->n[0]), "MUSIC1 random number plugin requires GridRes to be a power of 2");
   MFX_REQUIRE(seed > 0, "Seed values need to be >0");
   MFX_REQUIRE(cubesize > 0, "cube size must be positive");
+}
+static void check_threefry(const mfx_grid* g, unsigned long long seed) {
+  require_cubic(g);
+  MFX_REQUIRE(is_pow2(g->n[0]), "Threefry fill requires a power-of-two grid");
+  MFX_REQUIRE((seed >> 32) == 0, "Threefry seed must satisfy 0 <= seed < 2^32");
+}
+
+// fills from device-resident seeds (stream-ordered, allocation-free)
+static void music1_dev(mfx_grid* g, const long* dseed, int cubesize, void* scratch) {
   if (g->prec == MFX_FLOAT)
-    music1_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], seed, cubesize, 1);
+    music1_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], dseed, cubesize, 1, scratch);
   else
-    music1_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], seed, cubesize, 1);
+    music1_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], dseed, cubesize, 1, scratch);
+  g->space = MFX_RSPACE;
+}
+static void ngenic_dev(mfx_grid* g, const unsigned* dtable) {
+  if (g->prec == MFX_FLOAT)
+    ngenic_fill<float>(g->ctx->c, (cx<float>*)g->d, (int)g->n[0], dtable);
+  else
+    ngenic_fill<double>(g->ctx->c, (cx<double>*)g->d, (int)g->n[0], dtable);
+  g->space = MFX_KSPACE;
+}
+static void threefry_dev(mfx_grid* g, const unsigned long long* dseed) {
+  if (g->prec == MFX_FLOAT)
+    threefry_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], dseed);
+  else
+    threefry_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], dseed);
   g->space = MFX_RSPACE;
 }
 
+// one-shot fills of the RNG_plugin interface: temporaries, blocking
 int mfx_rng_music1_fill(mfx_grid* g, long seed, int cubesize) {
   return guard([&] {
     require_grid(g);
-    music1_impl(g, seed, cubesize);
+    check_music1(g, seed, cubesize);
+    Ctx& c = g->ctx->c;
+    long* ds = (long*)c.alloc(sizeof(long));
+    void* scr = c.alloc(music1_scratch_bytes((int)g->n[0], cubesize));
+    MFX_HIP(hipMemcpyAsync(ds, &seed, sizeof(long), hipMemcpyHostToDevice, c.stream));
+    music1_dev(g, ds, cubesize, scr);
+    MFX_HIP(hipStreamSynchronize(c.stream));
+    c.release(ds);
+    c.release(scr);
   });
-}
-
-static void ngenic_impl(mfx_grid* g, long seed) {
-  require_cubic(g);
-  const int n = (int)g->n[0];
-  std::vector<unsigned> table((size_t)n * n);
-  ngenic_seedtable(table.data(), n, seed);
-  Ctx& c = g->ctx->c;
-  unsigned* dt = (unsigned*)c.alloc(sizeof(unsigned) * table.size());
-  MFX_HIP(hipMemcpyAsync(dt, table.data(), sizeof(unsigned) * table.size(), hipMemcpyHostToDevice, c.stream));
-  if (g->prec == MFX_FLOAT)
-    ngenic_fill<float>(c, (cx<float>*)g->d, n, dt);
-  else
-    ngenic_fill<double>(c, (cx<double>*)g->d, n, dt);
-  MFX_HIP(hipStreamSynchronize(c.stream));
-  c.release(dt);
-  g->space = MFX_KSPACE;
 }
 
 int mfx_rng_ngenic_fill(mfx_grid* g, long seed) {
   return guard([&] {
     require_grid(g);
-    ngenic_impl(g, seed);
+    require_cubic(g);
+    const int n = (int)g->n[0];
+    std::vector<unsigned> table((size_t)n * n);
+    ngenic_seedtable(table.data(), n, seed);
+    Ctx& c = g->ctx->c;
+    unsigned* dt = (unsigned*)c.alloc(sizeof(unsigned) * table.size());
+    MFX_HIP(hipMemcpyAsync(dt, table.data(), sizeof(unsigned) * table.size(), hipMemcpyHostToDevice, c.stream));
+    ngenic_dev(g, dt);
+    MFX_HIP(hipStreamSynchronize(c.stream));
+    c.release(dt);
   });
-}
-
-static void threefry_impl(mfx_grid* g, unsigned long long seed) {
-  require_cubic(g);
-  if (g->prec == MFX_FLOAT)
-    threefry_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], seed);
-  else
-    threefry_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], seed);
-  g->space = MFX_RSPACE;
 }
 
 int mfx_rng_threefry_fill(mfx_grid* g, unsigned long long seed) {
   return guard([&] {
     require_grid(g);
-    threefry_impl(g, seed);
+    check_threefry(g, seed);
+    Ctx& c = g->ctx->c;
+    unsigned long long* ds = (unsigned long long*)c.alloc(sizeof(unsigned long long));
+    MFX_HIP(hipMemcpyAsync(ds, &seed, sizeof(seed), hipMemcpyHostToDevice, c.stream));
+    threefry_dev(g, ds);
+    MFX_HIP(hipStreamSynchronize(c.stream));
+    c.release(ds);
   });
 }
 
@@ -1102,6 +1136,7 @@ int mfx_plt_create(mfx_ctx* c, int ngrid, mfx_plt** out) {
       delete p;
       throw;
     }
+    c->children++;
     *out = p;
   });
 }
@@ -1111,6 +1146,7 @@ int mfx_plt_destroy(mfx_plt* p) {
     if (!p) return;
     MFX_HIP(hipStreamSynchronize(p->ctx->c.stream));
     plt_free(p);
+    p->ctx->children--;
     delete p;
   });
 }
@@ -1175,9 +1211,22 @@ struct mfx_icgen {
   mfx_plt* plt = nullptr;
   bool part_in_ws = false;  // particle arrays live in the LPT workspace
   double dplus = 0, vfact = 0;
+  // noise seed of the next realisation, staged host -> device before the
+  // device sequence runs (or its graph is launched)
+  void* dev_seed = nullptr;          // 8 bytes
+  unsigned* dev_table = nullptr;     // NGENIC seed table, n^2
+  void* rng_scratch = nullptr;       // MUSIC1 partial sums
+  void* host_stage[2] = {};          // pinned: seed at 0, NGENIC table at 16
+  hipEvent_t staged[2] = {};         // last copy out of each staging slot
+  int slot = 0;
+  bool have_staged = false;          // a seed has been staged for the next run
+  std::future<void> table_job;       // NGENIC seed table being computed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 };
 
 static void icgen_free(mfx_icgen* ic) {
+  if (ic->table_job.valid()) ic->table_job.wait();
   for (auto*& gg : ic->g)
     if (gg) {
       mfx_grid_destroy(gg);
@@ -1190,6 +1239,15 @@ static void icgen_free(mfx_icgen* ic) {
   }
   if (ic->amp) ic->ctx->c.release(ic->amp);
   if (ic->plt) mfx_plt_destroy(ic->plt);
+  if (ic->gexec) (void)hipGraphExecDestroy(ic->gexec);
+  if (ic->graph) (void)hipGraphDestroy(ic->graph);
+  if (ic->dev_seed) ic->ctx->c.release(ic->dev_seed);
+  if (ic->dev_table) ic->ctx->c.release(ic->dev_table);
+  if (ic->rng_scratch) ic->ctx->c.release(ic->rng_scratch);
+  for (int q = 0; q < 2; ++q) {
+    if (ic->host_stage[q]) (void)hipHostFree(ic->host_stage[q]);
+    if (ic->staged[q]) (void)hipEventDestroy(ic->staged[q]);
+  }
 }
 
 extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen** out) {
@@ -1237,6 +1295,15 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);
       ic->amp = (double*)c->c.alloc(sizeof(double) * nt);
       MFX_HIP(hipMemcpy(ic->amp, t.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+      // noise seed staging (pinned, double-buffered) and device-side seeds
+      ic->dev_seed = c->c.alloc(16);
+      const size_t table_bytes = p->generator == 1 ? sizeof(unsigned) * p->n * p->n : 0;
+      if (p->generator == 1) ic->dev_table = (unsigned*)c->c.alloc(table_bytes);
+      if (p->generator == 0) ic->rng_scratch = c->c.alloc(music1_scratch_bytes((int)p->n, 32));
+      for (int q = 0; q < 2; ++q) {
+        MFX_HIP(hipHostMalloc(&ic->host_stage[q], 16 + table_bytes, hipHostMallocDefault));
+        MFX_HIP(hipEventCreateWithFlags(&ic->staged[q], hipEventDisableTiming));
+      }
       // PLT tables are computed once, outside the timed realisations
       if (p->use_plt) {
         int st = mfx_plt_create(c, p->plt_ngrid > 0 ? p->plt_ngrid : 64, &ic->plt);
@@ -1255,6 +1322,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       delete ic;
       throw;
     }
+    c->children++;
     *out = ic;
   });
 }
@@ -1264,77 +1332,174 @@ extern "C" int mfx_icgen_destroy(mfx_icgen* ic) {
     if (!ic) return;
     MFX_HIP(hipStreamSynchronize(ic->ctx->c.stream));
     icgen_free(ic);
+    ic->ctx->children--;
     delete ic;
   });
+}
+
+// Host half of the noise fill for the next realisation.  The seed (and for
+// NGENIC the per-column seed table of random_ngenic.cc:230-265, a serial
+// RANLUX stream of n^2 draws, computed by a worker thread so that it overlaps
+// the device work already enqueued) is written to one of two pinned staging
+// slots; icgen_commit_seed then enqueues the copy to the device.  A slot is
+// reused only after its previous copy has run.
+static void icgen_stage_seed(mfx_icgen* ic, long seed) {
+  const mfx_icgen_params& p = ic->p;
+  mfx_grid* wn = ic->g[0];
+  if (p.generator == 0) check_music1(wn, seed, 32);
+  if (p.generator == 2) check_threefry(wn, (unsigned long long)seed);
+  if (ic->table_job.valid()) ic->table_job.wait();
+  const int q = ic->slot ^ 1;
+  ic->slot = q;
+  MFX_HIP(hipEventSynchronize(ic->staged[q]));
+  char* h = (char*)ic->host_stage[q];
+  const long long s64 = (long long)seed;
+  std::memcpy(h, &s64, sizeof(s64));
+  if (p.generator == 1) {
+    const int n = (int)p.n;
+    ic->table_job = std::async(std::launch::async, [h, n, seed] { ngenic_seedtable((unsigned*)(h + 16), n, seed); });
+  }
+  ic->have_staged = true;
+}
+
+static void icgen_commit_seed(mfx_icgen* ic) {
+  if (!ic->have_staged) icgen_stage_seed(ic, ic->p.seed);
+  if (ic->table_job.valid()) ic->table_job.get();
+  Ctx& c = ic->ctx->c;
+  const int q = ic->slot;
+  char* h = (char*)ic->host_stage[q];
+  MFX_HIP(hipMemcpyAsync(ic->dev_seed, h, 8, hipMemcpyHostToDevice, c.stream));
+  if (ic->p.generator == 1)
+    MFX_HIP(hipMemcpyAsync(ic->dev_table, h + 16, sizeof(unsigned) * ic->p.n * ic->p.n, hipMemcpyHostToDevice,
+                           c.stream));
+  MFX_HIP(hipEventRecord(ic->staged[q], c.stream));
+  ic->have_staged = false;
+}
+
+// The device sequence of one realisation, from the staged seed to the FP64
+// particles: stream-ordered kernels only (no allocation, host copy, memset or
+// synchronisation once the generator has run once), so it can be captured.
+static void icgen_device_run(mfx_icgen* ic) {
+  const mfx_icgen_params& p = ic->p;
+  mfx_grid *wn = ic->g[0], *phi = ic->g[1], *phi2 = ic->g[2], *phi3 = ic->g[3], *tmp = ic->g[7];
+  mfx_grid* A3[3] = {ic->g[4], ic->g[5], ic->g[6]};
+  const double D = ic->dplus;
+  const double g1 = -D, g2 = -3.0 / 7.0 * D * D, g3 = 1.0 / 3.0 * D * D * D, g3c = 1.0 / 7.0 * D * D * D;
+  const double volfac = std::pow(p.boxlen / (double)p.n / 2.0 / M_PI, 1.5);
+  // white noise (ic_generator.cc:249-257)
+  if (p.generator == 0)
+    music1_dev(wn, (const long*)ic->dev_seed, 32, ic->rng_scratch);
+  else if (p.generator == 1)
+    ngenic_dev(wn, ic->dev_table);
+  else
+    threefry_dev(wn, (const unsigned long long*)ic->dev_seed);
+  grid_forward(wn, true);
+  // normalisation (341-346) and phi(1) (450-457)
+  if (wn->prec == MFX_FLOAT)
+    noise_normalise<float>(ic->ctx->c, (cx<float>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
+  else
+    noise_normalise<double>(ic->ctx->c, (cx<double>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
+  phi->space = MFX_KSPACE;
+  phi1_impl(phi, wn, ic->amp);
+  // phi(2), phi(3), A(3)
+  if (p.lpt_order > 1) {
+    if (p.use_fused_lpt)
+      lpt_fused(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
+    else
+      lpt_sequence(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
+  }
+  // growth factors (550-563)
+  auto sc = [&](mfx_grid* g, double s) {
+    if (g->prec == MFX_FLOAT)
+      scale<float>(ic->ctx->c, (float*)g->d, (long long)g->ntot, s);
+    else
+      scale<double>(ic->ctx->c, (double*)g->d, (long long)g->ntot, s);
+  };
+  sc(phi, g1);
+  if (p.lpt_order > 1) sc(phi2, g2);
+  if (p.lpt_order > 2) {
+    sc(phi3, g3);
+    for (auto* a : A3) sc(a, g3c);
+  }
+  // positions and velocities (821-931), lunit = vunit = 1
+  for (int kind = 0; kind < 2; ++kind)
+    for (int idim = 0; idim < 3; ++idim) {
+      tmp->space = MFX_KSPACE;
+      auto P = [](mfx_grid* g) { return g ? g->d : nullptr; };
+      if (tmp->prec == MFX_FLOAT)
+        assemble<float>(ic->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2),
+                        (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
+                        (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact, 1.0,
+                        ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
+      else
+        assemble<double>(ic->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2),
+                         (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]), (const cx<double>*)P(A3[1]),
+                         (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact,
+                         1.0, ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
+      grid_backward(tmp, true);
+      if (tmp->prec == MFX_FLOAT)
+        set_particles<float>(ic->ctx->c, (const float*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0, kind);
+      else
+        set_particles<double>(ic->ctx->c, (const double*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0,
+                              kind);
+    }
 }
 
 extern "C" int mfx_icgen_run(mfx_icgen* ic) {
   return guard([&] {
     MFX_REQUIRE(ic, "null generator");
-    const mfx_icgen_params& p = ic->p;
-    mfx_grid *wn = ic->g[0], *phi = ic->g[1], *phi2 = ic->g[2], *phi3 = ic->g[3], *tmp = ic->g[7];
-    mfx_grid* A3[3] = {ic->g[4], ic->g[5], ic->g[6]};
-    const double D = ic->dplus;
-    const double g1 = -D, g2 = -3.0 / 7.0 * D * D, g3 = 1.0 / 3.0 * D * D * D, g3c = 1.0 / 7.0 * D * D * D;
-    const double volfac = std::pow(p.boxlen / (double)p.n / 2.0 / M_PI, 1.5);
-    // white noise (ic_generator.cc:249-257)
-    if (p.generator == 0) {
-      music1_impl(wn, p.seed, 32);
-    } else if (p.generator == 1) {
-      ngenic_impl(wn, p.seed);
-    } else {
-      threefry_impl(wn, (unsigned long long)p.seed);
+    icgen_commit_seed(ic);
+    icgen_device_run(ic);
+  });
+}
+
+extern "C" int mfx_icgen_set_seed(mfx_icgen* ic, long seed) {
+  return guard([&] {
+    MFX_REQUIRE(ic, "null generator");
+    icgen_stage_seed(ic, seed);
+    ic->p.seed = seed;
+  });
+}
+
+extern "C" int mfx_icgen_capture(mfx_icgen* ic) {
+  return guard([&] {
+    MFX_REQUIRE(ic, "null generator");
+    Ctx& c = ic->ctx->c;
+    if (ic->gexec) {
+      (void)hipGraphExecDestroy(ic->gexec);
+      ic->gexec = nullptr;
     }
-    grid_forward(wn, true);
-    // normalisation (341-346) and phi(1) (450-457)
-    if (wn->prec == MFX_FLOAT)
-      noise_normalise<float>(ic->ctx->c, (cx<float>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
-    else
-      noise_normalise<double>(ic->ctx->c, (cx<double>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
-    phi->space = MFX_KSPACE;
-    phi1_impl(phi, wn, ic->amp);
-    // phi(2), phi(3), A(3)
-    if (p.lpt_order > 1) {
-      if (p.use_fused_lpt)
-        lpt_fused(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
-      else
-        lpt_sequence(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
+    if (ic->graph) {
+      (void)hipGraphDestroy(ic->graph);
+      ic->graph = nullptr;
     }
-    // growth factors (550-563)
-    auto sc = [&](mfx_grid* g, double s) {
-      if (g->prec == MFX_FLOAT)
-        scale<float>(ic->ctx->c, (float*)g->d, (long long)g->ntot, s);
-      else
-        scale<double>(ic->ctx->c, (double*)g->d, (long long)g->ntot, s);
-    };
-    sc(phi, g1);
-    if (p.lpt_order > 1) sc(phi2, g2);
-    if (p.lpt_order > 2) {
-      sc(phi3, g3);
-      for (auto* a : A3) sc(a, g3c);
+    // one eager realisation creates every lazily built table (twiddles)
+    icgen_commit_seed(ic);
+    icgen_device_run(ic);
+    MFX_HIP(hipStreamSynchronize(c.stream));
+    const bool prof = c.profile;
+    c.profile = false;
+    MFX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+      icgen_device_run(ic);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(c.stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      c.profile = prof;
+      throw;
     }
-    // positions and velocities (821-931), lunit = vunit = 1
-    for (int kind = 0; kind < 2; ++kind)
-      for (int idim = 0; idim < 3; ++idim) {
-        tmp->space = MFX_KSPACE;
-        auto P = [](mfx_grid* g) { return g ? g->d : nullptr; };
-        if (tmp->prec == MFX_FLOAT)
-          assemble<float>(ic->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2),
-                          (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
-                          (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact, 1.0,
-                          ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
-        else
-          assemble<double>(ic->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2),
-                           (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]), (const cx<double>*)P(A3[1]),
-                           (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact,
-                           1.0, ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
-        grid_backward(tmp, true);
-        if (tmp->prec == MFX_FLOAT)
-          set_particles<float>(ic->ctx->c, (const float*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0, kind);
-        else
-          set_particles<double>(ic->ctx->c, (const double*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0,
-                                kind);
-      }
+    MFX_HIP(hipStreamEndCapture(c.stream, &ic->graph));
+    c.profile = prof;
+    MFX_HIP(hipGraphInstantiate(&ic->gexec, ic->graph, nullptr, nullptr, 0));
+  });
+}
+
+extern "C" int mfx_icgen_launch(mfx_icgen* ic) {
+  return guard([&] {
+    MFX_REQUIRE(ic && ic->gexec, "generator has no captured graph (mfx_icgen_capture)");
+    icgen_commit_seed(ic);
+    MFX_HIP(hipGraphLaunch(ic->gexec, ic->ctx->c.stream));
   });
 }
 

@@ -31,10 +31,13 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
                    int maxbins);
 
 // white noise
+// The fills read their seed from device memory and allocate nothing, so that
+// they can be captured into a graph and replayed with a new seed.
+size_t music1_scratch_bytes(int n, int cubesize);
 template <typename T>
-void music1_fill(Ctx& c, T* data, int n, long baseseed, int cubesize, int zeromean);
+void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int zeromean, void* scratch);
 template <typename T>
-void threefry_fill(Ctx& c, T* data, int n, unsigned long long seed);
+void threefry_fill(Ctx& c, T* data, int n, const unsigned long long* dseed);
 void threefry_uniforms(Ctx& c, double* out, int ncells, const unsigned long long* cells /*l,j1,j2,j3*/,
                        unsigned long long seed, int count);
 template <typename T>

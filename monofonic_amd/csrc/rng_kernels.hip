@@ -140,8 +140,9 @@ __device__ int block_scan(int v, int* sh, int* total) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, long baseseed, dd* part) {
+__global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, const long* seedp, dd* part) {
   __shared__ unsigned mt[MT_N];
+  const long baseseed = *seedp;
   __shared__ int scan[256];
   __shared__ dd red[256];
   const int tid = threadIdx.x;
@@ -226,22 +227,37 @@ __global__ void k_sub_mean(T* data, int n, const double* mean, int zeromean) {
   if (zeromean) data[p] = T((double)data[p] - mean[0]);
 }
 
-template <typename T>
-void music1_fill(Ctx& c, T* data, int n, long baseseed, int cubesize, int zeromean) {
-  int nc = (int)((double)n / cubesize);
+// cubes per axis and cube size of the MUSIC1 layout of an n-grid
+static void music1_cubes(int n, int cubesize, int& nc, int& cs) {
+  nc = (int)((double)n / cubesize);
   if (nc < 1) nc = 1;
-  int cs = cubesize;
+  cs = cubesize;
   if (n < cubesize) {
     nc = 1;
     cs = n;
   }
   MFX_REQUIRE((long long)nc * cs == n, "MUSIC1 fill: grid size must be a multiple of the cube size");
+}
+
+size_t music1_scratch_bytes(int n, int cubesize) {
+  int nc, cs;
+  music1_cubes(n, cubesize, nc, cs);
+  return sizeof(dd) * (size_t)nc * nc * nc + 256;
+}
+
+// Stream-ordered and allocation-free: the seed is read from device memory and
+// the per-cube partial sums go to `scratch` (music1_scratch_bytes), so the
+// fill can be captured into a graph and replayed with a new seed.
+template <typename T>
+void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int zeromean, void* scratch) {
+  int nc, cs;
+  music1_cubes(n, cubesize, nc, cs);
   const long long ncube = (long long)nc * nc * nc;
-  dd* part = (dd*)c.alloc(sizeof(dd) * ncube);
-  double* mean = (double*)c.alloc(sizeof(double));
+  double* mean = (double*)scratch;
+  dd* part = (dd*)((char*)scratch + 256);
   {
     LaunchScope ls(c, "music1_cubes", (double)n * n * n * sizeof(T));
-    hipLaunchKernelGGL(k_music1_cubes<T>, dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs, baseseed,
+    hipLaunchKernelGGL(k_music1_cubes<T>, dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs, dseed,
                        part);
     check_launch("music1_cubes");
   }
@@ -255,9 +271,6 @@ void music1_fill(Ctx& c, T* data, int n, long baseseed, int cubesize, int zerome
                        zeromean);
     check_launch("music1_submean");
   }
-  MFX_HIP(hipStreamSynchronize(c.stream));
-  c.release(part);
-  c.release(mean);
 }
 
 // raw MT19937 words for stream tests: nseeds streams of `count` words
@@ -363,7 +376,8 @@ __device__ void pan_uniforms(unsigned long long l, unsigned long long j1, unsign
 }
 
 template <typename T>
-__global__ void k_threefry_fill(T* data, int n, unsigned long long l, unsigned long long seed) {
+__global__ void k_threefry_fill(T* data, int n, unsigned long long l, const unsigned long long* seedp) {
+  const unsigned long long seed = *seedp;
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long pitch = n + 2;
   if (p >= (long long)n * n * pitch) return;
@@ -383,16 +397,16 @@ __global__ void k_threefry_fill(T* data, int n, unsigned long long l, unsigned l
   data[p] = T(radius * cos(angle));
 }
 
+// the seed (0 <= seed < 2^32, checked by the caller) is read from device memory
 template <typename T>
-void threefry_fill(Ctx& c, T* data, int n, unsigned long long seed) {
+void threefry_fill(Ctx& c, T* data, int n, const unsigned long long* dseed) {
   unsigned long long l = 0;
   while ((1ull << l) < (unsigned long long)n) ++l;
   MFX_REQUIRE((1ull << l) == (unsigned long long)n, "Threefry fill requires a power-of-two grid");
-  MFX_REQUIRE((seed >> 32) == 0, "Threefry seed must satisfy 0 <= seed < 2^32");
   const long long tot = (long long)n * n * (n + 2);
   LaunchScope ls(c, "threefry_fill", (double)tot * sizeof(T));
   hipLaunchKernelGGL(k_threefry_fill<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, data, n, l,
-                     seed);
+                     dseed);
   check_launch("threefry_fill");
 }
 
@@ -555,19 +569,28 @@ __global__ void k_ngenic(cx<T>* g, int n, const unsigned* table) {
 }
 
 template <typename T>
+__global__ void k_zero_cx(cx<T>* g, long long tot) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < tot) g[p] = mk(T(0), T(0));
+}
+
+// the grid is cleared by a kernel rather than a memset node, so that the fill
+// can be captured into a graph
+template <typename T>
 void ngenic_fill(Ctx& c, cx<T>* data, int n, const unsigned* table) {
   const long long tot = (long long)n * n * (n / 2 + 1);
-  MFX_HIP(hipMemsetAsync(data, 0, sizeof(cx<T>) * tot, c.stream));
+  hipLaunchKernelGGL(k_zero_cx<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, data, tot);
+  check_launch("ngenic_zero");
   LaunchScope ls(c, "ngenic_fill", (double)tot * sizeof(cx<T>));
   hipLaunchKernelGGL(k_ngenic<T>, dim3((unsigned)(((long long)n * n + 63) / 64)), dim3(64), 0, c.stream, data, n,
                      table);
   check_launch("ngenic_fill");
 }
 
-template void music1_fill<float>(Ctx&, float*, int, long, int, int);
-template void music1_fill<double>(Ctx&, double*, int, long, int, int);
-template void threefry_fill<float>(Ctx&, float*, int, unsigned long long);
-template void threefry_fill<double>(Ctx&, double*, int, unsigned long long);
+template void music1_fill<float>(Ctx&, float*, int, const long*, int, int, void*);
+template void music1_fill<double>(Ctx&, double*, int, const long*, int, int, void*);
+template void threefry_fill<float>(Ctx&, float*, int, const unsigned long long*);
+template void threefry_fill<double>(Ctx&, double*, int, const unsigned long long*);
 template void ngenic_fill<float>(Ctx&, cx<float>*, int, const unsigned*);
 template void ngenic_fill<double>(Ctx&, cx<double>*, int, const unsigned*);
 
