@@ -70,6 +70,18 @@ int mfx_device_free(mfx_ctx *c, void *ptr);
 int mfx_memcpy_d2h(mfx_ctx *c, void *host, const void *dev, size_t bytes);
 int mfx_memcpy_h2d(mfx_ctx *c, void *dev, const void *host, size_t bytes);
 
+/* ------------------------------------------------------------ autotuning
+ * No counterpart in the reference (FFTW plans are made with FFTW_ESTIMATE,
+ * src/grid_fft.cc:53-54).  Times the tile-width candidates of the column
+ * passes of lengths n and 3n/2 in `precision` and keeps the faster per
+ * length and direction in the context; results never depend on the choice. */
+int mfx_autotune(mfx_ctx *c, size_t n, int precision);
+/* set the tile width of column passes of length L explicitly (the widest
+ * feasible tile or half of it) */
+int mfx_set_column_tile(mfx_ctx *c, int L, int precision, int inverse, int tile);
+/* text table of the decisions taken so far */
+int mfx_autotune_report(mfx_ctx *c, char *buf, size_t len);
+
 /* ------------------------------------------------------------ Grid_FFT
  * Grid_FFT<real_t,false> ctor/allocate/reset (include/grid_fft.hh:77-101,
  * src/grid_fft.cc:43-112).  precision MFX_FLOAT or MFX_DOUBLE. */
@@ -243,6 +255,7 @@ typedef struct {
   int use_fused_lpt;    /* 1: mfx_lpt_sources, 0: the reference's convolve call sequence */
   int use_plt;          /* 1: PLT-corrected assembly (ENABLE_PLT builds, ic_generator.cc:366-370) */
   int plt_ngrid;        /* PLT mesh; 0 selects the reference's 64 */
+  int autotune;         /* 1: mfx_autotune for this grid size at creation (once per context) */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;

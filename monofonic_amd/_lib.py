@@ -19,6 +19,7 @@ EXPORTS = [
     "mfx_last_error", "mfx_version", "mfx_ctx_create", "mfx_ctx_destroy", "mfx_ctx_synchronize",
     "mfx_event_record", "mfx_event_elapsed", "mfx_profile_enable", "mfx_profile_reset", "mfx_profile_count",
     "mfx_profile_get", "mfx_device_alloc", "mfx_device_free", "mfx_memcpy_d2h", "mfx_memcpy_h2d",
+    "mfx_autotune", "mfx_autotune_report", "mfx_set_column_tile",
     "mfx_grid_create", "mfx_grid_destroy", "mfx_grid_info", "mfx_grid_data", "mfx_grid_upload",
     "mfx_grid_download", "mfx_grid_zero", "mfx_grid_copy_from", "mfx_grid_fft_forward", "mfx_grid_fft_backward",
     "mfx_grid_scale", "mfx_grid_inverse_laplacian", "mfx_grid_zero_dc", "mfx_grid_stats",
@@ -52,7 +53,7 @@ class IcgenParams(ctypes.Structure):
                 ("H0", ctypes.c_double), ("Omega_m", ctypes.c_double), ("Omega_b", ctypes.c_double),
                 ("n_s", ctypes.c_double), ("sigma_8", ctypes.c_double), ("Tcmb", ctypes.c_double),
                 ("zero_radiation", ctypes.c_int), ("use_fused_lpt", ctypes.c_int), ("use_plt", ctypes.c_int),
-                ("plt_ngrid", ctypes.c_int)]
+                ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 
 def load():
@@ -79,6 +80,8 @@ def load():
                             ctypes.POINTER(dbl)],
         "mfx_device_alloc": [vp, sz, pp], "mfx_device_free": [vp, vp],
         "mfx_memcpy_d2h": [vp, vp, vp, sz], "mfx_memcpy_h2d": [vp, vp, vp, sz],
+        "mfx_autotune": [vp, sz, i32], "mfx_autotune_report": [vp, ctypes.c_char_p, sz],
+        "mfx_set_column_tile": [vp, i32, i32, i32, i32],
         "mfx_grid_create": [vp, sz, sz, sz, dbl, dbl, dbl, i32, i32, pp],
         "mfx_grid_destroy": [vp],
         "mfx_grid_info": [vp, ctypes.POINTER(sz), ctypes.POINTER(i32), ctypes.POINTER(i32)],

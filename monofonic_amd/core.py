@@ -74,6 +74,18 @@ class Context:
     def free(self, p):
         call("mfx_device_free", self._h, p)
 
+    def autotune(self, n, precision="float"):
+        """time the column-pass tile widths for grid size n (lengths n and 3n/2)"""
+        call("mfx_autotune", self._h, int(n), _prec(precision))
+
+    def set_column_tile(self, L, precision, inverse, tile):
+        call("mfx_set_column_tile", self._h, int(L), _prec(precision), int(inverse), int(tile))
+
+    def autotune_report(self):
+        buf = ctypes.create_string_buffer(8192)
+        call("mfx_autotune_report", self._h, buf, 8192)
+        return buf.value.decode()
+
     def d2h(self, dev, nbytes, dtype):
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype=dtype)
         call("mfx_memcpy_d2h", self._h, out.ctypes.data_as(ctypes.c_void_p), dev, nbytes)
@@ -486,7 +498,8 @@ class ICGenerator:
 
     def __init__(self, ctx, GridRes, BoxLength, zstart, LPTorder=3, precision="double", generator="MUSIC1",
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
-                 sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64):
+                 sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
+                 autotune=True):
         self.ctx = ctx
         self.n = int(GridRes)
         p = _lib.IcgenParams()
@@ -496,6 +509,7 @@ class ICGenerator:
         p.H0, p.Omega_m, p.Omega_b, p.n_s, p.sigma_8, p.Tcmb = H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb
         p.zero_radiation, p.use_fused_lpt = int(ZeroRadiation), int(fused)
         p.use_plt, p.plt_ngrid = int(PLT), int(plt_ngrid)
+        p.autotune = int(autotune)
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 

@@ -1071,6 +1071,59 @@ int mfx_cosmo_amplitude_table(const mfx_cosmo* cc, size_t n, double boxlen, doub
 
 }  // extern "C"
 
+// ------------------------------------------------------------------ autotuning
+extern "C" {
+
+int mfx_autotune(mfx_ctx* c, size_t n, int precision) {
+  return guard([&] {
+    MFX_REQUIRE(c, "null context");
+    MFX_REQUIRE(precision == MFX_FLOAT || precision == MFX_DOUBLE, "invalid precision");
+    MFX_REQUIRE(n >= 2 && n % 2 == 0, "grid size must be even");
+    // the plain transform length n and the padded length 3n/2 of the
+    // Orszag/LPT passes, where supported
+    const int lens[2] = {(int)n, (int)(3 * n / 2)};
+    int tuned = 0;
+    for (int L : lens) {
+      if (!column_length_supported(L)) continue;
+      if (precision == MFX_FLOAT)
+        autotune_columns<float>(c->c, L);
+      else
+        autotune_columns<double>(c->c, L);
+      ++tuned;
+    }
+    MFX_REQUIRE(tuned > 0, "no supported column length for this grid size");
+  });
+}
+
+int mfx_set_column_tile(mfx_ctx* c, int L, int precision, int inverse, int tile) {
+  return guard([&] {
+    MFX_REQUIRE(c, "null context");
+    MFX_REQUIRE(column_length_supported(L), "unsupported column length");
+    MFX_REQUIRE(precision == MFX_FLOAT || precision == MFX_DOUBLE, "invalid precision");
+    const int cm = column_tile_max(L, precision);
+    MFX_REQUIRE(tile == cm || (cm >= 2 && tile == cm / 2),
+                "tile must be " + std::to_string(cm) + (cm >= 2 ? " or " + std::to_string(cm / 2) : std::string()));
+    c->c.coltile[std::make_tuple(L, precision, inverse ? 1 : 0)] = tile;
+  });
+}
+
+int mfx_autotune_report(mfx_ctx* c, char* buf, size_t len) {
+  return guard([&] {
+    MFX_REQUIRE(c && buf && len > 0, "invalid argument");
+    std::string out;
+    char line[192];
+    for (const auto& r : c->c.tune_log) {
+      std::snprintf(line, sizeof line, "L=%d %s %s: C=%d %.4f ms, C=%d %.4f ms -> C=%d\n", r.L,
+                    r.tsize == 4 ? "f32" : "f64", r.inv ? "inverse" : "forward", r.cand[0], r.ms[0], r.cand[1],
+                    r.ms[1], r.chosen);
+      out += line;
+    }
+    std::snprintf(buf, len, "%s", out.c_str());
+  });
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------------ PLT
 struct mfx_plt {
   mfx_ctx* ctx;
@@ -1303,6 +1356,16 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       for (int q = 0; q < 2; ++q) {
         MFX_HIP(hipHostMalloc(&ic->host_stage[q], 16 + table_bytes, hipHostMallocDefault));
         MFX_HIP(hipEventCreateWithFlags(&ic->staged[q], hipEventDisableTiming));
+      }
+      // column tile widths for the grid's transform lengths, once per context
+      if (p->autotune) {
+        bool done = false;
+        for (const auto& r : c->c.tune_log)
+          if (r.L == (int)(3 * p->n / 2) && r.tsize == p->precision) done = true;
+        if (!done) {
+          int st = mfx_autotune(c, p->n, p->precision);
+          if (st) throw Error(st, g_err);
+        }
       }
       // PLT tables are computed once, outside the timed realisations
       if (p->use_plt) {

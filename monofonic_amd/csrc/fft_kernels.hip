@@ -110,6 +110,90 @@ void conv2(Ctx& c, const cx<T>* fa, const MultSpec& ma, const cx<T>* fb, const M
                        bT1 + (op == OP_ASSIGN ? bN : 2 * bN));
 }
 
+// ------------------------------------------------------------------ autotuning
+// Column-pass tile width: the widest tile whose exchange area fits the LDS
+// budget keeps 8-16 adjacent columns per line access (wider HBM segments),
+// half of it doubles the workgroups per CU.  Which is faster depends on the
+// length and precision, so both are timed on a block of the library's own
+// shape -- L/2+1 adjacent columns, up to 256 batches -- in both directions,
+// five launches each after two warm-ups, and the faster is kept.
+template <typename T>
+void autotune_columns(Ctx& c, int L) {
+  const int cm = col_tile_max<T>(L);
+  const int ncols = L / 2 + 1;
+  const int nb = L < 256 ? L : 256;
+  const long long cnt = (long long)nb * L * ncols;
+  cx<T>* buf = reinterpret_cast<cx<T>*>(c.alloc(sizeof(cx<T>) * cnt));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool prof = c.profile;
+  c.profile = false;
+  try {
+    MFX_HIP(hipMemsetAsync(buf, 0, sizeof(cx<T>) * cnt, c.stream));
+    MFX_HIP(hipEventCreate(&e0));
+    MFX_HIP(hipEventCreate(&e1));
+    const long long sb = (long long)L * ncols, sp = ncols;
+    for (int inv = 0; inv < 2; ++inv) {
+      TuneRec rec{L, (int)sizeof(T), inv, {cm, cm / 2}, {0.0, 0.0}, cm};
+      const int ncand = cm >= 2 ? 2 : 1;
+      double best = 1e300;
+      for (int q = 0; q < ncand; ++q) {
+        c.coltile[std::make_tuple(L, (int)sizeof(T), inv)] = rec.cand[q];
+        auto pass = [&] {
+          if (inv)
+            launch_col<T, true>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune", 0);
+          else
+            launch_col<T, false>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune", 0);
+        };
+        pass();
+        pass();
+        MFX_HIP(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 5; ++r) pass();
+        MFX_HIP(hipEventRecord(e1, c.stream));
+        MFX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        rec.ms[q] = ms / 5.0;
+        if (rec.ms[q] < best) {
+          best = rec.ms[q];
+          rec.chosen = rec.cand[q];
+        }
+      }
+      c.coltile[std::make_tuple(L, (int)sizeof(T), inv)] = rec.chosen;
+      c.tune_log.push_back(rec);
+    }
+  } catch (...) {
+    c.profile = prof;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c.stream);
+    c.release(buf);
+    throw;
+  }
+  c.profile = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c.release(buf);
+}
+
+template void autotune_columns<float>(Ctx&, int);
+template void autotune_columns<double>(Ctx&, int);
+
+int column_tile_max(int L, int tsize) {
+  return tsize == 4 ? col_tile_max<float>(L) : col_tile_max<double>(L);
+}
+
+int column_length_supported(int L) {
+  switch (L) {
+#define MFX_CASE(LL) \
+  case LL:           \
+    return 1;
+    MFX_COL_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      return 0;
+  }
+}
+
 template void conv2<float>(Ctx&, const cx<float>*, const MultSpec&, const cx<float>*, const MultSpec&, cx<float>*,
                            int, double, int, double, void*);
 template void conv2<double>(Ctx&, const cx<double>*, const MultSpec&, const cx<double>*, const MultSpec&,

@@ -217,10 +217,10 @@ __device__ __forceinline__ void xcd_tile(int& tile, int& b) {
   b = (int)(w / ntiles);
 }
 
-template <typename T, int L, bool INV, class Src, class Dst>
-__global__ void __launch_bounds__((ColCfg<T, L>::C * ColCfg<T, L>::TPS))
+// C adjacent columns per workgroup (the tile width chosen by the autotuner)
+template <typename T, int L, bool INV, class Src, class Dst, int C>
+__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
     k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
-  constexpr int C = ColCfg<T, L>::C;
   constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
   __shared__ cx<T> lds[L * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -243,19 +243,46 @@ __global__ void __launch_bounds__((ColCfg<T, L>::C * ColCfg<T, L>::TPS))
   }
 }
 
-template <typename T, int L, bool INV, class Src, class Dst>
-void launch_col_L(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
-  constexpr int C = ColCfg<T, L>::C;
-  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
-  LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst>), grid, block, 0, c.stream, s, d, c.twiddles<T>(L), ncols);
-  check_launch(name);
-}
-
 #define MFX_COL_LENGTHS(X) \
   X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(768) X(1024) X(1536)
 #define MFX_ROW_LENGTHS(X) \
   X(8) X(12) X(16) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(768)
+
+template <typename T, int L, int C, bool INV, class Src, class Dst>
+void launch_col_LC(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst, C>), grid, block, 0, c.stream, s, d, c.twiddles<T>(L), ncols);
+  check_launch(name);
+}
+
+// Tile width candidates: the widest tile whose exchange area fits 64 KiB of
+// LDS (ColCfg) and half of it; the context's tuning table picks one.
+template <typename T, int L, bool INV, class Src, class Dst>
+void launch_col_L(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  constexpr int CM = ColCfg<T, L>::C;
+  if constexpr (CM >= 2) {
+    if (c.col_tile(L, (int)sizeof(T), INV, CM) == CM / 2) {
+      launch_col_LC<T, L, CM / 2, INV>(c, s, d, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
+  launch_col_LC<T, L, CM, INV>(c, s, d, ncols, nbatch, name, bytes);
+}
+
+// widest tile for length L, precision T
+template <typename T>
+int col_tile_max(int L) {
+  switch (L) {
+#define MFX_CASE(LL) \
+  case LL:           \
+    return ColCfg<T, LL>::C;
+    MFX_COL_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      throw Error(1, "unsupported transform length " + std::to_string(L));
+  }
+}
 
 template <typename T, bool INV, class Src, class Dst>
 void launch_col(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
@@ -268,6 +295,26 @@ void launch_col(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const char* 
 #undef MFX_CASE
     default:
       throw Error(1, "unsupported transform length " + std::to_string(L));
+  }
+}
+
+// The padded lengths M = 3n/2 of the Orszag and LPT passes (n = 16 ... 1024)
+// and the corresponding z-row lengths K = M/2: the fused engine instantiates
+// its passes for these only.
+#define MFX_PAD_LENGTHS(X) X(24) X(48) X(72) X(96) X(144) X(192) X(288) X(384) X(768) X(1536)
+#define MFX_PAD_K_LENGTHS(X) X(12) X(24) X(36) X(48) X(72) X(96) X(144) X(192) X(384) X(768)
+
+template <typename T, bool INV, class Src, class Dst>
+void launch_col_pad(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  switch (L) {
+#define MFX_CASE(LL)                                                    \
+  case LL:                                                              \
+    launch_col_L<T, LL, INV, Src, Dst>(c, s, d, ncols, nbatch, name, bytes); \
+    return;
+    MFX_PAD_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      throw Error(1, "unsupported padded transform length " + std::to_string(L));
   }
 }
 

@@ -28,85 +28,6 @@
 
 namespace mfx {
 
-// ------------------------------------------------------------------ x/y stages
-template <typename T>
-struct OutSet {
-  cx<T>* p[3];
-  int pw[3];
-  int nout;
-};
-
-// one padded source line, up to three outputs IDFT[pad(g^pw * line)]
-template <typename T, int L>
-__global__ void __launch_bounds__((ColCfg<T, L>::C * ColCfg<T, L>::TPS))
-    k_col_pad_multi(const cx<T>* src, long long ssb, long long ssp, int n, T scale, T kfac, OutSet<T> outs,
-                    long long dsb, long long dsp, int ncols, const cx<T>* __restrict__ tw) {
-  constexpr int C = ColCfg<T, L>::C;
-  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
-  __shared__ cx<T> lds[L * C];
-  const int c = threadIdx.x % C, t = threadIdx.x / C;
-  const int col = blockIdx.x * C + c;
-  const int b = blockIdx.y;
-  const bool act = col < ncols;
-  const int colc = act ? col : ncols - 1;
-  const int nh = n >> 1;
-  cx<T> v0[EPT];
-  T g[EPT];
-#pragma unroll
-  for (int e = 0; e < EPT; ++e) {
-    const int pos = t + TPS * e;
-    const bool pad = (pos > nh) & (pos <= n);
-    const int i = pad ? 0 : (pos <= nh ? pos : pos - nh);
-    // unconditional load (clamped index), padding applied by select
-    const cx<T> x = src[(long long)b * ssb + (long long)i * ssp + colc];
-    g[e] = pad ? T(0) : kval<T>(i, n, kfac, 1);
-    v0[e] = pad ? mk(T(0), T(0)) : x * scale;
-  }
-  for (int o = 0; o < outs.nout; ++o) {
-    cx<T> v[EPT];
-    const int pw = outs.pw[o];
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      T f = pw == 0 ? T(1) : (pw == 1 ? g[e] : g[e] * g[e]);
-      v[e] = v0[e] * f;
-    }
-    fft_exec<T, L, true>(v, lds + c, C, t, tw);
-    if (act) {
-      cx<T>* d = outs.p[o];
-#pragma unroll
-      for (int e = 0; e < EPT; ++e) d[(long long)b * dsb + (long long)(t + TPS * e) * dsp + col] = v[e];
-    }
-  }
-}
-
-template <typename T, int L>
-void launch_pad_multi_L(Ctx& c, const cx<T>* src, long long ssb, long long ssp, int n, T scale, T kfac,
-                        const OutSet<T>& outs, long long dsb, long long dsp, int ncols, int nbatch, const char* name,
-                        double bytes) {
-  constexpr int C = ColCfg<T, L>::C;
-  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
-  LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_col_pad_multi<T, L>), grid, block, 0, c.stream, src, ssb, ssp, n, scale, kfac, outs, dsb,
-                     dsp, ncols, c.twiddles<T>(L));
-  check_launch(name);
-}
-
-template <typename T>
-void launch_pad_multi(Ctx& c, int L, const cx<T>* src, long long ssb, long long ssp, int n, T scale, T kfac,
-                      const OutSet<T>& outs, long long dsb, long long dsp, int ncols, int nbatch, const char* name,
-                      double bytes) {
-  switch (L) {
-#define MFX_CASE(LL)                                                                                          \
-  case LL:                                                                                                    \
-    launch_pad_multi_L<T, LL>(c, src, ssb, ssp, n, scale, kfac, outs, dsb, dsp, ncols, nbatch, name, bytes); \
-    return;
-    MFX_COL_LENGTHS(MFX_CASE)
-#undef MFX_CASE
-    default:
-      throw Error(1, "unsupported padded length " + std::to_string(L));
-  }
-}
-
 // ------------------------------------------------------------------ z stage
 // Y arrays of one field: 0 -> Y0, 1 -> Yx, 2 -> Yy, 3 -> Yxx, 4 -> Yxy, 5 -> Yyy
 template <typename T>
@@ -397,7 +318,7 @@ void launch_zC2_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, int i, cx<T>
 
 #define MFX_LPT_KSWITCH(K, CALL)                                          \
   switch (K) {                                                            \
-    MFX_ROW_LENGTHS(CALL)                                                 \
+    MFX_PAD_K_LENGTHS(CALL)                                               \
     default:                                                              \
       throw Error(1, "unsupported padded length " + std::to_string(2 * (K))); \
   }
@@ -471,12 +392,12 @@ struct Engine {
     // cheaper than holding three transforms in registers)
     const int nx = hessian ? 3 : 1;
     for (int p = 0; p < nx; ++p)
-      launch_col<T, true>(c, m, SrcPadPow<T>{F, HN, (long long)n * HN, n, p, s_in, kfac},
+      launch_col_pad<T, true>(c, m, SrcPadPow<T>{F, HN, (long long)n * HN, n, p, s_in, kfac},
                           DstPlain<T>{T1[p], hp, (long long)n * hp, T(1)}, l.h, n, "lpt_x_inv", bN + bT1);
     // y stage: line along j of T1 [x][j][kz], batch x, output [x][y][kz]
     auto ystage = [&](int src, std::initializer_list<std::pair<int, int>> outs) {
       for (auto& o : outs)
-        launch_col<T, true>(c, m, SrcPadPow<T>{T1[src], (long long)n * hp, hp, n, o.second, T(1), kfac},
+        launch_col_pad<T, true>(c, m, SrcPadPow<T>{T1[src], (long long)n * hp, hp, n, o.second, T(1), kfac},
                             DstPlain<T>{Y[o.first], (long long)m * hp, hp, T(1)}, l.h, m, "lpt_y_inv", bT1 + bT2);
     };
     if (hessian) {
@@ -492,10 +413,10 @@ struct Engine {
   void forward(cx<T>* Z, cx<T>* out, bool invlap) {
     const int n = l.n, m = l.m, hp = l.hp;
     const long long HN = l.h;
-    launch_col<T, false>(c, m, SrcPlain<T>{Z, (long long)m * hp, hp}, DstTrunc<T>{T1[0], (long long)n * hp, hp, n},
+    launch_col_pad<T, false>(c, m, SrcPlain<T>{Z, (long long)m * hp, hp}, DstTrunc<T>{T1[0], (long long)n * hp, hp, n},
                          l.h, m, "lpt_y_fwd", bT2 + bT1);
     DstTruncOp<T> d{out, HN, (long long)n * HN, n, OP_ASSIGN, T(0), s_out, invlap ? 1 : 0, kfac};
-    launch_col<T, false>(c, m, SrcPlain<T>{T1[0], hp, (long long)n * hp}, d, l.h, n, "lpt_x_fwd", bT1 + bN);
+    launch_col_pad<T, false>(c, m, SrcPlain<T>{T1[0], hp, (long long)n * hp}, d, l.h, n, "lpt_x_fwd", bT1 + bN);
   }
 
   FieldY<T> fy(int base) const {

@@ -6,6 +6,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -30,6 +31,15 @@ struct Error : std::runtime_error {
     if (!(cond)) throw ::mfx::Error(1, (msg));         \
   } while (0)
 
+// one autotuning decision: candidate tile widths of a column pass and their
+// measured times
+struct TuneRec {
+  int L, tsize, inv;
+  int cand[2];
+  double ms[2];
+  int chosen;
+};
+
 // one profiled kernel launch
 struct ProfRec {
   std::string name;
@@ -50,6 +60,13 @@ struct Ctx {
   bool profile = false;
   std::vector<ProfRec> pending;
   std::map<std::string, KStat> stats;
+  // column-pass tile width per (length, sizeof(T), inverse): autotuned
+  std::map<std::tuple<int, int, int>, int> coltile;
+  std::vector<TuneRec> tune_log;
+  int col_tile(int L, int tsize, bool inv, int cmax) const {
+    auto it = coltile.find(std::make_tuple(L, tsize, (int)inv));
+    return it == coltile.end() ? cmax : it->second;
+  }
   ~Ctx();
   template <typename T>
   const cx<T>* twiddles(int L);
@@ -107,5 +124,14 @@ void conv2(Ctx& c, const cx<T>* fa, const MultSpec& ma, const cx<T>* fb, const M
            cx<T>* out, int op, double opfac, int n, double boxlen, void* ws);
 size_t conv_workspace_bytes(int n, int prec);
 int conv_pitch(int n);
+
+// Times the column-pass tile candidates of line length L (both directions)
+// and stores the faster in c.coltile.  Throws for unsupported lengths.
+template <typename T>
+void autotune_columns(Ctx& c, int L);
+// 1 if column passes of line length L are built
+int column_length_supported(int L);
+// widest tile of length L for element size tsize (4: FP32, 8: FP64)
+int column_tile_max(int L, int tsize);
 
 }  // namespace mfx

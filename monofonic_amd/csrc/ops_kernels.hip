@@ -90,9 +90,9 @@ __global__ void k_fill(T* f, long long cnt, T v) {
 // ic_generator.cc:821-860 (kind 0, displacement) and 877-919 (kind 1,
 // velocity), followed by zero_DC_mode.  Potentials are already growth-scaled.
 // The gradient is op::fourier_gradient (operators.hh:38-61, vfac_corr = 1),
-// or with PLT tables (plt.tab != nullptr) particle::lattice_gradient, whose
-// growth-rate factor then multiplies the velocities (ENABLE_PLT builds).
-template <typename T>
+// or with PLT (tables in plt) particle::lattice_gradient, whose growth-rate
+// factor then multiplies the velocities (ENABLE_PLT builds).
+template <typename T, bool PLT>
 __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
                            const cx<T>* a3y, const cx<T>* a3z, int n, double k0, int order, int kind, int idim,
                            double vfac, double unit_over_L, PltView plt, double boxlen) {
@@ -104,7 +104,7 @@ __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, cons
   int j = (int)(ij % n), i = (int)(ij / n);
   const int ijk[3] = {i, j, k};
   double g[3], vc = 1.0;
-  if (plt.tab) {
+  if constexpr (PLT) {
     plt_gradient(plt, n, boxlen, i, j, k, g, vc);
   } else {
 #pragma unroll
@@ -148,7 +148,7 @@ __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, cons
     tr += cr;
     ti += ci;
   }
-  if (plt.tab && kind == 1) {
+  if (PLT && kind == 1) {
     tr *= vc;
     ti *= vc;
   }
@@ -292,9 +292,14 @@ void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<
   long long cnt = (long long)n * n * (n / 2 + 1);
   int nin = order == 1 ? 1 : order == 2 ? 2 : 5;
   LaunchScope ls(c, plt_tab ? "assemble_kspace_plt" : "assemble_kspace", (nin + 1.0) * cnt * sizeof(cx<T>));
-  hipLaunchKernelGGL(k_assemble<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, tmp, phi, phi2, phi3, a3x, a3y, a3z,
-                     n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen, PltView{plt_tab, plt_ng},
-                     boxlen);
+  if (plt_tab)
+    hipLaunchKernelGGL((k_assemble<T, true>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, tmp, phi, phi2, phi3, a3x,
+                       a3y, a3z, n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen,
+                       PltView{plt_tab, plt_ng}, boxlen);
+  else
+    hipLaunchKernelGGL((k_assemble<T, false>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, tmp, phi, phi2, phi3, a3x,
+                       a3y, a3z, n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen,
+                       PltView{nullptr, 0}, boxlen);
   check_launch("assemble");
 }
 
