@@ -365,7 +365,8 @@ __global__ void __launch_bounds__(64)
               const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
   constexpr int E = RowCtx<T, K>::E;
-  const cx<T>* buf = rc.c2r(src + rowc * spitch, hin);
+  cx<T> v[E];
+  rc.c2r(v, src + rowc * spitch, hin);
   // K packed pairs, and the two padding reals of an in-place row set to zero;
   // an inactive row stores nothing (zero-length resource)
   const unsigned nst = act ? (unsigned)(dpitch >= 2 * K + 2 ? K + 1 : K) : 0u;
@@ -373,7 +374,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
   for (int e = 0; e <= E; ++e) {
     const int n = lane + P_ * e;
-    if (n <= K) buf_store<T>(n < K ? buf[n] * scale : mk(T(0), T(0)), r, (unsigned)n * sizeof(cx<T>));
+    if (n <= K) buf_store<T>(n < K ? v[e < E ? e : E - 1] * scale : mk(T(0), T(0)), r, (unsigned)n * sizeof(cx<T>));
   }
 }
 
@@ -385,8 +386,8 @@ __global__ void __launch_bounds__(64)
   MFX_ROW_PROLOGUE(T, K)
   constexpr int E = RowCtx<T, K>::E;
   cx<T> va[E], vb[E];
-  rc.get(va, rc.c2r(a + rowc * spitch, hin));
-  rc.get(vb, rc.c2r(b + rowc * spitch, hin));
+  rc.c2r(va, a + rowc * spitch, hin);
+  rc.c2r(vb, b + rowc * spitch, hin);
   // packed real product: z[n] = x[2n] + i x[2n+1]
 #pragma unroll
   for (int e = 0; e < E; ++e) va[e] = mk(va[e].x * vb[e].x, va[e].y * vb[e].y);

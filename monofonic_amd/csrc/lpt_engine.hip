@@ -89,16 +89,16 @@ struct ZRow {
   template <int A, int B>
   __device__ __forceinline__ void hess(cx<T> (&v)[E], const Raw& x) const {
     const KzMul<T> m = kz(hess_pz(A, B));
-    rc.get(v, rc.c2r_from([&](int e, int k) { return x[e] * m(k); }, hin));
+    rc.c2r_to(v, [&](int e, int k) { return x[e] * m(k); });
   }
   // Hessian components (A,B) - (C,D) of one field, combined before the c2r
   template <int A, int B, int C, int D>
   __device__ __forceinline__ void hess_diff(cx<T> (&v)[E], const Raw& x, const Raw& y) const {
     const KzMul<T> m1 = kz(hess_pz(A, B)), m2 = kz(hess_pz(C, D));
-    rc.get(v, rc.c2r_from([&](int e, int k) { return x[e] * m1(k) - y[e] * m2(k); }, hin));
+    rc.c2r_to(v, [&](int e, int k) { return x[e] * m1(k) - y[e] * m2(k); });
   }
   __device__ __forceinline__ void plain(cx<T> (&v)[E], const Raw& x) const {
-    rc.get(v, rc.c2r_from([&](int e, int) { return x[e]; }, hin));
+    rc.c2r_to(v, [&](int e, int) { return x[e]; });
   }
   __device__ __forceinline__ void emit(const cx<T> (&v)[E], cx<T>* p) const {
     rc.r2c_store(v, p + off, hin, act, T(1));

@@ -6,11 +6,13 @@
 // lane takes butterflies j = lane, lane + P, ... of the K/R butterflies,
 // reads its R inputs x[j + q K/R] from one buffer and writes the Stockham
 // outputs to the other.  Between stages a barrier of the one-wave workgroup
-// orders the LDS traffic.  Per-lane register state is one butterfly (R values
-// and R-1 twiddles) whatever the row length, which keeps the fused z-stage
-// kernels that combine many rows at high occupancy; high radices (8, 16) keep
-// the number of LDS round trips at 3-4 per transform; radix 16 is available
-// but its 16-value working set costs more registers than it saves.
+// orders the LDS traffic.  Rows enter and leave a transform in registers, in
+// natural order (lane element e is index lane + P*e); where the radix plan
+// allows, the first stage runs on those registers, and the last stage of an
+// r2c leaves its outputs in them (row_fft_regs), which saves one to two LDS
+// round trips per transform.  Per-lane register state is otherwise one
+// butterfly, which keeps the fused z-stage kernels that combine many rows at
+// high occupancy.
 #pragma once
 #include "fft_core.hpp"
 
@@ -61,6 +63,10 @@ struct RowPlan;
     static constexpr int R[] = {__VA_ARGS__};           \
     static constexpr int NR = sizeof(R) / sizeof(int);  \
   };
+// First and last radix chosen, where the length allows, to divide the
+// elements per lane E = K/P: the first stage then runs in registers on the
+// lane's own elements and the last stage leaves the lane's outputs in its
+// registers in natural order (see row_fft_regs).
 MFX_RPLAN(8, 8)
 MFX_RPLAN(12, 3, 4)
 MFX_RPLAN(16, 4, 4)
@@ -74,11 +80,11 @@ MFX_RPLAN(96, 3, 4, 8)
 MFX_RPLAN(128, 2, 8, 8)
 MFX_RPLAN(144, 3, 3, 4, 4)
 MFX_RPLAN(192, 3, 8, 8)
-MFX_RPLAN(256, 4, 8, 8)
+MFX_RPLAN(256, 4, 16, 4)
 MFX_RPLAN(288, 3, 3, 4, 8)
-MFX_RPLAN(384, 3, 8, 4, 4)
+MFX_RPLAN(384, 3, 8, 8, 2)
 MFX_RPLAN(512, 8, 8, 8)
-MFX_RPLAN(768, 3, 4, 8, 8)
+MFX_RPLAN(768, 3, 8, 8, 4)
 #undef MFX_RPLAN
 
 // lanes per row
@@ -88,43 +94,130 @@ struct RowLanes {
   static constexpr int RPW = 64 / P;        // rows per wavefront (= per workgroup)
   static constexpr int E = (K + P - 1) / P;  // elements per lane for row-wise access
   static constexpr int LB = K + 2;           // one LDS buffer per row (K+1 used)
+  static constexpr int NR = RowPlan<K>::NR;
+  // lane-owned stages (row_fft_regs): every lane holds E whole elements
+  static constexpr bool EXACT = K % P == 0;
+  static constexpr bool FIRST_REG = EXACT && NR >= 2 && E % RowPlan<K>::R[0] == 0;
+  static constexpr bool LAST_REG = EXACT && NR >= 2 && E % RowPlan<K>::R[NR - 1] == 0;
 };
 
 // one-wave workgroup barrier: orders the LDS traffic of the lanes
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
-template <typename T, int K, bool INV, int S, int NS>
-__device__ __forceinline__ cx<T>* row_stages(cx<T>* in, cx<T>* out, int lane, const cx<T>* __restrict__ tw) {
-  if constexpr (S < RowPlan<K>::NR) {
-    constexpr int R = RowPlan<K>::R[S];
-    constexpr int NBF = K / R;
-    constexpr int P = RowLanes<K>::P;
-    for (int j = lane; j < NBF; j += P) {
-      cx<T> a[R];
+// product of the radices of the stages before stage S
+template <int K, int S>
+struct RowNS {
+  static constexpr int value = S == 0 ? 1 : RowPlan<K>::R[S - 1] * RowNS<K, (S > 0 ? S - 1 : 0)>::value;
+};
+template <int K>
+struct RowNS<K, 0> {
+  static constexpr int value = 1;
+};
+
+// one Stockham stage S from LDS buffer `in` to LDS buffer `out`
+template <typename T, int K, bool INV, int S>
+__device__ __forceinline__ void row_stage(const cx<T>* in, cx<T>* out, int lane, const cx<T>* tw) {
+  constexpr int R = RowPlan<K>::R[S];
+  constexpr int NS = RowNS<K, S>::value;
+  constexpr int NBF = K / R;
+  constexpr int P = RowLanes<K>::P;
+  for (int j = lane; j < NBF; j += P) {
+    cx<T> a[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) a[q] = in[j + q * NBF];
-      const int k = j % NS;
-      if (NS > 1) {
+    for (int q = 0; q < R; ++q) a[q] = in[j + q * NBF];
+    const int k = j % NS;
+    if (NS > 1) {
 #pragma unroll
-        for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], tw[q * k * (K / (NS * R))]);
-      }
-      BF<R, INV, T>::run(a);
-      const int base = (j / NS) * NS * R + k;
-#pragma unroll
-      for (int q = 0; q < R; ++q) out[base + q * NS] = a[q];
+      for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], tw[q * k * (K / (NS * R))]);
     }
+    BF<R, INV, T>::run(a);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[base + q * NS] = a[q];
+  }
+}
+
+// stages S .. SEND-1, ping-pong between in and out with a barrier after
+// each; returns the buffer holding the result
+template <typename T, int K, bool INV, int S, int SEND>
+__device__ __forceinline__ cx<T>* row_stages(cx<T>* in, cx<T>* out, int lane, const cx<T>* tw) {
+  if constexpr (S < SEND) {
+    row_stage<T, K, INV, S>(in, out, lane, tw);
     wave_sync();
-    return row_stages<T, K, INV, S + 1, NS * R>(out, in, lane, tw);
+    return row_stages<T, K, INV, S + 1, SEND>(out, in, lane, tw);
   } else {
     return in;
   }
 }
 
-// transform the row in buffer A (B is scratch); returns the buffer holding
-// the result in natural order.  All lanes of the workgroup must call it.
-template <typename T, int K, bool INV>
-__device__ __forceinline__ cx<T>* row_fft(cx<T>* A, cx<T>* B, int lane, const cx<T>* __restrict__ tw) {
-  return row_stages<T, K, INV, 0, 1>(A, B, lane, tw);
+// Transform of a row held in registers in natural order -- lane element e is
+// index lane + P*e -- back into registers in natural order; A and B are the
+// row's LDS buffers, free on entry (the caller has synchronised).
+//   first stage in registers (FIRST_REG): radix R0 divides E, so butterfly
+//     j = lane + P*it has its inputs j + q*K/R0 = lane + P*(it + q*E/R0) in
+//     the lane's own registers; only its outputs go to LDS
+//   last stage to registers (LAST_REG): for the last radix RL dividing E, the
+//     outputs of butterfly j = lane + P*it land at j + q*K/RL =
+//     lane + P*(it + q*E/RL), the lane's own natural positions, and its
+//     twiddles are w^(q*j)
+// Lengths without these properties go through LDS on entry and exit.
+// LASTOK = false keeps the last stage in LDS followed by a barrier and a
+// natural-order read: the c2r of the fused z stages, whose outputs are
+// consumed by products with other rows, then compiles to fewer registers
+// (the barrier bounds how far the scheduler moves the last stage's work).
+// Returns the buffer that no lane reads after the last barrier: the caller
+// may write it without a further barrier.
+template <typename T, int K, bool INV, bool LASTOK = true>
+__device__ __forceinline__ cx<T>* row_fft_regs(cx<T> (&v)[RowLanes<K>::E], cx<T>* A, cx<T>* B, int lane,
+                                             const cx<T>* tw) {
+  using RL_ = RowLanes<K>;
+  constexpr int P = RL_::P, E = RL_::E, NR = RL_::NR;
+  constexpr bool LASTR = RL_::LAST_REG && LASTOK;
+  constexpr int S0 = RL_::FIRST_REG ? 1 : 0;
+  constexpr int SEND = LASTR ? NR - 1 : NR;
+  if constexpr (RL_::FIRST_REG) {
+    constexpr int R0 = RowPlan<K>::R[0], m = E / R0;
+#pragma unroll
+    for (int it = 0; it < m; ++it) {
+      cx<T> a[R0];
+#pragma unroll
+      for (int q = 0; q < R0; ++q) a[q] = v[it + q * m];
+      BF<R0, INV, T>::run(a);
+      const int j = lane + P * it;
+#pragma unroll
+      for (int q = 0; q < R0; ++q) A[j * R0 + q] = a[q];
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = lane + P * e;
+      if (k < K) A[k] = v[e];
+    }
+  }
+  wave_sync();
+  cx<T>* cur = row_stages<T, K, INV, S0, SEND>(A, B, lane, tw);
+  if constexpr (LASTR) {
+    constexpr int RL = RowPlan<K>::R[NR - 1], NSL = K / RL, w = E / RL;
+#pragma unroll
+    for (int it = 0; it < w; ++it) {
+      const int j = lane + P * it;
+      cx<T> a[RL];
+#pragma unroll
+      for (int q = 0; q < RL; ++q) a[q] = cur[j + q * NSL];
+#pragma unroll
+      for (int q = 1; q < RL; ++q) a[q] = twmul<INV>(a[q], tw[q * j]);
+      BF<RL, INV, T>::run(a);
+#pragma unroll
+      for (int q = 0; q < RL; ++q) v[it + q * w] = a[q];
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = lane + P * e;
+      v[e] = k < K ? cur[k] : mk(T(0), T(0));
+    }
+  }
+  return cur == A ? B : A;
 }
 
 // Row I/O through raw buffer instructions.  A row is described by a buffer
@@ -168,16 +261,6 @@ struct RowCtx {
   const cx<T>* twK;  // exp(-2 pi i n/K), n < K, in LDS
   const cx<T>* twM;  // exp(-2 pi i n/(2K)), n < K, in LDS
 
-  // Load the half spectrum X[0..K] (entries k >= hin are zero, entry k is
-  // scaled by mul(k)) and c2r it; returns the buffer holding the packed real
-  // row z[n] = x[2n] + i x[2n+1], n < K.  Unnormalised.
-  template <class Mul>
-  __device__ __forceinline__ cx<T>* c2r(const cx<T>* s, int hin, Mul mul) const {
-    cx<T> x[E + 1];
-    load_raw(x, s, hin);
-    return c2r_from([&](int e, int k) { return x[e] * mul(k); }, hin);
-  }
-
   // the lane's entries k = lane + P*e (e <= E) of a half-spectrum row whose
   // first hin entries are stored; entries k >= hin load as zero
   __device__ __forceinline__ void load_raw(cx<T> (&x)[E + 1], const cx<T>* s, int hin) const {
@@ -187,10 +270,12 @@ struct RowCtx {
   }
 
   // c2r of the half spectrum whose entry k = lane + P*e is val(e, k) (val
-  // returns zero for the entries k >= hin that load_raw fetched as zero)
+  // returns zero for the entries k >= hin that load_raw fetched as zero).
+  // Result: the packed real row z[n] = x[2n] + i x[2n+1] in registers,
+  // v[e] = z[lane + P*e].  Unnormalised.
   template <class Val>
-  __device__ __forceinline__ cx<T>* c2r_from(Val val, int hin) const {
-    wave_sync();  // the previous transform's result buffer may still be read
+  __device__ __forceinline__ void c2r_to(cx<T> (&v)[E], Val val) const {
+    wave_sync();  // the previous transform's buffers may still be read
 #pragma unroll
     for (int e = 0; e <= E; ++e) {
       const int k = lane + P * e;
@@ -202,7 +287,10 @@ struct RowCtx {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = lane + P * e;
+      cx<T> z = mk(T(0), T(0));
       if (k < K) {
+        // X[k] read back rather than recomputed: the operands of val die at
+        // the barrier
         cx<T> Ak = B[k];
         cx<T> Bk = conj(B[K - k]);
         if (k == 0) {
@@ -211,40 +299,43 @@ struct RowCtx {
         }
         cx<T> Ev = Ak + Bk;
         cx<T> O = twmul<true>(Ak - Bk, twM[k]);
-        A[k] = mk(Ev.x - O.y, Ev.y + O.x);
+        z = mk(Ev.x - O.y, Ev.y + O.x);
       }
+      v[e] = z;
     }
-    wave_sync();
-    return row_fft<T, K, true>(A, B, lane, twK);
+    // the transform's first write goes to A; B is next written after its
+    // first barrier
+    row_fft_regs<T, K, true, false>(v, A, B, lane, twK);
   }
 
-  __device__ __forceinline__ cx<T>* c2r(const cx<T>* s, int hin) const {
-    return c2r(s, hin, One{});
+  // c2r of a stored half-spectrum row (first hin entries, the rest zero)
+  // scaled by mul(k)
+  template <class Mul>
+  __device__ __forceinline__ void c2r(cx<T> (&v)[E], const cx<T>* s, int hin, Mul mul) const {
+    cx<T> x[E + 1];
+    load_raw(x, s, hin);
+    c2r_to(v, [&](int e, int k) { return x[e] * mul(k); });
   }
   struct One {
     __device__ __forceinline__ T operator()(int) const { return T(1); }
   };
-
-  // lane's elements n = lane + P*e of a row buffer
-  __device__ __forceinline__ void get(cx<T> (&v)[E], const cx<T>* buf) const {
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int n = lane + P * e;
-      v[e] = n < K ? buf[n] : mk(T(0), T(0));
-    }
-  }
+  __device__ __forceinline__ void c2r(cx<T> (&v)[E], const cx<T>* s, int hin) const { c2r(v, s, hin, One{}); }
 
   // r2c of the packed real row held as v[e] at n = lane + P*e; stores
   // X[k]*scale for k < hout.  Unnormalised.
   __device__ __forceinline__ void r2c_store(const cx<T> (&v)[E], cx<T>* d, int hout, bool act, T scale) const {
-    wave_sync();  // A may still be read by the previous step of this row
+    cx<T> z[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) z[e] = v[e];
+    wave_sync();  // the buffers may still be read by the previous step of this row
+    cx<T>* F = row_fft_regs<T, K, false, true>(z, A, B, lane, twK);
+    // Z in natural order in registers; the partner Z[K-k] comes through LDS
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int n = lane + P * e;
-      if (n < K) A[n] = v[e];
+      const int k = lane + P * e;
+      if (k < K) F[k] = z[e];
     }
     wave_sync();
-    cx<T>* Z = row_fft<T, K, false>(A, B, lane, twK);
     // entries k >= hout (all of them for an inactive row) are dropped by the
     // range check of the row's buffer resource
     const __amdgpu_buffer_rsrc_t r = row_rsrc(d, act ? (unsigned)hout * sizeof(cx<T>) : 0u);
@@ -254,10 +345,11 @@ struct RowCtx {
       if (k <= K) {
         cx<T> X;
         if (k == K) {
-          X = mk(Z[0].x - Z[0].y, T(0));
+          const cx<T> Z0 = F[0];
+          X = mk(Z0.x - Z0.y, T(0));
         } else {
           // X[k] = (Z[k] + conj Z[K-k])/2 - i/2 e^{-2 pi i k/(2K)} (Z[k] - conj Z[K-k])
-          cx<T> Zk = Z[k], Zc = conj(Z[(K - k) % K]);
+          cx<T> Zk = z[e < E ? e : E - 1], Zc = conj(F[(K - k) % K]);
           cx<T> a = (Zk + Zc) * T(0.5);
           cx<T> dd = cmul((Zk - Zc) * T(0.5), twM[k]);
           X = mk(a.x + dd.y, a.y - dd.x);
@@ -265,7 +357,8 @@ struct RowCtx {
         buf_store<T>(X * scale, r, (unsigned)k * sizeof(cx<T>));
       }
     }
-    wave_sync();
+    // F is read until here: the next transform of this row context begins
+    // with a barrier
   }
 };
 
