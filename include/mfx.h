@@ -167,6 +167,26 @@ int mfx_conv_multiply_field(mfx_conv *cv, mfx_grid *inl, mfx_grid *inr, mfx_grid
 int mfx_lpt_sources(mfx_conv *cv, mfx_grid *phi, mfx_grid *phi2, mfx_grid *phi3, mfx_grid *A3x, mfx_grid *A3y,
                     mfx_grid *A3z, int order);
 
+/* 3LPT evaluation path of mfx_lpt_sources (no counterpart in the reference,
+ * whose convolve calls have no such choice):
+ *   MFX_LPT_STREAMED  every stage transforms the Hessian rows it needs
+ *                     (workspace: 3 m*n*(n/2+1) + 16 m*m*(n/2+1) complex
+ *                     + 4 n-grids, m = 3n/2);
+ *   MFX_LPT_STORED    stage A also stores phi's real-space Hessian rows and
+ *                     the phi3 and A3 sources are formed in one pass over them
+ *                     (half the padded transforms; workspace 11 instead of 16
+ *                     m*m*(n/2+1) slots plus 6 m^3 reals);
+ *   MFX_LPT_AUTO      stored when its workspace fits in free device memory
+ *                     with headroom, streamed otherwise (the default).
+ * Both paths give the same fields up to rounding.  The choice is made when the
+ * 3LPT workspace is first reserved and kept until the path is set again.
+ * mfx_conv_lpt_path reports it (MFX_LPT_AUTO while not yet made). */
+#define MFX_LPT_AUTO 0
+#define MFX_LPT_STREAMED 1
+#define MFX_LPT_STORED 2
+int mfx_conv_set_lpt_path(mfx_conv *cv, int path);
+int mfx_conv_lpt_path(mfx_conv *cv, int *path);
+
 /* ------------------------------------------------------------ assembly
  * k-space displacement (kind 0) or velocity (kind 1) component idim built from
  * the growth-scaled potentials, times unit/L, zero DC

@@ -26,7 +26,7 @@ EXPORTS = [
     "mfx_grid_power_spectrum", "mfx_noise_normalise", "mfx_phi1", "mfx_conv_create", "mfx_conv_destroy",
     "mfx_conv2", "mfx_conv3", "mfx_conv_gradients", "mfx_conv_gradient_and_hessian", "mfx_conv_hessians",
     "mfx_conv_hessians3", "mfx_conv_sum_of_hessians", "mfx_conv_difference_of_hessians",
-    "mfx_conv_multiply_field", "mfx_lpt_sources", "mfx_plt_create", "mfx_plt_destroy", "mfx_plt_tables",
+    "mfx_conv_multiply_field", "mfx_lpt_sources", "mfx_conv_set_lpt_path", "mfx_conv_lpt_path", "mfx_plt_create", "mfx_plt_destroy", "mfx_plt_tables",
     "mfx_plt_dmatrix", "mfx_plt_gradient", "mfx_assemble_kspace_plt", "mfx_assemble_kspace", "mfx_set_particles",
     "mfx_rng_music1_fill", "mfx_rng_ngenic_fill", "mfx_rng_threefry_fill", "mfx_rng_mt19937_raw",
     "mfx_rng_ranlxd1_raw", "mfx_rng_threefry_uniforms", "mfx_cosmo_create", "mfx_cosmo_destroy",
@@ -105,6 +105,7 @@ def load():
         "mfx_conv_difference_of_hessians": [vp, vp, vp, vp, vp, vp, vp, i32, dbl],
         "mfx_conv_multiply_field": [vp, vp, vp, vp, i32, dbl],
         "mfx_lpt_sources": [vp, vp, vp, vp, vp, vp, vp, i32],
+        "mfx_conv_set_lpt_path": [vp, i32], "mfx_conv_lpt_path": [vp, ctypes.POINTER(i32)],
         "mfx_assemble_kspace": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, dbl, dbl],
         "mfx_assemble_kspace_plt": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, dbl, dbl, vp],
         "mfx_plt_create": [vp, i32, pp], "mfx_plt_destroy": [vp],

@@ -295,6 +295,23 @@ class OrszagConvolver:
         h = lambda g: g.handle if g is not None else None
         call("mfx_lpt_sources", self._h, phi.handle, phi2.handle, h(phi3), h(A3[0]), h(A3[1]), h(A3[2]), order)
 
+    LPT_PATHS = {"auto": 0, "streamed": 1, "stored": 2}
+
+    def set_lpt_path(self, path):
+        """3LPT evaluation path of lpt_sources: "streamed" (every stage
+        transforms its Hessian rows), "stored" (phi's real-space Hessian rows
+        are kept, half the padded transforms, larger workspace) or "auto"
+        (stored when it fits in free device memory)."""
+        call("mfx_conv_set_lpt_path", self._h, self.LPT_PATHS[path])
+
+    @property
+    def lpt_path(self):
+        """path the 3LPT workspace was reserved for ("auto" before the first
+        3LPT call)"""
+        v = ctypes.c_int()
+        call("mfx_conv_lpt_path", self._h, ctypes.byref(v))
+        return {i: k for k, i in self.LPT_PATHS.items()}[v.value]
+
 
 # ------------------------------------------------------------------ RNG plugins (include/random_plugin.hh)
 class RNG_music:

@@ -2,6 +2,7 @@
 // counterpart of ic_generator::run for the sc lattice.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -116,6 +117,8 @@ struct mfx_conv {
   size_t ws_bytes = 0;
   void* lpt_ws = nullptr;
   size_t lpt_bytes = 0;
+  int lpt_path = MFX_LPT_AUTO;  // requested 3LPT path
+  int lpt_chosen = MFX_LPT_AUTO;  // path the 3LPT workspace was reserved for
   mfx_grid* fbuf = nullptr;  // convolve3 intermediate (convolution.hh:469)
 };
 
@@ -797,9 +800,21 @@ static void lpt_sequence(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* 
   }
 }
 
+// 3LPT path: as requested, or stored when its workspace fits beside what
+// is allocated with 8 GiB (and 5 % of the device) to spare
+static int lpt_choose(mfx_conv* cv) {
+  if (cv->lpt_path != MFX_LPT_AUTO) return cv->lpt_path;
+  const size_t st = lpt_workspace_bytes(cv->n, cv->prec, 3, true);
+  size_t fr = 0, tot = 0;
+  MFX_HIP(hipMemGetInfo(&fr, &tot));
+  const size_t headroom = std::max((size_t)8 << 30, tot / 20);
+  return st + headroom <= fr + cv->lpt_bytes ? MFX_LPT_STORED : MFX_LPT_STREAMED;
+}
+
 // workspace of the fused engine for LPT order `order` (grown on demand)
 static void lpt_reserve(mfx_conv* cv, int order) {
-  const size_t need = lpt_workspace_bytes(cv->n, cv->prec, order);
+  if (order > 2 && cv->lpt_chosen == MFX_LPT_AUTO) cv->lpt_chosen = lpt_choose(cv);
+  const size_t need = lpt_workspace_bytes(cv->n, cv->prec, order, cv->lpt_chosen == MFX_LPT_STORED);
   if (cv->lpt_bytes < need) {
     MFX_HIP(hipStreamSynchronize(cv->ctx->c.stream));
     cv->ctx->c.release(cv->lpt_ws);
@@ -822,11 +837,27 @@ static void lpt_fused(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi
   if (cv->prec == MFX_FLOAT)
     lpt_sources<float>(cv->ctx->c, (const cx<float>*)phi->d, (cx<float>*)phi2->d, (cx<float>*)P(phi3),
                        (cx<float>*)P(A3[0]), (cx<float>*)P(A3[1]), (cx<float>*)P(A3[2]), cv->n, cv->L, order,
-                       cv->lpt_ws);
+                       cv->lpt_chosen == MFX_LPT_STORED, cv->lpt_ws);
   else
     lpt_sources<double>(cv->ctx->c, (const cx<double>*)phi->d, (cx<double>*)phi2->d, (cx<double>*)P(phi3),
                         (cx<double>*)P(A3[0]), (cx<double>*)P(A3[1]), (cx<double>*)P(A3[2]), cv->n, cv->L, order,
-                        cv->lpt_ws);
+                        cv->lpt_chosen == MFX_LPT_STORED, cv->lpt_ws);
+}
+
+int mfx_conv_set_lpt_path(mfx_conv* cv, int path) {
+  return guard([&] {
+    MFX_REQUIRE(cv, "null convolver");
+    MFX_REQUIRE(path == MFX_LPT_AUTO || path == MFX_LPT_STREAMED || path == MFX_LPT_STORED, "invalid LPT path");
+    cv->lpt_path = path;
+    cv->lpt_chosen = MFX_LPT_AUTO;
+  });
+}
+
+int mfx_conv_lpt_path(mfx_conv* cv, int* path) {
+  return guard([&] {
+    MFX_REQUIRE(cv && path, "null argument");
+    *path = cv->lpt_chosen;
+  });
 }
 
 int mfx_lpt_sources(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3x, mfx_grid* A3y,

@@ -78,6 +78,7 @@ struct ZRow {
   long long off;  // row * pitch
   int hin, n;
   T kfac;
+  long long roff;  // row * K: offset of the row in a real-space component array
   __device__ __forceinline__ void fetch(Raw& x, const cx<T>* p) const { rc.load_raw(x, p + off, hin); }
   template <int A, int B>
   __device__ __forceinline__ void fetch(Raw& x, const FieldY<T>& f) const {
@@ -103,6 +104,18 @@ struct ZRow {
   __device__ __forceinline__ void emit(const cx<T> (&v)[E], cx<T>* p) const {
     rc.r2c_store(v, p + off, hin, act, T(1));
   }
+  // real-space row of a stored Hessian component: K packed pairs per row,
+  // rows contiguous; an inactive row stores nothing
+  __device__ __forceinline__ void store_real(const cx<T> (&v)[E], cx<T>* p) const {
+    const __amdgpu_buffer_rsrc_t r = row_rsrc(p + roff, act ? (unsigned)(K * sizeof(cx<T>)) : 0u);
+#pragma unroll
+    for (int e = 0; e < E; ++e) buf_store<T>(v[e], r, (unsigned)(rc.lane + RowCtx<T, K>::P * e) * sizeof(cx<T>));
+  }
+  __device__ __forceinline__ void load_real(cx<T> (&v)[E], const cx<T>* p) const {
+    const __amdgpu_buffer_rsrc_t r = row_rsrc(p + roff, (unsigned)(K * sizeof(cx<T>)));
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = buf_load<T>(r, (unsigned)(rc.lane + RowCtx<T, K>::P * e) * sizeof(cx<T>));
+  }
 };
 
 template <typename T>
@@ -114,29 +127,39 @@ __device__ __forceinline__ cx<T> pmul(cx<T> a, cx<T> b) {
   MFX_ROW_PROLOGUE(T, K)                                                            \
   constexpr int EPT = RowCtx<T, K>::E;                                              \
   using Raw = typename ZRow<T, K>::Raw;                                             \
-  ZRow<T, K> z{rc, act, rowc * pitch, hin, n, kfac};
+  ZRow<T, K> z{rc, act, rowc * pitch, hin, n, kfac, rowc * (long long)(K)};
 
 // stage A: 2LPT source S2 = sum_{i<j} (phi_ii phi_jj - phi_ij^2) (ic_generator.cc:471-475)
 // and, for 3LPT, the 3a intermediates I1 = phi_xx phi_yy - phi_xy^2, I2 = phi_xy phi_xz,
 // I3 = phi_yz^2, I4 = phi_xz^2 (ic_generator.cc:511-515)
-template <typename T, int K, bool THREE>
+// With STORE, the six real-space Hessian rows are also written to the stored
+// component arrays R + c*rstride in the order xx, yy, zz, xy, xz, yz.
+template <typename T, int K, bool THREE, bool STORE>
 __global__ void __launch_bounds__(64)
-    k_lpt_zA(FieldY<T> f, cx<T>* S2, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4, long long pitch, int hin, int n,
-             T kfac, long long nrows, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+    k_lpt_zA(FieldY<T> f, cx<T>* S2, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4, cx<T>* R, long long rstride,
+             long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
+             const cx<T>* __restrict__ twM) {
   MFX_ZROW_PROLOGUE
   Raw r0, r1;
   cx<T> dxx[EPT], dyy[EPT], s2[EPT], w[EPT], oxy[EPT];
+  auto keep = [&](const cx<T>(&v)[EPT], int c) {
+    if constexpr (STORE) z.store_real(v, R + c * rstride);
+  };
   z.template fetch<0, 0>(r0, f);
   z.template fetch<1, 1>(r1, f);
   z.template hess<0, 0>(dxx, r0);
+  keep(dxx, 0);
   z.template fetch<2, 2>(r0, f);
   z.template hess<1, 1>(dyy, r1);
+  keep(dyy, 1);
   z.template fetch<0, 1>(r1, f);
   z.template hess<2, 2>(w, r0);
+  keep(w, 2);
   z.template fetch<0, 2>(r0, f);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = pmul(dxx[e], dyy[e]) + pmul(dxx[e] + dyy[e], w[e]);
   z.template hess<0, 1>(oxy, r1);
+  keep(oxy, 3);
   z.template fetch<1, 2>(r1, f);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(oxy[e], oxy[e]);
@@ -146,6 +169,7 @@ __global__ void __launch_bounds__(64)
     z.emit(w, I1);
   }
   z.template hess<0, 2>(dxx, r0);  // dxx now holds phi_xz
+  keep(dxx, 4);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(dxx[e], dxx[e]);
   if (THREE) {
@@ -157,6 +181,7 @@ __global__ void __launch_bounds__(64)
     z.emit(w, I4);
   }
   z.template hess<1, 2>(dyy, r1);  // dyy now holds phi_yz
+  keep(dyy, 5);
 #pragma unroll
   for (int e = 0; e < EPT; ++e) s2[e] = s2[e] - pmul(dyy[e], dyy[e]);
   if (THREE) {
@@ -275,18 +300,150 @@ __global__ void __launch_bounds__(64)
   z.emit(acc, out);
 }
 
+// stage C over the stored phi Hessian rows: the phi3 source of stage C1 and
+// the three A(3) components of stage C2 (ic_generator.cc:511-545) in one pass
+// over a z row.  The six phi2 Hessian rows are transformed once and held in
+// registers; the stored real phi rows are streamed one component at a time
+// and every product with the held component is accumulated at once; each 3a
+// intermediate is transformed while its phi partner is held:
+//   phi_xx: 3b -5/7 phi_xx (phi2_yy + phi2_zz), 3a -I3 phi_xx,
+//           A3y += phi2_xz phi_xx, A3z -= phi2_xy phi_xx
+//   phi_yy: 3b -5/7 phi_yy (phi2_xx + phi2_zz), 3a -I4 phi_yy,
+//           A3x -= phi2_yz phi_yy, A3z += phi2_xy phi_yy
+//   phi_zz: 3b -5/7 phi_zz (phi2_xx + phi2_yy), 3a +I1 phi_zz,
+//           A3x += phi2_yz phi_zz, A3y -= phi2_xz phi_zz
+//   phi_xy: 3b +10/7 phi_xy phi2_xy, A3x -= phi2_xz phi_xy,
+//           A3y += phi2_yz phi_xy, A3z += phi_xy (phi2_xx - phi2_yy)
+//   phi_xz: 3b +10/7 phi_xz phi2_xz, A3x += phi2_xy phi_xz,
+//           A3y += phi_xz (phi2_zz - phi2_xx), A3z -= phi2_yz phi_xz
+//   phi_yz: 3b +10/7 phi_yz phi2_yz, 3a +2 I2 phi_yz,
+//           A3x += phi_yz (phi2_yy - phi2_zz), A3y -= phi2_xy phi_yz,
+//           A3z += phi2_xz phi_yz
+// (the diagonal 3b terms are 5/7 phi_ii phi2_ii - 5/7 phi_ii tr(phi2)).  The
+// outputs overwrite the intermediates' rows -- phi3 -> I1, A3x -> I2,
+// A3y -> I3, A3z -> I4 -- all of which are consumed before the first output
+// is written.
+namespace cs {
+// stored component order xx, yy, zz, xy, xz, yz
+constexpr int ca[6] = {0, 1, 2, 0, 0, 1};
+constexpr int cb[6] = {0, 1, 2, 1, 2, 2};
+}  // namespace cs
+
 template <typename T, int K>
-void launch_zA_K(Ctx& c, const FieldY<T>& f, cx<T>* S2, cx<T>* const* I, bool three, long long pitch, int hin, int n,
-                 T kfac, long long nrows, double bytes) {
+__global__ void __launch_bounds__(64)
+    k_lpt_zC(FieldY<T> f2, const cx<T>* R, long long rstride, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4,
+             long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
+             const cx<T>* __restrict__ twM) {
+  MFX_ZROW_PROLOGUE
+  cx<T> h[6][EPT];
+  {
+    Raw rb[2];
+    z.template fetch<cs::ca[0], cs::cb[0]>(rb[0], f2);
+    static_for<0, 6>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      if constexpr (c + 1 < 6) z.template fetch<cs::ca[c + 1], cs::cb[c + 1]>(rb[(c + 1) & 1], f2);
+      z.template hess<cs::ca[c], cs::cb[c]>(h[c], rb[c & 1]);
+    });
+  }
+  const T c57 = T(5.0 / 7.0), c107 = T(10.0 / 7.0);
+  cx<T> p3[EPT], ax[EPT], ay[EPT], az[EPT], a[EPT], an[EPT], v[EPT];
+  Raw ri;
+  z.load_real(a, R);
+  z.fetch(ri, I3);
+  // phi_xx
+  z.load_real(an, R + rstride);
+  z.plain(v, ri);
+  z.fetch(ri, I4);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    p3[e] = pmul(a[e], h[1][e] + h[2][e]) * (-c57) - pmul(a[e], v[e]);
+    ax[e] = mk(T(0), T(0));
+    ay[e] = pmul(h[4][e], a[e]);
+    az[e] = mk(T(0), T(0)) - pmul(h[3][e], a[e]);
+    a[e] = an[e];
+  }
+  // phi_yy
+  z.load_real(an, R + 2 * rstride);
+  z.plain(v, ri);
+  z.fetch(ri, I1);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    p3[e] = p3[e] - pmul(a[e], h[0][e] + h[2][e]) * c57 - pmul(a[e], v[e]);
+    ax[e] = ax[e] - pmul(h[5][e], a[e]);
+    az[e] = az[e] + pmul(h[3][e], a[e]);
+    a[e] = an[e];
+  }
+  // phi_zz
+  z.load_real(an, R + 3 * rstride);
+  z.plain(v, ri);
+  z.fetch(ri, I2);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    p3[e] = p3[e] - pmul(a[e], h[0][e] + h[1][e]) * c57 + pmul(a[e], v[e]);
+    ax[e] = ax[e] + pmul(h[5][e], a[e]);
+    ay[e] = ay[e] - pmul(h[4][e], a[e]);
+    a[e] = an[e];
+  }
+  // phi_xy
+  z.load_real(an, R + 4 * rstride);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    p3[e] = p3[e] + pmul(a[e], h[3][e]) * c107;
+    ax[e] = ax[e] - pmul(h[4][e], a[e]);
+    ay[e] = ay[e] + pmul(h[5][e], a[e]);
+    az[e] = az[e] + pmul(a[e], h[0][e] - h[1][e]);
+    a[e] = an[e];
+  }
+  // phi_xz
+  z.load_real(an, R + 5 * rstride);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    p3[e] = p3[e] + pmul(a[e], h[4][e]) * c107;
+    ax[e] = ax[e] + pmul(h[3][e], a[e]);
+    ay[e] = ay[e] + pmul(a[e], h[2][e] - h[0][e]);
+    az[e] = az[e] - pmul(h[5][e], a[e]);
+    a[e] = an[e];
+  }
+  // phi_yz
+  z.plain(v, ri);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    p3[e] = p3[e] + pmul(a[e], h[5][e]) * c107 + pmul(a[e], v[e]) * T(2);
+    ax[e] = ax[e] + pmul(a[e], h[1][e] - h[2][e]);
+    ay[e] = ay[e] - pmul(h[3][e], a[e]);
+    az[e] = az[e] + pmul(h[4][e], a[e]);
+  }
+  z.emit(p3, I1);
+  z.emit(ax, I2);
+  z.emit(ay, I3);
+  z.emit(az, I4);
+}
+
+template <typename T, int K>
+void launch_zA_K(Ctx& c, const FieldY<T>& f, cx<T>* S2, cx<T>* const* I, bool three, cx<T>* R, long long rstride,
+                 long long pitch, int hin, int n, T kfac, long long nrows, double bytes) {
   const dim3 gd = row_grid<T, K>(nrows), bd(64);
   LaunchScope ls(c, "lpt_zA", bytes);
-  if (three)
-    hipLaunchKernelGGL((k_lpt_zA<T, K, true>), gd, bd, 0, c.stream, f, S2, I[0], I[1], I[2], I[3], pitch, hin, n,
-                       kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  if (three && R)
+    hipLaunchKernelGGL((k_lpt_zA<T, K, true, true>), gd, bd, 0, c.stream, f, S2, I[0], I[1], I[2], I[3], R, rstride,
+                       pitch, hin, n, kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  else if (three)
+    hipLaunchKernelGGL((k_lpt_zA<T, K, true, false>), gd, bd, 0, c.stream, f, S2, I[0], I[1], I[2], I[3], S2, 0LL,
+                       pitch, hin, n, kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
   else
-    hipLaunchKernelGGL((k_lpt_zA<T, K, false>), gd, bd, 0, c.stream, f, S2, S2, S2, S2, S2, pitch, hin, n, kfac,
-                       nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+    hipLaunchKernelGGL((k_lpt_zA<T, K, false, false>), gd, bd, 0, c.stream, f, S2, S2, S2, S2, S2, S2, 0LL, pitch,
+                       hin, n, kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
   check_launch("lpt_zA");
+}
+
+template <typename T, int K>
+void launch_zC_K(Ctx& c, const FieldY<T>& f2, const cx<T>* R, long long rstride, cx<T>* const* I, long long pitch,
+                 int hin, int n, T kfac, long long nrows, double bytes) {
+  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  LaunchScope ls(c, "lpt_zC", bytes);
+  hipLaunchKernelGGL((k_lpt_zC<T, K>), gd, bd, 0, c.stream, f2, R, rstride, I[0], I[1], I[2], I[3], pitch, hin, n,
+                     kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch("lpt_zC");
 }
 
 template <typename T, int K>
@@ -327,6 +484,7 @@ void launch_zC2_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, int i, cx<T>
 struct Layout {
   int n, m, h, hp;
   size_t t1, t2;  // elements of one x-stage slot (m*n*hp) and one y/z slot (m*m*hp)
+  size_t treal;   // complex elements of one stored real-space array (m*m*m/2)
 };
 
 static Layout layout(int n) {
@@ -337,16 +495,24 @@ static Layout layout(int n) {
   l.hp = conv_pitch(n);
   l.t1 = (size_t)l.m * n * l.hp;
   l.t2 = (size_t)l.m * l.m * l.hp;
+  l.treal = (size_t)l.m * l.m * (l.m / 2);
   return l;
 }
 
-// slots: 3 x-stage (also the y-forward output), 16 y/z, 4 n-grid intermediates
-size_t lpt_workspace_bytes(int n, int prec, int order) {
+// y/z slots of the engine: 7 for 2LPT; for 3LPT 16 when every stage
+// transforms its Hessian rows (streamed), 11 plus the six stored real-space
+// phi Hessian arrays (m^3 reals each) when stage C reads them (stored)
+static int lpt_slots(int order, bool stored) { return order < 3 ? 7 : (stored ? 11 : 16); }
+
+// 3 x-stage slots (also the y-forward output), the y/z slots, 4 n-grid
+// intermediates and, stored, the real Hessian arrays
+size_t lpt_workspace_bytes(int n, int prec, int order, bool stored) {
   Layout l = layout(n);
   const size_t csz = 2 * (size_t)prec;
   const size_t ngrid = (size_t)n * n * (n / 2 + 1);
-  if (order < 3) return csz * (3 * l.t1 + 7 * l.t2);
-  return csz * (3 * l.t1 + 16 * l.t2 + 4 * ngrid);
+  const bool st = stored && order > 2;
+  const size_t real = st ? 6 * l.treal : 0;
+  return csz * (3 * l.t1 + lpt_slots(order, st) * l.t2 + (order > 2 ? 4 * ngrid : 0) + real);
 }
 
 template <typename T>
@@ -357,9 +523,10 @@ struct Engine {
   cx<T>* T1[3];
   cx<T>* T2[16];
   cx<T>* NI[4];
-  double cs, bN, bT1, bT2;
+  cx<T>* R = nullptr;  // stored real phi Hessian arrays (stride l.treal)
+  double cs, bN, bT1, bT2, bR;
 
-  Engine(Ctx& ctx, int n, double boxlen, int order, void* ws) : c(ctx), l(layout(n)) {
+  Engine(Ctx& ctx, int n, double boxlen, int order, bool stored, void* ws) : c(ctx), l(layout(n)) {
     kfac = T(2.0 * M_PI / boxlen);
     s_in = T(1.0 / std::sqrt((double)n * n * n));
     s_out = T(std::sqrt((double)n * n * n) / ((double)l.m * l.m * l.m));
@@ -368,7 +535,7 @@ struct Engine {
       q = p;
       p += l.t1;
     }
-    const int n2 = order < 3 ? 7 : 16;
+    const int n2 = lpt_slots(order, stored);
     for (int q = 0; q < 16; ++q) {
       T2[q] = q < n2 ? p : nullptr;
       if (q < n2) p += l.t2;
@@ -377,10 +544,12 @@ struct Engine {
       q = order < 3 ? nullptr : p;
       if (order >= 3) p += (size_t)n * n * (n / 2 + 1);
     }
+    if (stored && order > 2) R = p;
     cs = sizeof(cx<T>);
     bN = cs * (double)n * n * l.h;
     bT1 = cs * (double)l.m * n * l.h;
     bT2 = cs * (double)l.m * l.m * l.h;
+    bR = cs * (double)l.treal;
   }
 
   // x and y stages of a field: Hessian rows (six Y arrays) or one plain Y0
@@ -428,9 +597,9 @@ struct Engine {
 
 template <typename T>
 void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x, cx<T>* a3y, cx<T>* a3z, int n,
-                 double boxlen, int order, void* ws) {
+                 double boxlen, int order, bool stored, void* ws) {
   MFX_REQUIRE(n % 2 == 0, "Orszag convolver requires an even grid size");
-  Engine<T> E(c, n, boxlen, order, ws);
+  Engine<T> E(c, n, boxlen, order, stored, ws);
   const Layout& l = E.l;
   const int K = l.m / 2;
   const long long nrows = (long long)l.m * l.m;
@@ -441,10 +610,10 @@ void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x,
   // stage A: S2 -> T2[6]; I1..I4 -> T2[7..10]
   cx<T>* Iz[4] = {E.T2[7], E.T2[8], E.T2[9], E.T2[10]};
   {
-    const double bytes = 6 * E.bT2 + (three ? 5 : 1) * E.bT2;
-#define MFX_CASE(KK)                                                                                        \
-  case KK:                                                                                                  \
-    launch_zA_K<T, KK>(c, fphi, E.T2[6], Iz, three, l.hp, l.h, n, E.kfac, nrows, bytes); \
+    const double bytes = 6 * E.bT2 + (three ? 5 : 1) * E.bT2 + (E.R ? 6 * E.bR : 0);
+#define MFX_CASE(KK)                                                                                          \
+  case KK:                                                                                                    \
+    launch_zA_K<T, KK>(c, fphi, E.T2[6], Iz, three, E.R, (long long)l.treal, l.hp, l.h, n, E.kfac, nrows, bytes); \
     break;
     MFX_LPT_KSWITCH(K, MFX_CASE)
 #undef MFX_CASE
@@ -452,6 +621,27 @@ void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x,
   E.forward(E.T2[6], phi2, true);  // phi2 = -S2/k^2 (ic_generator.cc:488)
   if (!three) return;
   for (int q = 0; q < 4; ++q) E.forward(Iz[q], E.NI[q], false);
+  if (E.R) {
+    // stored: phi's Hessian rows are dead after stage A, so phi2's take
+    // T2[0..5] and the intermediates' rows T2[6..9]
+    E.inverse(phi2, true, E.T2);
+    cx<T>* Ir[4];
+    for (int q = 0; q < 4; ++q) {
+      Ir[q] = E.T2[6 + q];
+      cx<T>* Y[6] = {Ir[q], nullptr, nullptr, nullptr, nullptr, nullptr};
+      E.inverse(E.NI[q], false, Y);
+    }
+    const double bytes = 10 * E.bT2 + 6 * E.bR + 4 * E.bT2;
+#define MFX_CASE(KK)                                                                                   \
+  case KK:                                                                                             \
+    launch_zC_K<T, KK>(c, E.fy(0), E.R, (long long)l.treal, Ir, l.hp, l.h, n, E.kfac, nrows, bytes); \
+    break;
+    MFX_LPT_KSWITCH(K, MFX_CASE)
+#undef MFX_CASE
+    cx<T>* out[4] = {phi3, a3x, a3y, a3z};
+    for (int q = 0; q < 4; ++q) E.forward(Ir[q], out[q], true);
+    return;
+  }
   // phi2 Hessian rows -> T2[6..11]; I1..I4 rows -> T2[12..15]
   E.inverse(phi2, true, E.T2 + 6);
   const FieldY<T> fphi2 = E.fy(6);
@@ -487,8 +677,8 @@ void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x,
 }
 
 template void lpt_sources<float>(Ctx&, const cx<float>*, cx<float>*, cx<float>*, cx<float>*, cx<float>*, cx<float>*,
-                                 int, double, int, void*);
+                                 int, double, int, bool, void*);
 template void lpt_sources<double>(Ctx&, const cx<double>*, cx<double>*, cx<double>*, cx<double>*, cx<double>*,
-                                  cx<double>*, int, double, int, void*);
+                                  cx<double>*, int, double, int, bool, void*);
 
 }  // namespace mfx

@@ -47,9 +47,9 @@ void mt19937_uniforms(Ctx& c, unsigned int* out, const unsigned long* seeds, int
 void ngenic_seedtable(unsigned* table, int n, long seed);
 
 // fused LPT source engine
-size_t lpt_workspace_bytes(int n, int prec, int order);
+size_t lpt_workspace_bytes(int n, int prec, int order, bool stored);
 template <typename T>
 void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x, cx<T>* a3y, cx<T>* a3z, int n,
-                 double boxlen, int order, void* ws);
+                 double boxlen, int order, bool stored, void* ws);
 
 }  // namespace mfx

@@ -41,18 +41,20 @@ def test_kspace_phi_is_the_cosine_sum(orc):
 
 
 @pytest.mark.gpu
-def test_lpt_sources_1024_band_limited(ctx, orc):
+@pytest.mark.parametrize("path", ["streamed", "auto"])
+def test_lpt_sources_1024_band_limited(ctx, orc, path):
     import monofonic_amd as M
     n, ns, step = 1024, 64, 16
-    print("band-limited 3LPT sources at 1024^3 FP32 (1536^3 padded)", flush=True)
+    print(f"band-limited 3LPT sources at 1024^3 FP32 (1536^3 padded), path {path}", flush=True)
     phi = M.Grid_FFT(ctx, n, BOX, "float", initialspace=M.kspace_id)
     phi.upload(kspace_phi(n, np.float32))
     out = [M.Grid_FFT(ctx, n, BOX, "float", initialspace=M.kspace_id) for _ in range(5)]
     conv = M.OrszagConvolver(ctx, n, BOX, "float")
+    conv.set_lpt_path(path)
     conv.lpt_sources(phi, out[0], out[1], out[2:], 3)
     ctx.synchronize()
+    print(f"  sources done on the {conv.lpt_path} path", flush=True)
     del conv
-    print("  sources done", flush=True)
     r2, r3, rA = orc.lpt_terms(kspace_phi(ns, np.float64), BOX, 3)
     refs = [r2, r3] + list(rA)
     names = ["phi2", "phi3", "A3x", "A3y", "A3z"]

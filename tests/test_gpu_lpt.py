@@ -157,19 +157,22 @@ def test_icgen_fp32_bulk_vs_fp64(ctx):
         assert rel_rms(a, b) < 1e-5, w
 
 
-@pytest.mark.parametrize("n,order", [(32, 3), (48, 3), (64, 2)])
-def test_fused_lpt_engine_vs_oracle(ctx, orc, n, order):
+@pytest.mark.parametrize("n,order,path", [(32, 3, "streamed"), (32, 3, "stored"), (48, 3, "streamed"),
+                                          (48, 3, "stored"), (64, 2, "auto")])
+def test_fused_lpt_engine_vs_oracle(ctx, orc, n, order, path):
     """mfx_lpt_sources (Hessian-tensor engine) equals the reference's 5/33
-    convolve calls, restated by the oracle"""
+    convolve calls, restated by the oracle, on both 3LPT paths"""
     import monofonic_amd as M
-    print(f"fused lpt n={n} order={order}", flush=True)
+    print(f"fused lpt n={n} order={order} path={path}", flush=True)
     phi_h = random_kspace_field(orc, n, 8, slope=-3.0)
     phi = _kgrid(M, ctx, n, "double")
     phi.upload(phi_h)
     phi2, phi3 = _kgrid(M, ctx, n, "double"), _kgrid(M, ctx, n, "double")
     A3 = [_kgrid(M, ctx, n, "double") for _ in range(3)]
     conv = M.OrszagConvolver(ctx, n, BOX, "double")
+    conv.set_lpt_path(path)
     conv.lpt_sources(phi, phi2, phi3, A3, order)
+    assert conv.lpt_path == (path if order > 2 else "auto")
     r2, r3, rA = orc.lpt_terms(phi_h, BOX, order)
     pairs = [(phi2, r2)] + ([(phi3, r3)] + list(zip(A3, rA)) if order > 2 else [])
     for g, r in pairs:
@@ -177,23 +180,28 @@ def test_fused_lpt_engine_vs_oracle(ctx, orc, n, order):
         assert rel_max(g.kdata(), orc.kview(r)) < 1e-10
 
 
-def test_fused_fp32_vs_reference_sequence_fp32(ctx, orc):
+@pytest.mark.parametrize("n", [64, 128])
+def test_fused_fp32_vs_reference_sequence_fp32(ctx, orc, n):
+    """FP32: both 3LPT paths of the fused engine against the reference's
+    convolve sequence run in FP32 on the same device"""
     import monofonic_amd as M
-    n = 64
+    print(f"fused fp32 paths n={n}", flush=True)
     phi_h = random_kspace_field(orc, n, 9, slope=-3.0)
     res = {}
-    for fused in (True, False):
+    for mode in ("streamed", "stored", "sequence"):
         phi = _kgrid(M, ctx, n, "float")
         phi.upload(phi_h.astype(np.float32))
         out = [_kgrid(M, ctx, n, "float") for _ in range(5)]
         conv = M.OrszagConvolver(ctx, n, BOX, "float")
-        if fused:
-            conv.lpt_sources(phi, out[0], out[1], out[2:], 3)
-        else:
+        if mode == "sequence":
             gpu_lpt_reference_sequence(M, conv, phi, out[0], out[1], out[2:], 3)
-        res[fused] = [g.kdata() for g in out]
-    for a, b in zip(res[True], res[False]):
-        assert rel_rms(a, b) < 1e-5
+        else:
+            conv.set_lpt_path(mode)
+            conv.lpt_sources(phi, out[0], out[1], out[2:], 3)
+        res[mode] = [g.kdata() for g in out]
+    for mode in ("streamed", "stored"):
+        for a, b in zip(res[mode], res["sequence"]):
+            assert rel_rms(a, b) < 1e-5, mode
 
 
 @pytest.mark.parametrize("fused", [True, False])
