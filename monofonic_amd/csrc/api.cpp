@@ -1502,20 +1502,16 @@ static void icgen_device_run(mfx_icgen* ic) {
     else
       lpt_sequence(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
   }
-  // growth factors (550-563)
-  auto sc = [&](mfx_grid* g, double s) {
-    if (g->prec == MFX_FLOAT)
-      scale<float>(ic->ctx->c, (float*)g->d, (long long)g->ntot, s);
-    else
-      scale<double>(ic->ctx->c, (double*)g->d, (long long)g->ntot, s);
-  };
-  sc(phi, g1);
-  if (p.lpt_order > 1) sc(phi2, g2);
-  if (p.lpt_order > 2) {
-    sc(phi3, g3);
-    for (auto* a : A3) sc(a, g3c);
-  }
-  // positions and velocities (821-931), lunit = vunit = 1
+  // growth factors (550-563), applied in the assembly as the potentials are
+  // read; the potential grids keep the unscaled fields
+  Growth gr;
+  gr.g1 = g1;
+  gr.g2 = g2;
+  gr.g3 = g3;
+  gr.g3c = g3c;
+  // positions and velocities (821-931), lunit = vunit = 1; the inverse
+  // transform of each component writes its particle array directly and leaves
+  // tmp as scratch
   for (int kind = 0; kind < 2; ++kind)
     for (int idim = 0; idim < 3; ++idim) {
       tmp->space = MFX_KSPACE;
@@ -1524,18 +1520,18 @@ static void icgen_device_run(mfx_icgen* ic) {
         assemble<float>(ic->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2),
                         (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
                         (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact, 1.0,
-                        ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
+                        ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0, gr);
       else
         assemble<double>(ic->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2),
                          (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]), (const cx<double>*)P(A3[1]),
                          (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact,
-                         1.0, ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0);
-      grid_backward(tmp, true);
+                         1.0, ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0, gr);
+      double* out = ic->part[3 * kind + idim];
       if (tmp->prec == MFX_FLOAT)
-        set_particles<float>(ic->ctx->c, (const float*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0, kind);
+        fft_c2r_particles<float>(ic->ctx->c, (float*)tmp->d, (int)p.n, (float)norm_fac(tmp), out, idim, 1.0, kind);
       else
-        set_particles<double>(ic->ctx->c, (const double*)tmp->d, ic->part[3 * kind + idim], (int)p.n, idim, 1.0,
-                              kind);
+        fft_c2r_particles<double>(ic->ctx->c, (double*)tmp->d, (int)p.n, norm_fac(tmp), out, idim, 1.0, kind);
+      tmp->space = MFX_RSPACE;
     }
 }
 

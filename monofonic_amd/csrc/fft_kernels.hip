@@ -51,6 +51,25 @@ void fft_c2r_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale) {
   launch_row_c2r<T>(c, n2 / 2, cd, H, (int)H, data, n2 + 2, (long long)n0 * n1, scale, "fft_z_c2r", 2 * S);
 }
 
+// inverse transform of an n^3 k-space grid into one FP64 particle array: the
+// x and y passes in place, the z pass writing the particles (positions add
+// the lattice coordinate); the grid is left partially transformed
+template <typename T>
+void fft_c2r_particles(Ctx& c, T* data, int n, T scale, double* out, int idim, double lunit, int kind) {
+  MFX_REQUIRE(n % 2 == 0, "c2r requires an even last dimension");
+  const long long H = n / 2 + 1;
+  cx<T>* cd = reinterpret_cast<cx<T>*>(data);
+  const double S = (double)n * n * (n + 2) * sizeof(T);
+  launch_col<T, true>(c, n, SrcPlain<T>{cd, H, (long long)n * H}, DstPlain<T>{cd, H, (long long)n * H, T(1)},
+                      (int)H, n, "fft_x_inv", 2 * S);
+  launch_col<T, true>(c, n, SrcPlain<T>{cd, (long long)n * H, H}, DstPlain<T>{cd, (long long)n * H, H, T(1)},
+                      (int)H, n, "fft_y_inv", 2 * S);
+  launch_row_c2r_part<T>(c, n / 2, cd, H, (int)H, out, (long long)n * n, scale, idim, lunit, kind, "fft_z_c2r_part",
+                         S + (double)n * n * n * sizeof(double));
+}
+
+template void fft_c2r_particles<float>(Ctx&, float*, int, float, double*, int, double, int);
+template void fft_c2r_particles<double>(Ctx&, double*, int, double, double*, int, double, int);
 template void fft_r2c_3d<float>(Ctx&, float*, int, int, int, float);
 template void fft_r2c_3d<double>(Ctx&, double*, int, int, int, double);
 template void fft_c2r_3d<float>(Ctx&, float*, int, int, int, float);

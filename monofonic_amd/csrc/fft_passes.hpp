@@ -378,6 +378,41 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// c2r of the z rows of an n^3 grid straight into the FP64 particle arrays
+// (particle_generator.hh:525-541 positions, 622-632 velocities): each real
+// value is scaled and rounded to T as the in-place c2r stores it, converted
+// to double and, for positions (kind 0), added to the lattice coordinate
+// q_d = i_d/n * lunit.  Row r = i*n + j holds particles r*n .. r*n + n-1.
+template <typename T, int K>
+__global__ void __launch_bounds__(64)
+    k_row_c2r_part(const cx<T>* src, long long spitch, int hin, double* out, long long nrows, T scale, int idim,
+                   double lunit, int kind, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+  MFX_ROW_PROLOGUE(T, K)
+  constexpr int E = RowCtx<T, K>::E, n = 2 * K;
+  cx<T> v[E];
+  rc.c2r(v, src + rowc * spitch, hin);
+  const int i = (int)(rowc / n), j = (int)(rowc % n);
+  const double qrow = (double)(idim == 0 ? i : j) / (double)n * lunit;
+  const __amdgpu_buffer_rsrc_t r = row_rsrc(out + rowc * n, act ? (unsigned)(n * sizeof(double)) : 0u);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int m = lane + P_ * e;
+    if (m < K) {
+      double f0 = (double)(v[e].x * scale), f1 = (double)(v[e].y * scale);
+      if (kind == 0) {
+        if (idim == 2) {
+          f0 = (double)(2 * m) / (double)n * lunit + f0;
+          f1 = (double)(2 * m + 1) / (double)n * lunit + f1;
+        } else {
+          f0 = qrow + f0;
+          f1 = qrow + f1;
+        }
+      }
+      buf_store<double>(mk(f0, f1), r, (unsigned)m * sizeof(cx<double>));
+    }
+  }
+}
+
 // P3: c2r of two truncated half spectra, pointwise product, r2c, truncation
 template <typename T, int K>
 __global__ void __launch_bounds__(64)
@@ -453,6 +488,27 @@ void launch_row_c2r(Ctx& c, int K, const cx<T>* src, long long spitch, int hin, 
 #define MFX_CASE(KK)                                                                              \
   case KK:                                                                                        \
     launch_row_c2r_K<T, KK>(c, src, spitch, hin, dst, dpitch, nrows, scale, name, bytes);        \
+    return;
+  MFX_ROW_SWITCH(K, MFX_CASE)
+#undef MFX_CASE
+}
+
+template <typename T, int K>
+void launch_row_c2r_part_K(Ctx& c, const cx<T>* src, long long spitch, int hin, double* out, long long nrows, T scale,
+                           int idim, double lunit, int kind, const char* name, double bytes) {
+  const dim3 grid = row_grid<T, K>(nrows);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_row_c2r_part<T, K>), grid, dim3(64), 0, c.stream, src, spitch, hin, out, nrows, scale, idim,
+                     lunit, kind, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+  check_launch(name);
+}
+
+template <typename T>
+void launch_row_c2r_part(Ctx& c, int K, const cx<T>* src, long long spitch, int hin, double* out, long long nrows,
+                         T scale, int idim, double lunit, int kind, const char* name, double bytes) {
+#define MFX_CASE(KK)                                                                                      \
+  case KK:                                                                                                \
+    launch_row_c2r_part_K<T, KK>(c, src, spitch, hin, out, nrows, scale, idim, lunit, kind, name, bytes); \
     return;
   MFX_ROW_SWITCH(K, MFX_CASE)
 #undef MFX_CASE

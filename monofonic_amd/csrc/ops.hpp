@@ -15,11 +15,15 @@ template <typename T>
 void scale(Ctx& c, T* f, long long cnt, double s);
 template <typename T>
 void fill(Ctx& c, T* f, long long cnt, T v);
+// growth factors of ic_generator.cc:550-563: phi, phi2, phi3 and A3
+struct Growth {
+  double g1 = 1, g2 = 1, g3 = 1, g3c = 1;
+};
 // plt_tab: PLT tables (plt.hpp) or nullptr for the Fourier gradient
 template <typename T>
 void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
               const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, int kind, int idim, double vfac,
-              double unit, const double* plt_tab = nullptr, int plt_ng = 0);
+              double unit, const double* plt_tab = nullptr, int plt_ng = 0, Growth gr = Growth{});
 template <typename T>
 void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind);
 template <typename T>

@@ -88,14 +88,16 @@ __global__ void k_fill(T* f, long long cnt, T v) {
 
 // ------------------------------------------------------------------ assembly
 // ic_generator.cc:821-860 (kind 0, displacement) and 877-919 (kind 1,
-// velocity), followed by zero_DC_mode.  Potentials are already growth-scaled.
+// velocity), followed by zero_DC_mode.  The growth factors of
+// ic_generator.cc:550-563 are applied here (gr; all 1 for potentials that are
+// already growth-scaled).
 // The gradient is op::fourier_gradient (operators.hh:38-61, vfac_corr = 1),
 // or with PLT (tables in plt) particle::lattice_gradient, whose growth-rate
 // factor then multiplies the velocities (ENABLE_PLT builds).
 template <typename T, bool PLT>
 __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
                            const cx<T>* a3y, const cx<T>* a3z, int n, double k0, int order, int kind, int idim,
-                           double vfac, double unit_over_L, PltView plt, double boxlen) {
+                           double vfac, double unit_over_L, PltView plt, double boxlen, Growth gr) {
   const int nh = n / 2 + 1;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)n * n * nh) return;
@@ -110,35 +112,41 @@ __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, cons
 #pragma unroll
     for (int d = 0; d < 3; ++d) g[d] = gd_of(ijk[d], n, 1.0) * k0;
   }
+  // growth-scaled potentials, each rounded to T as the reference's in-place
+  // scaling of a real_t grid stores it
+  auto gs = [](cx<T> v, double g) { return mk(T(v.x * g), T(v.y * g)); };
   double pr, pi;
+  const cx<T> s1 = gs(phi[idx], gr.g1);
+  const cx<T> s2 = order > 1 ? gs(phi2[idx], gr.g2) : mk(T(0), T(0));
+  const cx<T> s3 = order > 2 ? gs(phi3[idx], gr.g3) : mk(T(0), T(0));
   if (kind == 0) {
-    pr = phi[idx].x;
-    pi = phi[idx].y;
+    pr = s1.x;
+    pi = s1.y;
     if (order > 1) {
-      pr += phi2[idx].x;
-      pi += phi2[idx].y;
+      pr += s2.x;
+      pi += s2.y;
     }
     if (order > 2) {
-      pr += phi3[idx].x;
-      pi += phi3[idx].y;
+      pr += s3.x;
+      pi += s3.y;
     }
   } else {
-    pr = vfac * phi[idx].x;
-    pi = vfac * phi[idx].y;
+    pr = vfac * s1.x;
+    pi = vfac * s1.y;
     if (order > 1) {
-      pr += 2 * vfac * phi2[idx].x;
-      pi += 2 * vfac * phi2[idx].y;
+      pr += 2 * vfac * s2.x;
+      pi += 2 * vfac * s2.y;
     }
     if (order > 2) {
-      pr += 3 * vfac * phi3[idx].x;
-      pi += 3 * vfac * phi3[idx].y;
+      pr += 3 * vfac * s3.x;
+      pi += 3 * vfac * s3.y;
     }
   }
   const int idp = (idim + 1) % 3, idpp = (idim + 2) % 3;
   double tr = -g[idim] * pi, ti = g[idim] * pr;
   if (order > 2) {
     const cx<T>* A3[3] = {a3x, a3y, a3z};
-    cx<T> a2 = A3[idpp][idx], a1 = A3[idp][idx];
+    cx<T> a2 = gs(A3[idpp][idx], gr.g3c), a1 = gs(A3[idp][idx], gr.g3c);
     double cr = (-g[idp] * (double)a2.y) - (-g[idpp] * (double)a1.y);
     double ci = (g[idp] * (double)a2.x) - (g[idpp] * (double)a1.x);
     if (kind == 1) {
@@ -288,18 +296,18 @@ void fill(Ctx& c, T* f, long long cnt, T v) {
 template <typename T>
 void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
               const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, int kind, int idim, double vfac,
-              double unit, const double* plt_tab, int plt_ng) {
+              double unit, const double* plt_tab, int plt_ng, Growth gr) {
   long long cnt = (long long)n * n * (n / 2 + 1);
   int nin = order == 1 ? 1 : order == 2 ? 2 : 5;
   LaunchScope ls(c, plt_tab ? "assemble_kspace_plt" : "assemble_kspace", (nin + 1.0) * cnt * sizeof(cx<T>));
   if (plt_tab)
     hipLaunchKernelGGL((k_assemble<T, true>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, tmp, phi, phi2, phi3, a3x,
                        a3y, a3z, n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen,
-                       PltView{plt_tab, plt_ng}, boxlen);
+                       PltView{plt_tab, plt_ng}, boxlen, gr);
   else
     hipLaunchKernelGGL((k_assemble<T, false>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, tmp, phi, phi2, phi3, a3x,
                        a3y, a3z, n, 2.0 * M_PI / boxlen, order, kind, idim, vfac, unit / boxlen,
-                       PltView{nullptr, 0}, boxlen);
+                       PltView{nullptr, 0}, boxlen, gr);
   check_launch("assemble");
 }
 
@@ -378,7 +386,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
   template void scale<T>(Ctx&, T*, long long, double);                                                            \
   template void fill<T>(Ctx&, T*, long long, T);                                                                  \
   template void assemble<T>(Ctx&, cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*,   \
-                            const cx<T>*, int, double, int, int, int, double, double, const double*, int);        \
+                            const cx<T>*, int, double, int, int, int, double, double, const double*, int, Growth); \
   template void set_particles<T>(Ctx&, const T*, double*, int, int, double, int);                                 \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \
