@@ -276,6 +276,9 @@ typedef struct {
   int use_plt;          /* 1: PLT-corrected assembly (ENABLE_PLT builds, ic_generator.cc:366-370) */
   int plt_ngrid;        /* PLT mesh; 0 selects the reference's 64 */
   int autotune;         /* 1: mfx_autotune for this grid size at creation (once per context) */
+  int assemble_each;    /* 1: assemble and transform one component at a time in one grid
+                           (least memory); 0: all six components from one read of the
+                           potentials when device memory allows */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;

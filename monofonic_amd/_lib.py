@@ -53,7 +53,8 @@ class IcgenParams(ctypes.Structure):
                 ("H0", ctypes.c_double), ("Omega_m", ctypes.c_double), ("Omega_b", ctypes.c_double),
                 ("n_s", ctypes.c_double), ("sigma_8", ctypes.c_double), ("Tcmb", ctypes.c_double),
                 ("zero_radiation", ctypes.c_int), ("use_fused_lpt", ctypes.c_int), ("use_plt", ctypes.c_int),
-                ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int)]
+                ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("assemble_each", ctypes.c_int)]
 
 
 def load():

@@ -516,7 +516,7 @@ class ICGenerator:
     def __init__(self, ctx, GridRes, BoxLength, zstart, LPTorder=3, precision="double", generator="MUSIC1",
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
-                 autotune=True):
+                 autotune=True, assemble_each=False):
         self.ctx = ctx
         self.n = int(GridRes)
         p = _lib.IcgenParams()
@@ -527,6 +527,7 @@ class ICGenerator:
         p.zero_radiation, p.use_fused_lpt = int(ZeroRadiation), int(fused)
         p.use_plt, p.plt_ngrid = int(PLT), int(plt_ngrid)
         p.autotune = int(autotune)
+        p.assemble_each = int(assemble_each)
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 

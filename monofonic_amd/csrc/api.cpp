@@ -1294,6 +1294,11 @@ struct mfx_icgen {
   double* amp = nullptr;
   mfx_plt* plt = nullptr;
   bool part_in_ws = false;  // particle arrays live in the LPT workspace
+  // the six k-space components assembled in one pass (n^2 (n/2+1) complex
+  // each): in the LPT workspace after the particle arrays, in an allocation
+  // of their own, or (nullptr) not held, then assembled one at a time in tmp
+  void* comp6 = nullptr;
+  bool comp6_in_ws = false;
   double dplus = 0, vfact = 0;
   // noise seed of the next realisation, staged host -> device before the
   // device sequence runs (or its graph is launched)
@@ -1321,6 +1326,8 @@ static void icgen_free(mfx_icgen* ic) {
     if (p && !ic->part_in_ws) ic->ctx->c.release(p);
     p = nullptr;
   }
+  if (ic->comp6 && !ic->comp6_in_ws) ic->ctx->c.release(ic->comp6);
+  ic->comp6 = nullptr;
   if (ic->amp) ic->ctx->c.release(ic->amp);
   if (ic->plt) mfx_plt_destroy(ic->plt);
   if (ic->gexec) (void)hipGraphExecDestroy(ic->gexec);
@@ -1374,6 +1381,19 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       }
       if (!ic->part_in_ws)
         for (auto*& pp : ic->part) pp = (double*)c->c.alloc(sizeof(double) * npart);
+      // six component grids: after the particles in the LPT workspace, else
+      // their own allocation when it leaves 8 GiB free
+      const size_t comp_bytes = 6 * (size_t)p->n * p->n * (p->n / 2 + 1) * 2 * (size_t)p->precision;
+      if (p->assemble_each) {
+        // one component at a time in tmp
+      } else if (ic->part_in_ws && ic->conv->lpt_bytes >= 6 * npart * sizeof(double) + comp_bytes) {
+        ic->comp6 = (char*)ic->conv->lpt_ws + 6 * npart * sizeof(double);
+        ic->comp6_in_ws = true;
+      } else {
+        size_t fr = 0, tot = 0;
+        MFX_HIP(hipMemGetInfo(&fr, &tot));
+        if (comp_bytes + ((size_t)8 << 30) <= fr) ic->comp6 = c->c.alloc(comp_bytes);
+      }
       const size_t nt = 3 * (p->n / 2) * (p->n / 2) + 1;
       std::vector<double> t(nt);
       amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);
@@ -1511,21 +1531,44 @@ static void icgen_device_run(mfx_icgen* ic) {
   gr.g3c = g3c;
   // positions and velocities (821-931), lunit = vunit = 1; the inverse
   // transform of each component writes its particle array directly and leaves
-  // tmp as scratch
+  // its grid as scratch
+  auto P = [](mfx_grid* g) { return g ? g->d : nullptr; };
+  const double* plt_tab = ic->plt ? ic->plt->tab : nullptr;
+  const int plt_ng = ic->plt ? ic->plt->ng : 0;
+  if (ic->comp6) {
+    const long long ws = (long long)p.n * p.n * (p.n / 2 + 1);
+    if (tmp->prec == MFX_FLOAT) {
+      assemble6<float>(ic->ctx->c, (cx<float>*)ic->comp6, ws, (const cx<float>*)phi->d, (const cx<float>*)P(phi2),
+                       (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
+                       (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, ic->vfact, 1.0, plt_tab, plt_ng,
+                       gr);
+      for (int w = 0; w < 6; ++w)
+        fft_c2r_particles<float>(ic->ctx->c, (float*)((cx<float>*)ic->comp6 + w * ws), (int)p.n,
+                                 (float)norm_fac(tmp), ic->part[w], w % 3, 1.0, w / 3);
+    } else {
+      assemble6<double>(ic->ctx->c, (cx<double>*)ic->comp6, ws, (const cx<double>*)phi->d,
+                        (const cx<double>*)P(phi2), (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]),
+                        (const cx<double>*)P(A3[1]), (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order,
+                        ic->vfact, 1.0, plt_tab, plt_ng, gr);
+      for (int w = 0; w < 6; ++w)
+        fft_c2r_particles<double>(ic->ctx->c, (double*)((cx<double>*)ic->comp6 + w * ws), (int)p.n, norm_fac(tmp),
+                                  ic->part[w], w % 3, 1.0, w / 3);
+    }
+    return;
+  }
   for (int kind = 0; kind < 2; ++kind)
     for (int idim = 0; idim < 3; ++idim) {
       tmp->space = MFX_KSPACE;
-      auto P = [](mfx_grid* g) { return g ? g->d : nullptr; };
       if (tmp->prec == MFX_FLOAT)
         assemble<float>(ic->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (const cx<float>*)P(phi2),
                         (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
                         (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact, 1.0,
-                        ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0, gr);
+                        plt_tab, plt_ng, gr);
       else
         assemble<double>(ic->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (const cx<double>*)P(phi2),
                          (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]), (const cx<double>*)P(A3[1]),
                          (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact,
-                         1.0, ic->plt ? ic->plt->tab : nullptr, ic->plt ? ic->plt->ng : 0, gr);
+                         1.0, plt_tab, plt_ng, gr);
       double* out = ic->part[3 * kind + idim];
       if (tmp->prec == MFX_FLOAT)
         fft_c2r_particles<float>(ic->ctx->c, (float*)tmp->d, (int)p.n, (float)norm_fac(tmp), out, idim, 1.0, kind);

@@ -24,6 +24,11 @@ template <typename T>
 void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
               const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, int kind, int idim, double vfac,
               double unit, const double* plt_tab = nullptr, int plt_ng = 0, Growth gr = Growth{});
+// the six components (kind 0 idim 0..2, kind 1 idim 0..2) into out + w*wstride
+template <typename T>
+void assemble6(Ctx& c, cx<T>* out, long long wstride, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3,
+               const cx<T>* a3x, const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, double vfac,
+               double unit, const double* plt_tab, int plt_ng, Growth gr);
 template <typename T>
 void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind);
 template <typename T>

@@ -94,31 +94,14 @@ __global__ void k_fill(T* f, long long cnt, T v) {
 // The gradient is op::fourier_gradient (operators.hh:38-61, vfac_corr = 1),
 // or with PLT (tables in plt) particle::lattice_gradient, whose growth-rate
 // factor then multiplies the velocities (ENABLE_PLT builds).
+// one component from the growth-scaled potentials s1..s3 and A3 components
+// a3 (each rounded to T as the reference's in-place scaling of a real_t grid
+// stores it), the gradient g and the PLT velocity factor vc
 template <typename T, bool PLT>
-__global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
-                           const cx<T>* a3y, const cx<T>* a3z, int n, double k0, int order, int kind, int idim,
-                           double vfac, double unit_over_L, PltView plt, double boxlen, Growth gr) {
-  const int nh = n / 2 + 1;
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)n * n * nh) return;
-  int k = (int)(idx % nh);
-  long long ij = idx / nh;
-  int j = (int)(ij % n), i = (int)(ij / n);
-  const int ijk[3] = {i, j, k};
-  double g[3], vc = 1.0;
-  if constexpr (PLT) {
-    plt_gradient(plt, n, boxlen, i, j, k, g, vc);
-  } else {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) g[d] = gd_of(ijk[d], n, 1.0) * k0;
-  }
-  // growth-scaled potentials, each rounded to T as the reference's in-place
-  // scaling of a real_t grid stores it
-  auto gs = [](cx<T> v, double g) { return mk(T(v.x * g), T(v.y * g)); };
+__device__ __forceinline__ cx<T> assemble_value(int kind, int idim, int order, cx<T> s1, cx<T> s2, cx<T> s3,
+                                                const cx<T> (&a3)[3], const double (&g)[3], double vc, double vfac,
+                                                double unit_over_L) {
   double pr, pi;
-  const cx<T> s1 = gs(phi[idx], gr.g1);
-  const cx<T> s2 = order > 1 ? gs(phi2[idx], gr.g2) : mk(T(0), T(0));
-  const cx<T> s3 = order > 2 ? gs(phi3[idx], gr.g3) : mk(T(0), T(0));
   if (kind == 0) {
     pr = s1.x;
     pi = s1.y;
@@ -145,8 +128,7 @@ __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, cons
   const int idp = (idim + 1) % 3, idpp = (idim + 2) % 3;
   double tr = -g[idim] * pi, ti = g[idim] * pr;
   if (order > 2) {
-    const cx<T>* A3[3] = {a3x, a3y, a3z};
-    cx<T> a2 = gs(A3[idpp][idx], gr.g3c), a1 = gs(A3[idp][idx], gr.g3c);
+    const cx<T> a2 = a3[idpp], a1 = a3[idp];
     double cr = (-g[idp] * (double)a2.y) - (-g[idpp] * (double)a1.y);
     double ci = (g[idp] * (double)a2.x) - (g[idpp] * (double)a1.x);
     if (kind == 1) {
@@ -162,7 +144,82 @@ __global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, cons
   }
   tr *= unit_over_L;
   ti *= unit_over_L;
-  tmp[idx] = (idx == 0) ? mk(T(0), T(0)) : mk(T(tr), T(ti));
+  return mk(T(tr), T(ti));
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> grow(cx<T> v, double g) {
+  return mk(T(v.x * g), T(v.y * g));
+}
+
+// gradient of mode (i,j,k): Fourier (operators.hh:38-61) or PLT lattice
+template <bool PLT>
+__device__ __forceinline__ void assemble_gradient(const PltView& plt, int n, double boxlen, double k0, int i, int j,
+                                                  int k, double (&g)[3], double& vc) {
+  vc = 1.0;
+  if constexpr (PLT) {
+    plt_gradient(plt, n, boxlen, i, j, k, g, vc);
+  } else {
+    const int ijk[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[d] = gd_of(ijk[d], n, 1.0) * k0;
+  }
+}
+
+template <typename T, bool PLT>
+__global__ void k_assemble(cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3, const cx<T>* a3x,
+                           const cx<T>* a3y, const cx<T>* a3z, int n, double k0, int order, int kind, int idim,
+                           double vfac, double unit_over_L, PltView plt, double boxlen, Growth gr) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  int k = (int)(idx % nh);
+  long long ij = idx / nh;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  double g[3], vc;
+  assemble_gradient<PLT>(plt, n, boxlen, k0, i, j, k, g, vc);
+  const cx<T> s1 = grow(phi[idx], gr.g1);
+  const cx<T> s2 = order > 1 ? grow(phi2[idx], gr.g2) : mk(T(0), T(0));
+  const cx<T> s3 = order > 2 ? grow(phi3[idx], gr.g3) : mk(T(0), T(0));
+  cx<T> a3[3] = {mk(T(0), T(0)), mk(T(0), T(0)), mk(T(0), T(0))};
+  if (order > 2) {
+    const int idp = (idim + 1) % 3, idpp = (idim + 2) % 3;
+    const cx<T>* A3[3] = {a3x, a3y, a3z};
+    a3[idp] = grow(A3[idp][idx], gr.g3c);
+    a3[idpp] = grow(A3[idpp][idx], gr.g3c);
+  }
+  const cx<T> v = assemble_value<T, PLT>(kind, idim, order, s1, s2, s3, a3, g, vc, vfac, unit_over_L);
+  tmp[idx] = (idx == 0) ? mk(T(0), T(0)) : v;
+}
+
+// all six components (positions x,y,z then velocities x,y,z) from one read of
+// the potentials: out + w*wstride is component w
+template <typename T, bool PLT>
+__global__ void k_assemble6(cx<T>* out, long long wstride, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3,
+                            const cx<T>* a3x, const cx<T>* a3y, const cx<T>* a3z, int n, double k0, int order,
+                            double vfac, double unit_over_L, PltView plt, double boxlen, Growth gr) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  int k = (int)(idx % nh);
+  long long ij = idx / nh;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  double g[3], vc;
+  assemble_gradient<PLT>(plt, n, boxlen, k0, i, j, k, g, vc);
+  const cx<T> s1 = grow(phi[idx], gr.g1);
+  const cx<T> s2 = order > 1 ? grow(phi2[idx], gr.g2) : mk(T(0), T(0));
+  const cx<T> s3 = order > 2 ? grow(phi3[idx], gr.g3) : mk(T(0), T(0));
+  cx<T> a3[3] = {mk(T(0), T(0)), mk(T(0), T(0)), mk(T(0), T(0))};
+  if (order > 2) {
+    a3[0] = grow(a3x[idx], gr.g3c);
+    a3[1] = grow(a3y[idx], gr.g3c);
+    a3[2] = grow(a3z[idx], gr.g3c);
+  }
+#pragma unroll
+  for (int w = 0; w < 6; ++w) {
+    const cx<T> v = assemble_value<T, PLT>(w / 3, w % 3, order, s1, s2, s3, a3, g, vc, vfac, unit_over_L);
+    out[w * wstride + idx] = (idx == 0) ? mk(T(0), T(0)) : v;
+  }
 }
 
 // particle_generator.hh:525-541 (sc lattice): pos = q_d * lunit + field, FP64;
@@ -312,6 +369,24 @@ void assemble(Ctx& c, cx<T>* tmp, const cx<T>* phi, const cx<T>* phi2, const cx<
 }
 
 template <typename T>
+void assemble6(Ctx& c, cx<T>* out, long long wstride, const cx<T>* phi, const cx<T>* phi2, const cx<T>* phi3,
+               const cx<T>* a3x, const cx<T>* a3y, const cx<T>* a3z, int n, double boxlen, int order, double vfac,
+               double unit, const double* plt_tab, int plt_ng, Growth gr) {
+  long long cnt = (long long)n * n * (n / 2 + 1);
+  int nin = order == 1 ? 1 : order == 2 ? 2 : 6;
+  LaunchScope ls(c, plt_tab ? "assemble6_plt" : "assemble6", (nin + 6.0) * cnt * sizeof(cx<T>));
+  if (plt_tab)
+    hipLaunchKernelGGL((k_assemble6<T, true>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, out, wstride, phi, phi2,
+                       phi3, a3x, a3y, a3z, n, 2.0 * M_PI / boxlen, order, vfac, unit / boxlen,
+                       PltView{plt_tab, plt_ng}, boxlen, gr);
+  else
+    hipLaunchKernelGGL((k_assemble6<T, false>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, out, wstride, phi, phi2,
+                       phi3, a3x, a3y, a3z, n, 2.0 * M_PI / boxlen, order, vfac, unit / boxlen, PltView{nullptr, 0},
+                       boxlen, gr);
+  check_launch("assemble6");
+}
+
+template <typename T>
 void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind) {
   long long cnt = (long long)n * n * n;
   LaunchScope ls(c, "set_particles", (double)cnt * (sizeof(T) + sizeof(double)));
@@ -387,6 +462,8 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
   template void fill<T>(Ctx&, T*, long long, T);                                                                  \
   template void assemble<T>(Ctx&, cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*,   \
                             const cx<T>*, int, double, int, int, int, double, double, const double*, int, Growth); \
+  template void assemble6<T>(Ctx&, cx<T>*, long long, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*,     \
+                             const cx<T>*, const cx<T>*, int, double, int, double, double, const double*, int, Growth); \
   template void set_particles<T>(Ctx&, const T*, double*, int, int, double, int);                                 \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \

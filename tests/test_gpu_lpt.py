@@ -204,11 +204,13 @@ def test_fused_fp32_vs_reference_sequence_fp32(ctx, orc, n):
             assert rel_rms(a, b) < 1e-5, mode
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_icgen_fused_matches_oracle(ctx, orc, fused):
+@pytest.mark.parametrize("fused,each", [(True, False), (True, True), (False, False)])
+def test_icgen_fused_matches_oracle(ctx, orc, fused, each):
+    """both LPT engines, and the six-component and one-at-a-time assembly"""
     import monofonic_amd as M
     n = 32
-    ic = M.ICGenerator(ctx, n, BOX, 24.0, LPTorder=3, precision="double", generator="MUSIC1", seed=31, fused=fused)
+    ic = M.ICGenerator(ctx, n, BOX, 24.0, LPTorder=3, precision="double", generator="MUSIC1", seed=31, fused=fused,
+                       assemble_each=each)
     ic.run()
     ref, _, _ = oracle_pipeline(orc, n, BOX, 24.0, 3, 31, orc.Cosmology())
     for w in range(6):
