@@ -170,12 +170,12 @@ int mfx_lpt_sources(mfx_conv *cv, mfx_grid *phi, mfx_grid *phi2, mfx_grid *phi3,
 /* 3LPT evaluation path of mfx_lpt_sources (no counterpart in the reference,
  * whose convolve calls have no such choice):
  *   MFX_LPT_STREAMED  every stage transforms the Hessian rows it needs
- *                     (workspace: 3 m*n*(n/2+1) + 16 m*m*(n/2+1) complex
- *                     + 4 n-grids, m = 3n/2);
+ *                     (workspace: 3 m*n*hp + 16 m*m*hp complex, m = 3n/2,
+ *                     hp = n/2+1 rounded up to a multiple of 16);
  *   MFX_LPT_STORED    stage A also stores phi's real-space Hessian rows and
  *                     the phi3 and A3 sources are formed in one pass over them
  *                     (half the padded transforms; workspace 11 instead of 16
- *                     m*m*(n/2+1) slots plus 6 m^3 reals);
+ *                     m*m*hp slots plus 6 m^3 reals);
  *   MFX_LPT_AUTO      stored when its workspace fits in free device memory
  *                     with headroom, streamed otherwise (the default).
  * Both paths give the same fields up to rounding.  The choice is made when the

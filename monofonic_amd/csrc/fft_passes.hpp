@@ -169,18 +169,11 @@ struct DstTruncOp {
     const int i = drop ? 0 : (pos <= nh ? pos : pos - nh);
     return out[(long long)i * sp + (long long)b * sb + col];
   }
-  __device__ __forceinline__ void store(int b, int col, int pos, cx<T> v, cx<T> prev) const {
+  // output value at retained index i of line b, column col
+  __device__ __forceinline__ cx<T> value(int b, int col, int i, cx<T> v, cx<T> prev) const {
     const int nh = n >> 1;
-    int i;
-    if (pos <= nh)
-      i = pos;
-    else if (pos > n)
-      i = pos - nh;
-    else
-      return;
     const bool nyq = (i == nh) | (b == nh) | (col == nh);
     cx<T> val = nyq ? mk(T(0), T(0)) : v * scale;
-    const long long idx = (long long)i * sp + (long long)b * sb + col;
     cx<T> o;
     if (op == OP_ASSIGN)
       o = val;
@@ -195,7 +188,18 @@ struct DstTruncOp {
       T k2 = kx * kx + ky * ky + kz * kz;
       o = (i == 0 && b == 0 && col == 0) ? mk(T(0), T(0)) : mk(-o.x / k2, -o.y / k2);
     }
-    out[idx] = o;
+    return o;
+  }
+  __device__ __forceinline__ void store(int b, int col, int pos, cx<T> v, cx<T> prev) const {
+    const int nh = n >> 1;
+    int i;
+    if (pos <= nh)
+      i = pos;
+    else if (pos > n)
+      i = pos - nh;
+    else
+      return;
+    out[(long long)i * sp + (long long)b * sb + col] = value(b, col, i, v, prev);
   }
 };
 
@@ -240,6 +244,68 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
   if (act) {
 #pragma unroll
     for (int e = 0; e < EPT; ++e) dst.store(b, col, t + TPS * e, v[e], old[e]);
+  }
+}
+
+// The table pointer as a value the compiler cannot see through: a second
+// transform in one kernel then loads its twiddles again rather than keeping
+// the first transform's twiddles in registers across it.
+template <typename T>
+__device__ __forceinline__ const cx<T>* opaque(const cx<T>* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// Forward transform of a line followed by its inverse transform: the x pass
+// of a y-forwarded field into an n-grid (truncation, Nyquist zeroing and the
+// scale fscale, as DstTruncOp with OP_ASSIGN and no inverse Laplacian) and
+// the x pass that pads that field again and transforms it back (SrcPadPow
+// with pw = 0 and scale iscale, DstPlain with scale 1).  The truncated and
+// padded positions coincide (position pos <-> index pos or pos - n/2), so the
+// line stays in registers and the n-grid field never exists.  The output may
+// be the source line itself: every line is read before it is written, by the
+// workgroup that owns it.
+template <typename T>
+struct TurnArgs {
+  const cx<T>* src;
+  long long sb, sp;  // source strides (batch j, line position x)
+  int n;
+  T fscale, iscale;
+  cx<T>* out;
+  long long osb, osp;
+};
+
+template <typename T, int L, int C>
+__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
+    k_col_turn(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols) {
+  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  __shared__ cx<T> lds[L * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  int tile, b;
+  xcd_tile(tile, b);
+  const int col = tile * C + c;
+  const bool act = col < ncols;
+  const int colc = act ? col : ncols - 1;
+  const int nh = a.n >> 1;
+  cx<T> x[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) x[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
+  fft_exec<T, L, false>(x, lds + c, C, t, tw);
+  const bool nyqbc = (b == nh) | (colc == nh);
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int pos = t + TPS * e;
+    // dropped positions, and the Nyquist planes of the n-grid (index n/2 is
+    // position n/2; the batch and column indices are n-grid indices)
+    const bool zero = ((pos > nh) & (pos <= a.n)) | (pos == nh) | nyqbc;
+    const cx<T> o = x[e] * a.fscale;  // the n-grid value
+    x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
+  }
+  __syncthreads();  // the forward transform's exchange reads are done
+  fft_exec<T, L, true>(x, lds + c, C, t, opaque(tw));
+  if (act) {
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(t + TPS * e) * a.osp + col] = x[e];
   }
 }
 
@@ -310,6 +376,41 @@ void launch_col_pad(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const ch
 #define MFX_CASE(LL)                                                    \
   case LL:                                                              \
     launch_col_L<T, LL, INV, Src, Dst>(c, s, d, ncols, nbatch, name, bytes); \
+    return;
+    MFX_PAD_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      throw Error(1, "unsupported padded transform length " + std::to_string(L));
+  }
+}
+
+template <typename T, int L, int C>
+void launch_turn_LC(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_col_turn<T, L, C>), grid, block, 0, c.stream, a, c.twiddles<T>(L), ncols);
+  check_launch(name);
+}
+
+// at the tile width tuned for the inverse transforms of (L, T)
+template <typename T, int L>
+void launch_turn_L(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+  constexpr int CM = ColCfg<T, L>::C;
+  if constexpr (CM >= 2) {
+    if (c.col_tile(L, (int)sizeof(T), true, CM) == CM / 2) {
+      launch_turn_LC<T, L, CM / 2>(c, a, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
+  launch_turn_LC<T, L, CM>(c, a, ncols, nbatch, name, bytes);
+}
+
+template <typename T>
+void launch_turn(Ctx& c, int L, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+  switch (L) {
+#define MFX_CASE(LL)                                           \
+  case LL:                                                     \
+    launch_turn_L<T, LL>(c, a, ncols, nbatch, name, bytes);     \
     return;
     MFX_PAD_LENGTHS(MFX_CASE)
 #undef MFX_CASE

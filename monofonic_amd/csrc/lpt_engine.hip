@@ -504,15 +504,14 @@ static Layout layout(int n) {
 // phi Hessian arrays (m^3 reals each) when stage C reads them (stored)
 static int lpt_slots(int order, bool stored) { return order < 3 ? 7 : (stored ? 11 : 16); }
 
-// 3 x-stage slots (also the y-forward output), the y/z slots, 4 n-grid
-// intermediates and, stored, the real Hessian arrays
+// 3 x-stage slots (also the y-forward output), the y/z slots and, stored,
+// the real Hessian arrays
 size_t lpt_workspace_bytes(int n, int prec, int order, bool stored) {
   Layout l = layout(n);
   const size_t csz = 2 * (size_t)prec;
-  const size_t ngrid = (size_t)n * n * (n / 2 + 1);
   const bool st = stored && order > 2;
   const size_t real = st ? 6 * l.treal : 0;
-  return csz * (3 * l.t1 + lpt_slots(order, st) * l.t2 + (order > 2 ? 4 * ngrid : 0) + real);
+  return csz * (3 * l.t1 + lpt_slots(order, st) * l.t2 + real);
 }
 
 template <typename T>
@@ -522,7 +521,6 @@ struct Engine {
   T kfac, s_in, s_out;
   cx<T>* T1[3];
   cx<T>* T2[16];
-  cx<T>* NI[4];
   cx<T>* R = nullptr;  // stored real phi Hessian arrays (stride l.treal)
   double cs, bN, bT1, bT2, bR;
 
@@ -540,10 +538,6 @@ struct Engine {
       T2[q] = q < n2 ? p : nullptr;
       if (q < n2) p += l.t2;
     }
-    for (auto& q : NI) {
-      q = order < 3 ? nullptr : p;
-      if (order >= 3) p += (size_t)n * n * (n / 2 + 1);
-    }
     if (stored && order > 2) R = p;
     cs = sizeof(cx<T>);
     bN = cs * (double)n * n * l.h;
@@ -557,17 +551,22 @@ struct Engine {
     const int n = l.n, m = l.m, hp = l.hp;
     const long long HN = l.h;
     // x stage: line along i of the n-grid [i][j][kz], batch j, output [x][j][kz];
-    // one launch per power of g_x (each re-reads the n-grid source, which is
-    // cheaper than holding three transforms in registers)
+    // one launch per power of g_x (each re-reads the n-grid source: holding the
+    // source line across several transforms costs more in occupancy)
     const int nx = hessian ? 3 : 1;
     for (int p = 0; p < nx; ++p)
       launch_col_pad<T, true>(c, m, SrcPadPow<T>{F, HN, (long long)n * HN, n, p, s_in, kfac},
-                          DstPlain<T>{T1[p], hp, (long long)n * hp, T(1)}, l.h, n, "lpt_x_inv", bN + bT1);
-    // y stage: line along j of T1 [x][j][kz], batch x, output [x][y][kz]
+                              DstPlain<T>{T1[p], hp, (long long)n * hp, T(1)}, l.h, n, "lpt_x_inv", bN + bT1);
+    y_inverse(hessian, Y);
+  }
+
+  // y stage: line along j of T1 [x][j][kz], batch x, output [x][y][kz]
+  void y_inverse(bool hessian, cx<T>* const* Y) {
+    const int n = l.n, m = l.m, hp = l.hp;
     auto ystage = [&](int src, std::initializer_list<std::pair<int, int>> outs) {
       for (auto& o : outs)
         launch_col_pad<T, true>(c, m, SrcPadPow<T>{T1[src], (long long)n * hp, hp, n, o.second, T(1), kfac},
-                            DstPlain<T>{Y[o.first], (long long)m * hp, hp, T(1)}, l.h, m, "lpt_y_inv", bT1 + bT2);
+                                DstPlain<T>{Y[o.first], (long long)m * hp, hp, T(1)}, l.h, m, "lpt_y_inv", bT1 + bT2);
     };
     if (hessian) {
       ystage(0, {{0, 0}, {2, 1}, {5, 2}});  // Y0, Yy, Yyy
@@ -578,14 +577,31 @@ struct Engine {
     }
   }
 
+  // y forward of a z-stage output into T1[0] (the n retained rows)
+  void y_forward(cx<T>* Z) {
+    const int n = l.n, m = l.m, hp = l.hp;
+    launch_col_pad<T, false>(c, m, SrcPlain<T>{Z, (long long)m * hp, hp}, DstTrunc<T>{T1[0], (long long)n * hp, hp, n},
+                             l.h, m, "lpt_y_fwd", bT2 + bT1);
+  }
+
   // y and x forward transforms of a z-stage output into an n-grid
   void forward(cx<T>* Z, cx<T>* out, bool invlap) {
     const int n = l.n, m = l.m, hp = l.hp;
     const long long HN = l.h;
-    launch_col_pad<T, false>(c, m, SrcPlain<T>{Z, (long long)m * hp, hp}, DstTrunc<T>{T1[0], (long long)n * hp, hp, n},
-                         l.h, m, "lpt_y_fwd", bT2 + bT1);
+    y_forward(Z);
     DstTruncOp<T> d{out, HN, (long long)n * HN, n, OP_ASSIGN, T(0), s_out, invlap ? 1 : 0, kfac};
     launch_col_pad<T, false>(c, m, SrcPlain<T>{T1[0], hp, (long long)n * hp}, d, l.h, n, "lpt_x_fwd", bT1 + bN);
+  }
+
+  // forward transforms of a z-stage output and, from the truncated field,
+  // the x and y stages again (one plain row Y[0]); the n-grid field is never
+  // stored: the 3a intermediates of OrszagConvolver::convolve3
+  void turn(cx<T>* Z, cx<T>* const* Y) {
+    const int n = l.n, hp = l.hp;
+    y_forward(Z);
+    TurnArgs<T> a{T1[0], hp, (long long)n * hp, n, s_out, s_in, T1[0], hp, (long long)n * hp};
+    launch_turn<T>(c, l.m, a, l.h, n, "lpt_x_turn", 2 * bT1);
+    y_inverse(false, Y);
   }
 
   FieldY<T> fy(int base) const {
@@ -618,18 +634,25 @@ void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x,
     MFX_LPT_KSWITCH(K, MFX_CASE)
 #undef MFX_CASE
   }
-  E.forward(E.T2[6], phi2, true);  // phi2 = -S2/k^2 (ic_generator.cc:488)
-  if (!three) return;
-  for (int q = 0; q < 4; ++q) E.forward(Iz[q], E.NI[q], false);
+  if (!three) {
+    E.forward(E.T2[6], phi2, true);  // phi2 = -S2/k^2 (ic_generator.cc:488)
+    return;
+  }
+  // the 3a intermediates go forward to the n-grid (truncation) and straight
+  // back to their rows
+  auto turn_I = [&](int q, cx<T>* row) {
+    cx<T>* Y[6] = {row, nullptr, nullptr, nullptr, nullptr, nullptr};
+    E.turn(Iz[q], Y);
+  };
   if (E.R) {
     // stored: phi's Hessian rows are dead after stage A, so phi2's take
-    // T2[0..5] and the intermediates' rows T2[6..9]
+    // T2[0..5], then intermediate q's row moves from T2[7+q] to T2[6+q]
+    E.forward(E.T2[6], phi2, true);  // phi2 = -S2/k^2 (ic_generator.cc:488)
     E.inverse(phi2, true, E.T2);
     cx<T>* Ir[4];
     for (int q = 0; q < 4; ++q) {
       Ir[q] = E.T2[6 + q];
-      cx<T>* Y[6] = {Ir[q], nullptr, nullptr, nullptr, nullptr, nullptr};
-      E.inverse(E.NI[q], false, Y);
+      turn_I(q, Ir[q]);
     }
     const double bytes = 10 * E.bT2 + 6 * E.bR + 4 * E.bT2;
 #define MFX_CASE(KK)                                                                                   \
@@ -642,15 +665,16 @@ void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x,
     for (int q = 0; q < 4; ++q) E.forward(Ir[q], out[q], true);
     return;
   }
-  // phi2 Hessian rows -> T2[6..11]; I1..I4 rows -> T2[12..15]
-  E.inverse(phi2, true, E.T2 + 6);
-  const FieldY<T> fphi2 = E.fy(6);
+  // streamed: I1..I4 rows -> T2[12..15] first, then phi2's Hessian rows over
+  // S2 and the intermediates' z rows in T2[6..11]
   cx<T>* Ir[4];
   for (int q = 0; q < 4; ++q) {
     Ir[q] = E.T2[12 + q];
-    cx<T>* Y[6] = {Ir[q], nullptr, nullptr, nullptr, nullptr, nullptr};
-    E.inverse(E.NI[q], false, Y);
+    turn_I(q, Ir[q]);
   }
+  E.forward(E.T2[6], phi2, true);
+  E.inverse(phi2, true, E.T2 + 6);
+  const FieldY<T> fphi2 = E.fy(6);
   // stage C1: phi3 source over I1's rows
   {
     const double bytes = 16 * E.bT2 + E.bT2;
