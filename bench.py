@@ -8,7 +8,8 @@ Orszag convolutions (768^3 padded) run in FP32; amplitude table, growth
 factors and the position/velocity assembly run in FP64.
 
 A step is one whole realisation on the device: noise fill -> r2c -> phi(1) ->
-phi(2), phi(3a), phi(3b), A(3) -> growth scaling -> 6 c2r transforms -> FP64
+phi(2), phi(3a), phi(3b), A(3) -> assembly of the six growth-scaled
+displacement and velocity components -> 6 c2r transforms writing the FP64
 positions and velocities, all resident in HBM.  value = particles/s = 512^3 /
 (time per step).
 
@@ -268,6 +269,7 @@ def main():
         ("" if args.generator == "MUSIC1" else "-" + args.generator.lower())
     tr = profiled_traffic(roof["kernel"], workload_id)
     roof["traffic"] = round(tr) if tr else None
+    lpt_path = "stored" if "lpt_zC" in stats else ("streamed" if "lpt_zC1" in stats else None)
 
     fp64 = None
     acc = None
@@ -302,7 +304,8 @@ def main():
                                f"Eisenstein-Hu, FP32 bulk ({3 * n // 2}^3 padded) + FP64 tables/assembly"
                                + (", PLT corrections" if args.plt else ""),
                    "n": n, "lpt_order": 3, "padded": 3 * n // 2, "fused_lpt": bool(args.fused),
-                   "generator": args.generator, "plt": bool(args.plt), "wall_s": round(wall, 4)},
+                   "lpt_path": lpt_path, "generator": args.generator, "plt": bool(args.plt),
+                   "column_tiles": ctx.autotune_report().strip().splitlines(), "wall_s": round(wall, 4)},
         "roofline": roof, "fft_gbps": round(fft_gbps, 1) if fft_gbps else None,
         "fft_frac": round(fft_gbps / HBM_PEAK_GBPS, 4) if fft_gbps else None,
         "fp64": fp64, "accuracy": acc, "cpu_baseline": cpu,
