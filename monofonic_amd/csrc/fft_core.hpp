@@ -112,6 +112,36 @@ struct BF<8, INV, T> {
   }
 };
 
+// radix 12 as 4 x 3: x[3 n1 + n2] -> 4-point DFTs over n1, twiddles
+// W12^(n2 k1), 3-point DFTs over n2; output k = k1 + 4 k2
+template <bool INV, typename T>
+struct BF<12, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    const T h = T(0.5), s = T(0.866025403784438646763723170752936183);
+    cx<T> t[12];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      cx<T> u[4] = {a[n2], a[3 + n2], a[6 + n2], a[9 + n2]};
+      BF<4, INV, T>::run(u);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) t[n2 * 4 + k1] = u[k1];
+    }
+    t[5] = twmul<INV>(t[5], mk(s, -h));    // W12^1
+    t[6] = twmul<INV>(t[6], mk(h, -s));    // W12^2
+    t[7] = mul_mi<INV>(t[7]);              // W12^3 = -i
+    t[9] = twmul<INV>(t[9], mk(h, -s));    // W12^2
+    t[10] = twmul<INV>(t[10], mk(-h, -s)); // W12^4
+    t[11] = mk(-t[11].x, -t[11].y);        // W12^6 = -1
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cx<T> u[3] = {t[k1], t[4 + k1], t[8 + k1]};
+      BF<3, INV, T>::run(u);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) a[k1 + 4 * k2] = u[k2];
+    }
+  }
+};
+
 // A twiddle table in global memory, typed with the global address space so
 // that its reads are global (not flat) loads: a flat load counts against the
 // LDS counter too, and every LDS wait of the transform would then also wait
@@ -149,25 +179,25 @@ struct Plan;
   };
 
 MFX_PLAN(8, 8, 8)
-MFX_PLAN(12, 12, 3, 4)
+MFX_PLAN(12, 12, 12)
 MFX_PLAN(16, 4, 4, 4)
-MFX_PLAN(24, 12, 3, 2, 4)
+MFX_PLAN(24, 12, 12, 2)
 MFX_PLAN(32, 8, 4, 8)
-MFX_PLAN(36, 12, 3, 3, 4)
-MFX_PLAN(48, 12, 3, 4, 4)
+MFX_PLAN(36, 12, 12, 3)
+MFX_PLAN(48, 12, 12, 4)
 MFX_PLAN(64, 8, 8, 8)
-MFX_PLAN(72, 12, 3, 3, 2, 4)
-MFX_PLAN(96, 12, 3, 2, 4, 4)
+MFX_PLAN(72, 12, 12, 3, 2)
+MFX_PLAN(96, 12, 12, 4, 2)
 MFX_PLAN(128, 8, 8, 4, 4)
-MFX_PLAN(144, 12, 3, 3, 4, 4)
-MFX_PLAN(192, 12, 3, 4, 4, 4)
+MFX_PLAN(144, 12, 12, 3, 4)
+MFX_PLAN(192, 12, 12, 4, 4)
 MFX_PLAN(256, 8, 8, 8, 4)
-MFX_PLAN(288, 12, 3, 3, 2, 4, 4)
-MFX_PLAN(384, 12, 3, 2, 4, 4, 4)
+MFX_PLAN(288, 12, 12, 3, 2, 4)
+MFX_PLAN(384, 12, 12, 4, 4, 2)
 MFX_PLAN(512, 8, 8, 8, 8)
-MFX_PLAN(768, 12, 3, 4, 4, 4, 4)
+MFX_PLAN(768, 12, 12, 4, 4, 4)
 MFX_PLAN(1024, 8, 8, 8, 4, 4)
-MFX_PLAN(1536, 12, 3, 2, 4, 4, 4, 4)
+MFX_PLAN(1536, 12, 12, 4, 4, 4, 2)
 #undef MFX_PLAN
 
 template <int L>
@@ -204,7 +234,10 @@ __device__ __forceinline__ void stage_twiddles(cx<T> (&w)[Plan<L>::EPT], gcx<T>*
 
 // one Stockham stage: radix R, NS = product of the radices already applied,
 // w = this stage's twiddles (stage_twiddles)
-template <typename T, int L, int EPT, int R, int NS, bool INV>
+// The last stage (NS = L/R) needs no exchange: butterfly j = t + TPS*m has
+// its outputs at j + q*L/R = t + TPS*(m + q*EPT/R), this thread's own slots
+// m + q*NB in natural order.
+template <typename T, int L, int EPT, int R, int NS, bool INV, bool LAST>
 __device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ lds, int ls, int t,
                                           const cx<T> (&w)[EPT]) {
   constexpr int TPS = L / EPT;
@@ -222,14 +255,21 @@ __device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ l
       for (int q = 1; q < R; ++q) a[q] = twmul<INV>(a[q], w[m * (R - 1) + q - 1]);
     }
     BF<R, INV, T>::run(a);
-    const int base = (j / NS) * NS * R + k;
+    if constexpr (LAST) {
 #pragma unroll
-    for (int q = 0; q < R; ++q) lds[(base + q * NS) * ls] = a[q];
+      for (int q = 0; q < R; ++q) v[m + q * NB] = a[q];
+    } else {
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) lds[(base + q * NS) * ls] = a[q];
+    }
   }
-  __syncthreads();
+  if constexpr (!LAST) {
+    __syncthreads();
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) v[e] = lds[(t + TPS * e) * ls];
-  __syncthreads();
+    for (int e = 0; e < EPT; ++e) v[e] = lds[(t + TPS * e) * ls];
+    __syncthreads();
+  }
 }
 
 // Stage S runs with its twiddles already in registers (wcur) and first issues
@@ -242,7 +282,7 @@ __device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds,
     constexpr int R = Plan<L>::R[S];
     cx<T> wnext[Plan<L>::EPT];
     stage_twiddles<T, L, S + 1>(wnext, tw, t);
-    fft_stage<T, L, Plan<L>::EPT, R, StageNS<L, S>::value, INV>(v, lds, ls, t, wcur);
+    fft_stage<T, L, Plan<L>::EPT, R, StageNS<L, S>::value, INV, S == Plan<L>::NR - 1>(v, lds, ls, t, wcur);
     stage_loop<T, L, INV, S + 1>(v, lds, ls, t, tw, wnext);
   }
 }
