@@ -289,9 +289,7 @@ struct RowCtx {
       const int k = lane + P * e;
       cx<T> z = mk(T(0), T(0));
       if (k < K) {
-        // X[k] read back rather than recomputed: the operands of val die at
-        // the barrier
-        cx<T> Ak = B[k];
+        cx<T> Ak = val(e, k);
         cx<T> Bk = conj(B[K - k]);
         if (k == 0) {
           Ak.y = T(0);
