@@ -503,11 +503,12 @@ int mfx_noise_normalise(mfx_grid* wn, double volfac, int fixing, int inversion) 
   });
 }
 
-static void phi1_impl(mfx_grid* phi, const mfx_grid* wn, const double* amp_dev) {
+static void phi1_impl(mfx_grid* phi, const mfx_grid* wn, const double* amp_dev, NoiseNorm nn = NoiseNorm{}) {
   if (phi->prec == MFX_FLOAT)
-    phi1<float>(phi->ctx->c, (cx<float>*)phi->d, (const cx<float>*)wn->d, (int)phi->n[0], phi->L[0], amp_dev);
+    phi1<float>(phi->ctx->c, (cx<float>*)phi->d, (const cx<float>*)wn->d, (int)phi->n[0], phi->L[0], amp_dev, nn);
   else
-    phi1<double>(phi->ctx->c, (cx<double>*)phi->d, (const cx<double>*)wn->d, (int)phi->n[0], phi->L[0], amp_dev);
+    phi1<double>(phi->ctx->c, (cx<double>*)phi->d, (const cx<double>*)wn->d, (int)phi->n[0], phi->L[0], amp_dev,
+                 nn);
 }
 
 int mfx_phi1(mfx_grid* phi, const mfx_grid* wn, const double* amp_table, size_t ntable) {
@@ -1508,13 +1509,10 @@ static void icgen_device_run(mfx_icgen* ic) {
   else
     threefry_dev(wn, (const unsigned long long*)ic->dev_seed);
   grid_forward(wn, true);
-  // normalisation (341-346) and phi(1) (450-457)
-  if (wn->prec == MFX_FLOAT)
-    noise_normalise<float>(ic->ctx->c, (cx<float>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
-  else
-    noise_normalise<double>(ic->ctx->c, (cx<double>*)wn->d, (int)p.n, volfac, p.fixing, p.inversion);
+  // normalisation (341-346) applied as phi(1) (450-457) reads the noise; the
+  // noise grid keeps the unnormalised field
   phi->space = MFX_KSPACE;
-  phi1_impl(phi, wn, ic->amp);
+  phi1_impl(phi, wn, ic->amp, noise_norm_spec(volfac, p.fixing, p.inversion));
   // phi(2), phi(3), A(3)
   if (p.lpt_order > 1) {
     if (p.use_fused_lpt)

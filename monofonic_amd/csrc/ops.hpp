@@ -7,8 +7,24 @@ namespace mfx {
 
 template <typename T>
 void noise_normalise(Ctx& c, cx<T>* wn, int n, double volfac, int fixing, int inversion);
+// white-noise normalisation of ic_generator.cc:341-346 (1/volfac, amplitude
+// fixing, phase inversion); on = 0 leaves the noise as it is
+struct NoiseNorm {
+  int on = 0;
+  double ivolfac = 1;
+  int fixing = 0;
+  double sgn = 1;
+};
+inline NoiseNorm noise_norm_spec(double volfac, int fixing, int inversion) {
+  NoiseNorm nn;
+  nn.on = 1;
+  nn.ivolfac = 1.0 / volfac;
+  nn.fixing = fixing;
+  nn.sgn = inversion ? -1.0 : 1.0;
+  return nn;
+}
 template <typename T>
-void phi1(Ctx& c, cx<T>* phi, const cx<T>* wn, int n, double boxlen, const double* amp);
+void phi1(Ctx& c, cx<T>* phi, const cx<T>* wn, int n, double boxlen, const double* amp, NoiseNorm nn = NoiseNorm{});
 template <typename T>
 void inverse_laplacian(Ctx& c, cx<T>* f, int n, double boxlen);
 template <typename T>

@@ -24,36 +24,45 @@ __device__ __forceinline__ double gd_of(int i, int n, double kfac) {
 // ------------------------------------------------------------------ noise normalisation
 // ic_generator.cc:341-346: wn <- (inv ? -1 : 1) * [fixing ? wn/|wn| : wn] / volfac
 template <typename T>
-__global__ void k_noise_norm(cx<T>* wn, long long cnt, double ivolfac, int fixing, double sgn) {
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= cnt) return;
-  cx<T> v = wn[idx];
+__device__ __forceinline__ cx<T> norm_noise(cx<T> v, const NoiseNorm& nn) {
   double re = v.x, im = v.y;
-  if (fixing) {
+  if (nn.fixing) {
     double a = hypot(re, im);
     if (a != 0.0) {
       re /= a;
       im /= a;
     }
   }
-  wn[idx] = mk(T(sgn * re * ivolfac), T(sgn * im * ivolfac));
+  return mk(T(nn.sgn * re * nn.ivolfac), T(nn.sgn * im * nn.ivolfac));
+}
+
+template <typename T>
+__global__ void k_noise_norm(cx<T>* wn, long long cnt, NoiseNorm nn) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cnt) return;
+  wn[idx] = norm_noise(wn[idx], nn);
 }
 
 // ic_generator.cc:450-457: phi = -wn * A(|k|) / |k|^2, zero_DC_mode.
-// A is tabulated (FP64) on the integer |n|^2 = ix^2+iy^2+iz^2.
-template <typename T>
-__global__ void k_phi1(cx<T>* phi, const cx<T>* wn, int n, double kfac, const double* __restrict__ amp) {
-  const int nh = n / 2 + 1;
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)n * n * nh) return;
+// A is tabulated (FP64) on the integer |n|^2 = ix^2+iy^2+iz^2.  With nn.on
+// the noise is normalised first (341-346), rounded to T as the normalised
+// grid would store it.  I is the index type: 32-bit where the grid allows,
+// whose division is far cheaper than the 64-bit one.
+template <typename T, typename I>
+__global__ void k_phi1(cx<T>* phi, const cx<T>* wn, int n, double kfac, const double* __restrict__ amp,
+                       NoiseNorm nn) {
+  const I nh = (I)(n / 2 + 1), nn_ = (I)n;
+  const I idx = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+  if (idx >= nn_ * nn_ * nh) return;
   int k = (int)(idx % nh);
-  long long ij = idx / nh;
-  int j = (int)(ij % n), i = (int)(ij / n);
+  I ij = idx / nh;
+  int j = (int)(ij % nn_), i = (int)(ij / nn_);
   int ix = i - (i > n / 2) * n, iy = j - (j > n / 2) * n, iz = k - (k > n / 2) * n;
   double kx = kd_of(i, n, kfac), ky = kd_of(j, n, kfac), kz = kd_of(k, n, kfac);
   double k2 = kx * kx + ky * ky + kz * kz;
   long long n2 = (long long)ix * ix + (long long)iy * iy + (long long)iz * iz;
   cx<T> w = wn[idx];
+  if (nn.on) w = norm_noise(w, nn);
   double f = (idx == 0) ? 0.0 : amp[n2] / k2;
   phi[idx] = mk(T(-(double)w.x * f), T(-(double)w.y * f));
 }
@@ -315,16 +324,21 @@ template <typename T>
 void noise_normalise(Ctx& c, cx<T>* wn, int n, double volfac, int fixing, int inversion) {
   long long cnt = (long long)n * n * (n / 2 + 1);
   LaunchScope ls(c, "noise_normalise", 2.0 * cnt * sizeof(cx<T>));
-  hipLaunchKernelGGL(k_noise_norm<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, wn, cnt, 1.0 / volfac, fixing,
-                     inversion ? -1.0 : 1.0);
+  hipLaunchKernelGGL(k_noise_norm<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, wn, cnt,
+                     noise_norm_spec(volfac, fixing, inversion));
   check_launch("noise_normalise");
 }
 
 template <typename T>
-void phi1(Ctx& c, cx<T>* phi, const cx<T>* wn, int n, double boxlen, const double* amp) {
+void phi1(Ctx& c, cx<T>* phi, const cx<T>* wn, int n, double boxlen, const double* amp, NoiseNorm nn) {
   long long cnt = (long long)n * n * (n / 2 + 1);
-  LaunchScope ls(c, "phi1", 2.0 * cnt * sizeof(cx<T>));
-  hipLaunchKernelGGL(k_phi1<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, phi, wn, n, 2.0 * M_PI / boxlen, amp);
+  LaunchScope ls(c, nn.on ? "phi1_norm" : "phi1", 2.0 * cnt * sizeof(cx<T>));
+  if (cnt < (1LL << 32))
+    hipLaunchKernelGGL((k_phi1<T, unsigned>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, phi, wn, n,
+                       2.0 * M_PI / boxlen, amp, nn);
+  else
+    hipLaunchKernelGGL((k_phi1<T, long long>), dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, phi, wn, n,
+                       2.0 * M_PI / boxlen, amp, nn);
   check_launch("phi1");
 }
 
@@ -456,7 +470,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
 
 #define MFX_INST(T)                                                                                               \
   template void noise_normalise<T>(Ctx&, cx<T>*, int, double, int, int);                                          \
-  template void phi1<T>(Ctx&, cx<T>*, const cx<T>*, int, double, const double*);                                  \
+  template void phi1<T>(Ctx&, cx<T>*, const cx<T>*, int, double, const double*, NoiseNorm);                       \
   template void inverse_laplacian<T>(Ctx&, cx<T>*, int, double);                                                  \
   template void scale<T>(Ctx&, T*, long long, double);                                                            \
   template void fill<T>(Ctx&, T*, long long, T);                                                                  \

@@ -122,28 +122,38 @@ __device__ __forceinline__ bool leva(double U1, double U2, double* out) {
   return !reject;
 }
 
-// exclusive scan over 256 threads (one value each), returns total
-__device__ int block_scan(int v, int* sh, int* total) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    int t = (tid >= off) ? sh[tid - off] : 0;
-    __syncthreads();
-    sh[tid] += t;
-    __syncthreads();
+// Ranks of the accepted pairs of one twist in stream order p = 0..311 (pair p
+// = tid + 256 r): a ballot per wave gives the rank within the wave, and the
+// per-wave counts, exchanged through LDS, the offsets of the waves.  The
+// caller synchronises before `cnt` is written again.
+__device__ __forceinline__ void accept_ranks(bool ok0, bool ok1, int* cnt, int& off0, int& off1, int& total0,
+                                             int& total1) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  const unsigned long long b0 = __ballot(ok0), b1 = __ballot(ok1);
+  if (lane == 0) {
+    cnt[w] = __popcll(b0);
+    cnt[4 + w] = __popcll(b1);
   }
-  int incl = sh[tid];
-  *total = sh[255];
   __syncthreads();
-  return incl - v;
+  int before0 = 0, before1 = 0;
+  total0 = total1 = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    before0 += q < w ? cnt[q] : 0;
+    before1 += q < w ? cnt[4 + q] : 0;
+    total0 += cnt[q];
+    total1 += cnt[4 + q];
+  }
+  off0 = before0 + __popcll(b0 & lt);
+  off1 = before1 + __popcll(b1 & lt);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, const long* seedp, dd* part) {
   __shared__ unsigned mt[MT_N];
   const long baseseed = *seedp;
-  __shared__ int scan[256];
+  __shared__ int wcnt[8];
   __shared__ dd red[256];
   const int tid = threadIdx.x;
   const long long icube = blockIdx.x;
@@ -170,9 +180,8 @@ __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, in
       }
     }
     // stream order is p = 0..311: first half pairs (r = 0) precede r = 1
-    int total0, total1;
-    int off0 = block_scan(ok[0] ? 1 : 0, scan, &total0);
-    int off1 = block_scan(ok[1] ? 1 : 0, scan, &total1);
+    int total0, total1, off0, off1;
+    accept_ranks(ok[0], ok[1], wcnt, off0, off1, total0, total1);
     nacc = total0 + total1;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
@@ -214,11 +223,12 @@ __global__ void k_dd_mean(const dd* part, long long nparts, double inv_count, do
   }
 }
 
-template <typename T>
+// I: the index type, 32-bit where the grid allows (cheaper division)
+template <typename T, typename I>
 __global__ void k_sub_mean(T* data, int n, const double* mean, int zeromean) {
-  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long pitch = n + 2;
-  if (p >= (long long)n * n * pitch) return;
+  const I p = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+  const I pitch = (I)(n + 2);
+  if (p >= (I)n * (I)n * pitch) return;
   int k = (int)(p % pitch);
   if (k >= n) {
     data[p] = T(0);
@@ -267,8 +277,12 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
   {
     const long long tot = (long long)n * n * (n + 2);
     LaunchScope ls(c, "music1_submean", 2.0 * tot * sizeof(T));
-    hipLaunchKernelGGL(k_sub_mean<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, data, n, mean,
-                       zeromean);
+    if (tot < (1LL << 32))
+      hipLaunchKernelGGL((k_sub_mean<T, unsigned>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, data,
+                         n, mean, zeromean);
+    else
+      hipLaunchKernelGGL((k_sub_mean<T, long long>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream,
+                         data, n, mean, zeromean);
     check_launch("music1_submean");
   }
 }
