@@ -256,19 +256,19 @@ __device__ __forceinline__ const cx<T>* opaque(const cx<T>* p) {
   return p;
 }
 
-// Forward transform of a line followed by its inverse transform: the x pass
-// of a y-forwarded field into an n-grid (truncation, Nyquist zeroing and the
-// scale fscale, as DstTruncOp with OP_ASSIGN and no inverse Laplacian) and
-// the x pass that pads that field again and transforms it back (SrcPadPow
-// with pw = 0 and scale iscale, DstPlain with scale 1).  The truncated and
-// padded positions coincide (position pos <-> index pos or pos - n/2), so the
-// line stays in registers and the n-grid field never exists.  The output may
-// be the source line itself: every line is read before it is written, by the
+// Projection of a padded line onto the modes of the n-grid: forward
+// transform, truncation to the n kept modes with the Nyquist mode and the
+// Nyquist column (kz = n/2) zeroed and the scale fscale (the truncating pass
+// of the Orszag chain, unpad), then padding with the scale iscale and the
+// inverse transform (pad_insert).  The truncated and padded positions
+// coincide (position pos <-> index pos or pos - n/2), so the line stays in
+// registers and the truncated field never exists.  The output may be the
+// source line itself: every line is read before it is written, by the
 // workgroup that owns it.
 template <typename T>
 struct TurnArgs {
   const cx<T>* src;
-  long long sb, sp;  // source strides (batch j, line position x)
+  long long sb, sp;  // source strides (batch, line position)
   int n;
   T fscale, iscale;
   cx<T>* out;
@@ -291,13 +291,12 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
 #pragma unroll
   for (int e = 0; e < EPT; ++e) x[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
   fft_exec<T, L, false>(x, lds + c, C, t, tw);
-  const bool nyqbc = (b == nh) | (colc == nh);
+  const bool nyqc = colc == nh;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     const int pos = t + TPS * e;
-    // dropped positions, and the Nyquist planes of the n-grid (index n/2 is
-    // position n/2; the batch and column indices are n-grid indices)
-    const bool zero = ((pos > nh) & (pos <= a.n)) | (pos == nh) | nyqbc;
+    // dropped positions and the Nyquist mode (index n/2 is position n/2)
+    const bool zero = ((pos > nh) & (pos <= a.n)) | (pos == nh) | nyqc;
     const cx<T> o = x[e] * a.fscale;  // the n-grid value
     x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
   }

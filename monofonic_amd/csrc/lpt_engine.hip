@@ -593,15 +593,18 @@ struct Engine {
     launch_col_pad<T, false>(c, m, SrcPlain<T>{T1[0], hp, (long long)n * hp}, d, l.h, n, "lpt_x_fwd", bT1 + bN);
   }
 
-  // forward transforms of a z-stage output and, from the truncated field,
-  // the x and y stages again (one plain row Y[0]); the n-grid field is never
-  // stored: the 3a intermediates of OrszagConvolver::convolve3
-  void turn(cx<T>* Z, cx<T>* const* Y) {
-    const int n = l.n, hp = l.hp;
-    y_forward(Z);
-    TurnArgs<T> a{T1[0], hp, (long long)n * hp, n, s_out, s_in, T1[0], hp, (long long)n * hp};
-    launch_turn<T>(c, l.m, a, l.h, n, "lpt_x_turn", 2 * bT1);
-    y_inverse(false, Y);
+  // the 3a intermediates of OrszagConvolver::convolve3: a z-stage output
+  // truncated to the n-grid and padded back, without the n-grid field.  The
+  // truncation (with its Nyquist zeroing) is a separable projection, so it
+  // runs as one y pass over the M x-planes of the slot (Z into Y) and one x
+  // pass over its M y rows (Y in place, with the Orszag scales sqrt(n^3)/M^3
+  // and 1/sqrt(n^3)).
+  void turn(cx<T>* Z, cx<T>* Y) {
+    const int n = l.n, m = l.m, hp = l.hp;
+    TurnArgs<T> ay{Z, (long long)m * hp, hp, n, T(1), T(1), Y, (long long)m * hp, hp};
+    launch_turn<T>(c, m, ay, l.h, m, "lpt_y_turn", 2 * bT2);
+    TurnArgs<T> ax{Y, hp, (long long)m * hp, n, s_out, s_in, Y, hp, (long long)m * hp};
+    launch_turn<T>(c, m, ax, l.h, m, "lpt_x_turn", 2 * bT2);
   }
 
   FieldY<T> fy(int base) const {
@@ -640,10 +643,7 @@ void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x,
   }
   // the 3a intermediates go forward to the n-grid (truncation) and straight
   // back to their rows
-  auto turn_I = [&](int q, cx<T>* row) {
-    cx<T>* Y[6] = {row, nullptr, nullptr, nullptr, nullptr, nullptr};
-    E.turn(Iz[q], Y);
-  };
+  auto turn_I = [&](int q, cx<T>* row) { E.turn(Iz[q], row); };
   if (E.R) {
     // stored: phi's Hessian rows are dead after stage A, so phi2's take
     // T2[0..5], then intermediate q's row moves from T2[7+q] to T2[6+q]
