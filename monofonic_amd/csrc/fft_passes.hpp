@@ -419,11 +419,13 @@ void launch_turn(Ctx& c, int L, const TurnArgs<T>& a, int ncols, int nbatch, con
 }
 
 // ------------------------------------------------------------------ row kernels (z axis)
-// One workgroup = one wavefront holding RowLanes<K>::RPW rows; the row
+// One workgroup = one wavefront holding RowLanes<K>::RPW rows (two wavefronts
+// holding one row for K >= 768); the row
 // transforms are the LDS-resident ones of row_fft.hpp.
 template <typename T, int K>
 struct RowBlk {
   static constexpr int P = RowLanes<K>::P, RPW = RowLanes<K>::RPW, LB = RowLanes<K>::LB;
+  static constexpr int THREADS = RowLanes<K>::THREADS;
   static constexpr int LDS = RPW * 2 * LB;  // complex elements of LDS per workgroup
 };
 
@@ -435,7 +437,7 @@ struct RowBlk {
   constexpr int P_ = RowBlk<T, K>::P, LB_ = RowBlk<T, K>::LB;                       \
   __shared__ cx<T> lds_[RowBlk<T, K>::LDS];                                         \
   __shared__ cx<T> tw_[2 * (K)];                                                    \
-  for (int i_ = threadIdx.x; i_ < (K); i_ += 64) {                                  \
+  for (int i_ = threadIdx.x; i_ < (K); i_ += RowBlk<T, K>::THREADS) {             \
     tw_[i_] = twK[i_];                                                              \
     tw_[(K) + i_] = twM[i_];                                                        \
   }                                                                                 \
@@ -447,7 +449,7 @@ struct RowBlk {
   RowCtx<T, K> rc{lds_ + r_ * 2 * LB_, lds_ + r_ * 2 * LB_ + LB_, lane, tw_, tw_ + (K)};
 
 template <typename T, int K>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_row_r2c(const T* __restrict__ src, long long spitch, cx<T>* dst, long long dpitch, int hout,
               long long nrows, T scale, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
@@ -460,7 +462,7 @@ __global__ void __launch_bounds__(64)
 }
 
 template <typename T, int K>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_row_c2r(const cx<T>* src, long long spitch, int hin, T* dst, long long dpitch, long long nrows, T scale,
               const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
@@ -484,7 +486,7 @@ __global__ void __launch_bounds__(64)
 // to double and, for positions (kind 0), added to the lattice coordinate
 // q_d = i_d/n * lunit.  Row r = i*n + j holds particles r*n .. r*n + n-1.
 template <typename T, int K>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_row_c2r_part(const cx<T>* src, long long spitch, int hin, double* out, long long nrows, T scale, int idim,
                    double lunit, int kind, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
@@ -515,7 +517,7 @@ __global__ void __launch_bounds__(64)
 
 // P3: c2r of two truncated half spectra, pointwise product, r2c, truncation
 template <typename T, int K>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_row_prod(const cx<T>* a, const cx<T>* b, long long spitch, int hin, cx<T>* dst, long long dpitch,
                int hout, long long nrows, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
@@ -539,7 +541,7 @@ void launch_row_r2c_K(Ctx& c, const T* src, long long spitch, cx<T>* dst, long l
                       long long nrows, T scale, const char* name, double bytes) {
   const dim3 grid = row_grid<T, K>(nrows);
   LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_row_r2c<T, K>), grid, dim3(64), 0, c.stream, src, spitch, dst, dpitch, hout, nrows, scale,
+  hipLaunchKernelGGL((k_row_r2c<T, K>), grid, dim3(RowBlk<T, K>::THREADS), 0, c.stream, src, spitch, dst, dpitch, hout, nrows, scale,
                      c.twiddles<T>(K), c.twiddles<T>(2 * K));
   check_launch(name);
 }
@@ -549,7 +551,7 @@ void launch_row_c2r_K(Ctx& c, const cx<T>* src, long long spitch, int hin, T* ds
                       long long nrows, T scale, const char* name, double bytes) {
   const dim3 grid = row_grid<T, K>(nrows);
   LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_row_c2r<T, K>), grid, dim3(64), 0, c.stream, src, spitch, hin, dst, dpitch, nrows, scale,
+  hipLaunchKernelGGL((k_row_c2r<T, K>), grid, dim3(RowBlk<T, K>::THREADS), 0, c.stream, src, spitch, hin, dst, dpitch, nrows, scale,
                      c.twiddles<T>(K), c.twiddles<T>(2 * K));
   check_launch(name);
 }
@@ -559,7 +561,7 @@ void launch_row_prod_K(Ctx& c, const cx<T>* a, const cx<T>* b, long long spitch,
                        long long dpitch, int hout, long long nrows, const char* name, double bytes) {
   const dim3 grid = row_grid<T, K>(nrows);
   LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_row_prod<T, K>), grid, dim3(64), 0, c.stream, a, b, spitch, hin, dst, dpitch, hout, nrows,
+  hipLaunchKernelGGL((k_row_prod<T, K>), grid, dim3(RowBlk<T, K>::THREADS), 0, c.stream, a, b, spitch, hin, dst, dpitch, hout, nrows,
                      c.twiddles<T>(K), c.twiddles<T>(2 * K));
   check_launch(name);
 }
@@ -598,7 +600,7 @@ void launch_row_c2r_part_K(Ctx& c, const cx<T>* src, long long spitch, int hin, 
                            int idim, double lunit, int kind, const char* name, double bytes) {
   const dim3 grid = row_grid<T, K>(nrows);
   LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_row_c2r_part<T, K>), grid, dim3(64), 0, c.stream, src, spitch, hin, out, nrows, scale, idim,
+  hipLaunchKernelGGL((k_row_c2r_part<T, K>), grid, dim3(RowBlk<T, K>::THREADS), 0, c.stream, src, spitch, hin, out, nrows, scale, idim,
                      lunit, kind, c.twiddles<T>(K), c.twiddles<T>(2 * K));
   check_launch(name);
 }

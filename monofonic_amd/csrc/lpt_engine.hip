@@ -135,7 +135,7 @@ __device__ __forceinline__ cx<T> pmul(cx<T> a, cx<T> b) {
 // With STORE, the six real-space Hessian rows are also written to the stored
 // component arrays R + c*rstride in the order xx, yy, zz, xy, xz, yz.
 template <typename T, int K, bool THREE, bool STORE>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_lpt_zA(FieldY<T> f, cx<T>* S2, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4, cx<T>* R, long long rstride,
              long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
              const cx<T>* __restrict__ twM) {
@@ -209,7 +209,7 @@ constexpr int pw[NOPS] = {0, 0, -1, 0, 0, -1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 2};
 }  // namespace c1
 
 template <typename T, int K>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_lpt_zC1(FieldY<T> f1, FieldY<T> f2, const cx<T>* I1, const cx<T>* I2, const cx<T>* I3, const cx<T>* I4,
               cx<T>* out, long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
               const cx<T>* __restrict__ twM) {
@@ -264,7 +264,7 @@ __global__ void __launch_bounds__(64)
 // The two differences are formed in k space, one c2r each.  Operand rows
 // rotate through three fetch buffers.
 template <typename T, int K, int I>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_lpt_zC2(FieldY<T> f1, FieldY<T> f2, cx<T>* out, long long pitch, int hin, int n, T kfac, long long nrows,
               const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ZROW_PROLOGUE
@@ -330,7 +330,7 @@ constexpr int cb[6] = {0, 1, 2, 1, 2, 2};
 }  // namespace cs
 
 template <typename T, int K>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_lpt_zC(FieldY<T> f2, const cx<T>* R, long long rstride, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4,
              long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
              const cx<T>* __restrict__ twM) {
@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(64)
 template <typename T, int K>
 void launch_zA_K(Ctx& c, const FieldY<T>& f, cx<T>* S2, cx<T>* const* I, bool three, cx<T>* R, long long rstride,
                  long long pitch, int hin, int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zA", bytes);
   if (three && R)
     hipLaunchKernelGGL((k_lpt_zA<T, K, true, true>), gd, bd, 0, c.stream, f, S2, I[0], I[1], I[2], I[3], R, rstride,
@@ -439,7 +439,7 @@ void launch_zA_K(Ctx& c, const FieldY<T>& f, cx<T>* S2, cx<T>* const* I, bool th
 template <typename T, int K>
 void launch_zC_K(Ctx& c, const FieldY<T>& f2, const cx<T>* R, long long rstride, cx<T>* const* I, long long pitch,
                  int hin, int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zC", bytes);
   hipLaunchKernelGGL((k_lpt_zC<T, K>), gd, bd, 0, c.stream, f2, R, rstride, I[0], I[1], I[2], I[3], pitch, hin, n,
                      kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
@@ -449,7 +449,7 @@ void launch_zC_K(Ctx& c, const FieldY<T>& f2, const cx<T>* R, long long rstride,
 template <typename T, int K>
 void launch_zC1_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, cx<T>* const* I, cx<T>* out, long long pitch,
                   int hin, int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zC1", bytes);
   hipLaunchKernelGGL((k_lpt_zC1<T, K>), gd, bd, 0, c.stream, f1, f2, I[0], I[1], I[2], I[3], out, pitch, hin, n,
                      kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
@@ -459,7 +459,7 @@ void launch_zC1_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, cx<T>* const
 template <typename T, int K>
 void launch_zC2_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, int i, cx<T>* out, long long pitch, int hin,
                   int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(64);
+  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zC2", bytes);
   if (i == 0)
     hipLaunchKernelGGL((k_lpt_zC2<T, K, 0>), gd, bd, 0, c.stream, f1, f2, out, pitch, hin, n, kfac, nrows,

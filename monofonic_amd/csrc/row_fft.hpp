@@ -1,12 +1,13 @@
 // row_fft.hpp -- LDS-resident Stockham FFT for the contiguous (z) axis.
 //
-// One row of K complex values is transformed by P lanes of one wavefront
-// (P = 64 for K >= 128, fewer for short rows so that a wave holds 64/P rows).
+// One row of K complex values is transformed by P lanes (P = 128, two
+// wavefronts, for K >= 768; 64 for K >= 128; fewer for short rows so that a
+// wave holds 64/P rows).
 // The row lives in two LDS buffers used in ping-pong; in a radix-R stage each
 // lane takes butterflies j = lane, lane + P, ... of the K/R butterflies,
 // reads its R inputs x[j + q K/R] from one buffer and writes the Stockham
-// outputs to the other.  Between stages a barrier of the one-wave workgroup
-// orders the LDS traffic.  Rows enter and leave a transform in registers, in
+// outputs to the other.  Between stages a workgroup barrier orders the LDS
+// traffic.  Rows enter and leave a transform in registers, in
 // natural order (lane element e is index lane + P*e); where the radix plan
 // allows, the first stage runs on those registers, and the last stage of an
 // r2c leaves its outputs in them (row_fft_regs), which saves one to two LDS
@@ -90,8 +91,12 @@ MFX_RPLAN(768, 3, 8, 8, 4)
 // lanes per row
 template <int K>
 struct RowLanes {
-  static constexpr int P = K >= 128 ? 64 : (K >= 64 ? 32 : (K >= 32 ? 16 : 8));
-  static constexpr int RPW = 64 / P;        // rows per wavefront (= per workgroup)
+  // lanes per row: two waves for the longest rows, so that a lane holds as
+  // few elements (E) as at K = 384 and the fused z kernels keep their
+  // register budget
+  static constexpr int P = K >= 768 ? 128 : (K >= 128 ? 64 : (K >= 64 ? 32 : (K >= 32 ? 16 : 8)));
+  static constexpr int THREADS = P > 64 ? P : 64;  // threads of a row workgroup
+  static constexpr int RPW = THREADS / P;          // rows per workgroup
   static constexpr int E = (K + P - 1) / P;  // elements per lane for row-wise access
   static constexpr int LB = K + 2;           // one LDS buffer per row (K+1 used)
   static constexpr int NR = RowPlan<K>::NR;
@@ -101,7 +106,7 @@ struct RowLanes {
   static constexpr bool LAST_REG = EXACT && NR >= 2 && E % RowPlan<K>::R[NR - 1] == 0;
 };
 
-// one-wave workgroup barrier: orders the LDS traffic of the lanes
+// workgroup barrier: orders the LDS traffic of the lanes of a row
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // product of the radices of the stages before stage S
