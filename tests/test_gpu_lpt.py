@@ -217,3 +217,53 @@ def test_icgen_fused_matches_oracle(ctx, orc, fused, each):
         a = ic.particles(w) - (lattice_q(n, w) if w < 3 else 0)
         b = ref[w] - (lattice_q(n, w) if w < 3 else 0)
         assert rel_rms(a, b) < 1e-11, w
+
+
+@pytest.mark.parametrize("prec,path", [("double", "stored"), ("double", "streamed"), ("float", "stored")])
+def test_vorticity_invariant(ctx, orc, prec, path):
+    """The Lagrangian vorticity invariant of src/testing.cc:102-241 from the
+    library's LPT sources, with its products formed by the library's own
+    convolve_Gradients as the reference does: it falls as D^4 (x16 when D is
+    halved) with A3 and as D^3 (x8) without (lpt_invariant.py)"""
+    import monofonic_amd as M
+    from lpt_invariant import band_limited_phi, displacement_velocity, grads
+    n = 64
+    print(f"vorticity invariant {prec} {path}", flush=True)
+    cdt = np.complex128 if prec == "double" else np.complex64
+    a = band_limited_phi(orc, n, 7, BOX, n // 8)
+    phi = _kgrid(M, ctx, n, prec)
+    phi.upload(a)
+    phi2, phi3 = _kgrid(M, ctx, n, prec), _kgrid(M, ctx, n, prec)
+    A3 = [_kgrid(M, ctx, n, prec) for _ in range(3)]
+    conv = M.OrszagConvolver(ctx, n, BOX, prec)
+    conv.set_lpt_path(path)
+    conv.lpt_sources(phi, phi2, phi3, A3, 3)
+    fields = [g.kdata().astype(np.complex128) for g in (phi, phi2, phi3)]
+    fields.append([g.kdata().astype(np.complex128) for g in A3])
+    G = grads(n, BOX)
+
+    def kgrid_of(kc):
+        g = _kgrid(M, ctx, n, prec)
+        g.upload(np.ascontiguousarray(kc.astype(cdt)).view(g.dtype).reshape(n, n, n + 2))
+        return g
+
+    def rms(D, with_a3):
+        psi, vel = displacement_velocity(G, *fields, D, with_a3)
+        gp, gv = [kgrid_of(x) for x in psi], [kgrid_of(x) for x in vel]
+        out = []
+        for i in range(3):
+            j = (i + 1) % 3
+            inv = kgrid_of(G[j] * vel[i] - G[i] * vel[j])
+            for k in range(3):
+                conv.convolve_Gradients(gp[k], (i,), gv[k], (j,), M.op.add_to(inv))
+                conv.convolve_Gradients(gp[k], (j,), gv[k], (i,), M.op.subtract_from(inv))
+            inv.FourierTransformBackward()
+            out.append(np.sqrt(np.mean(inv.rdata().astype(np.float64) ** 2)))
+        return np.array(out)
+
+    w = [rms(D, True) for D in (1.0, 0.5)]
+    wo = [rms(D, False) for D in (1.0, 0.5)]
+    print(f"  with A3 {w[0]} / {w[1]}, without {wo[0]} / {wo[1]}", flush=True)
+    assert np.all(np.abs(w[0] / w[1] - 16) < 0.5), w
+    assert np.all(np.abs(wo[0] / wo[1] - 8) < 0.3), wo
+    assert np.all(w[1] < wo[1] / 20)
