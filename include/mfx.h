@@ -305,7 +305,8 @@ int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
 /* grids of the realisation: 0 wnoise, 1 phi, 2 phi2, 3 phi3, 4..6 A3, 7 tmp.
  * After a run the noise holds the k-space field before its normalisation
  * (ic_generator.cc:341-346 normalises in place; here the normalisation is
- * applied as phi(1) reads it), the potentials hold the fields before the
+ * applied as phi(1) reads it) and, for MUSIC1, before its mean subtraction
+ * (which only reaches the k = 0 mode, zeroed by phi(1)), the potentials hold the fields before the
  * growth factors (ic_generator.cc:550-563 scales them in place; here the
  * factors are applied as the assembly reads them) and tmp is scratch. */
 int mfx_icgen_grid(mfx_icgen *ic, int which, mfx_grid **g);

@@ -928,11 +928,13 @@ static void check_threefry(const mfx_grid* g, unsigned long long seed) {
 }
 
 // fills from device-resident seeds (stream-ordered, allocation-free)
-static void music1_dev(mfx_grid* g, const long* dseed, int cubesize, void* scratch) {
+// zeromean: 1 subtracts the mean as the reference's generator does; 2 leaves
+// it, for callers that discard the k = 0 mode
+static void music1_dev(mfx_grid* g, const long* dseed, int cubesize, void* scratch, int zeromean = 1) {
   if (g->prec == MFX_FLOAT)
-    music1_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], dseed, cubesize, 1, scratch);
+    music1_fill<float>(g->ctx->c, (float*)g->d, (int)g->n[0], dseed, cubesize, zeromean, scratch);
   else
-    music1_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], dseed, cubesize, 1, scratch);
+    music1_fill<double>(g->ctx->c, (double*)g->d, (int)g->n[0], dseed, cubesize, zeromean, scratch);
   g->space = MFX_RSPACE;
 }
 static void ngenic_dev(mfx_grid* g, const unsigned* dtable) {
@@ -1503,7 +1505,8 @@ static void icgen_device_run(mfx_icgen* ic) {
   const double volfac = std::pow(p.boxlen / (double)p.n / 2.0 / M_PI, 1.5);
   // white noise (ic_generator.cc:249-257)
   if (p.generator == 0)
-    music1_dev(wn, (const long*)ic->dev_seed, 32, ic->rng_scratch);
+    // the mean of the noise only reaches its k = 0 mode, which phi(1) zeroes
+    music1_dev(wn, (const long*)ic->dev_seed, 32, ic->rng_scratch, 2);
   else if (p.generator == 1)
     ngenic_dev(wn, ic->dev_table);
   else

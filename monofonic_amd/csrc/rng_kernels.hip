@@ -271,6 +271,10 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
                        part);
     check_launch("music1_cubes");
   }
+  // zeromean 2: the field is only transformed and its k = 0 mode discarded
+  // (phi(1) of the IC sequence), so neither the mean nor the unused padding
+  // reals of each row matter
+  if (zeromean == 2) return;
   hipLaunchKernelGGL(k_dd_mean, dim3(1), dim3(256), 0, c.stream, part, ncube,
                      1.0 / (double)((size_t)n * (size_t)n * (size_t)n), mean);
   check_launch("music1_mean");
