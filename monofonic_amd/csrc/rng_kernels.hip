@@ -149,7 +149,8 @@ __device__ __forceinline__ void accept_ranks(bool ok0, bool ok1, int* cnt, int& 
   off1 = before1 + __popcll(b1 & lt);
 }
 
-template <typename T>
+// SUMS: accumulate the per-cube double-double sums for the mean
+template <typename T, bool SUMS>
 __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, const long* seedp, dd* part) {
   __shared__ unsigned mt[MT_N];
   const long baseseed = *seedp;
@@ -192,12 +193,13 @@ __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, in
         long long gi = (long long)ci * cs + ii, gj = (long long)cj * cs + jj, gk = (long long)ck * cs + kk;
         T sv = T(val[r]);
         data[(gi * n + gj) * pitch + gk] = sv;
-        acc = dd_add(acc, (double)sv);
+        if constexpr (SUMS) acc = dd_add(acc, (double)sv);
       }
     }
     count += nacc;
     __syncthreads();
   }
+  if constexpr (!SUMS) return;
   red[tid] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -267,8 +269,12 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
   dd* part = (dd*)((char*)scratch + 256);
   {
     LaunchScope ls(c, "music1_cubes", (double)n * n * n * sizeof(T));
-    hipLaunchKernelGGL(k_music1_cubes<T>, dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs, dseed,
-                       part);
+    if (zeromean == 2)
+      hipLaunchKernelGGL((k_music1_cubes<T, false>), dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs,
+                         dseed, part);
+    else
+      hipLaunchKernelGGL((k_music1_cubes<T, true>), dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs,
+                         dseed, part);
     check_launch("music1_cubes");
   }
   // zeromean 2: the field is only transformed and its k = 0 mode discarded
