@@ -75,7 +75,20 @@ def run_gpu(M, ctx, n, precision, steps, warmup, fused, profile_step=True, gener
         stats = ctx.profile_stats()
         ctx.profile(False)
         log(f"[bench] {precision} profiled step done")
+        calibration_copy(M, ctx, n, precision)
     return wall, ev_ms, stats, ic
+
+
+def calibration_copy(M, ctx, n, precision):
+    """one device-to-device copy of an n^3 grid (n^2 (n+2) reals read and
+    written, a blit kernel of known traffic), after the timed steps: the
+    reference dispatch for the calibration of the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE counters"""
+    a = M.Grid_FFT(ctx, n, 1.0, precision)
+    b = M.Grid_FFT(ctx, n, 1.0, precision)
+    b.copy_from(a)
+    ctx.synchronize()
+    del a, b
 
 
 def roofline(stats):
