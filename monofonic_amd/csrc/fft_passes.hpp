@@ -443,6 +443,39 @@ struct RowBlk {
   }                                                                                 \
   __syncthreads();                                                                  \
   const int lane = threadIdx.x % P_, r_ = threadIdx.x / P_;                         \
+  MFX_ROW_LOOP_BEGIN(T, K)
+
+// A row workgroup loops over rows base + r_, base = blockIdx.x*RPW, advancing
+// by gridDim.x*RPW (the same trip count for every lane): the twiddle tables
+// are copied to LDS once per workgroup, not once per row.  Every transform
+// begins with a barrier, so one row's buffers are free when the next row's
+// transform writes them.  The row context takes an opaque zero (lz_) so that
+// its per-lane twiddles and indices are loaded and computed per row rather
+// than held in registers across the loop.
+#define MFX_ROW_LOOP_BEGIN(T, K)                                                                    \
+  for (long long base_ = (long long)blockIdx.x * RowBlk<T, K>::RPW; base_ < nrows;                   \
+       base_ += (long long)gridDim.x * RowBlk<T, K>::RPW) {                                          \
+    const long long row = base_ + r_;                                                               \
+    const bool act = row < nrows;                                                                   \
+    const long long rowc = act ? row : 0;                                                           \
+    int lz_ = 0;                                                                                    \
+    asm volatile("" : "+v"(lz_));                                                                   \
+    RowCtx<T, K> rc{lds_ + r_ * 2 * LB_ + lz_, lds_ + r_ * 2 * LB_ + LB_ + lz_, lane + lz_, tw_ + lz_, \
+                    tw_ + (K) + lz_};
+#define MFX_ROW_LOOP_END }
+
+// One row (per row slot r_) per workgroup, for the fused z kernels: holding
+// their many operand rows leaves no registers for a row loop.
+#define MFX_ROW_PROLOGUE1(T, K)                                                     \
+  constexpr int P_ = RowBlk<T, K>::P, LB_ = RowBlk<T, K>::LB;                       \
+  __shared__ cx<T> lds_[RowBlk<T, K>::LDS];                                         \
+  __shared__ cx<T> tw_[2 * (K)];                                                    \
+  for (int i_ = threadIdx.x; i_ < (K); i_ += RowBlk<T, K>::THREADS) {             \
+    tw_[i_] = twK[i_];                                                              \
+    tw_[(K) + i_] = twM[i_];                                                        \
+  }                                                                                 \
+  __syncthreads();                                                                  \
+  const int lane = threadIdx.x % P_, r_ = threadIdx.x / P_;                         \
   const long long row = (long long)blockIdx.x * RowBlk<T, K>::RPW + r_;             \
   const bool act = row < nrows;                                                     \
   const long long rowc = act ? row : 0;                                             \
@@ -459,6 +492,7 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = buf_load<T>(r, (unsigned)(lane + P_ * e) * sizeof(cx<T>));
   rc.r2c_store(v, dst + rowc * dpitch, hout, act, scale);
+  MFX_ROW_LOOP_END
 }
 
 template <typename T, int K>
@@ -478,6 +512,7 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
     const int n = lane + P_ * e;
     if (n <= K) buf_store<T>(n < K ? v[e < E ? e : E - 1] * scale : mk(T(0), T(0)), r, (unsigned)n * sizeof(cx<T>));
   }
+  MFX_ROW_LOOP_END
 }
 
 // c2r of the z rows of an n^3 grid straight into the FP64 particle arrays
@@ -513,6 +548,7 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
       buf_store<double>(mk(f0, f1), r, (unsigned)m * sizeof(cx<double>));
     }
   }
+  MFX_ROW_LOOP_END
 }
 
 // P3: c2r of two truncated half spectra, pointwise product, r2c, truncation
@@ -529,10 +565,20 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
 #pragma unroll
   for (int e = 0; e < E; ++e) va[e] = mk(va[e].x * vb[e].x, va[e].y * vb[e].y);
   rc.r2c_store(va, dst + rowc * dpitch, hout, act, T(1));
+  MFX_ROW_LOOP_END
 }
 
 template <typename T, int K>
 dim3 row_grid(long long nrows) {
+  // enough workgroups for several residency rounds on 256 CUs; each loops
+  // over nrows / (16384 RPW) rows
+  const long long need = (nrows + RowBlk<T, K>::RPW - 1) / RowBlk<T, K>::RPW;
+  return dim3((unsigned)(need < 16384 ? need : 16384));
+}
+
+// one workgroup per RPW rows (MFX_ROW_PROLOGUE1 kernels)
+template <typename T, int K>
+dim3 row_grid1(long long nrows) {
   return dim3((unsigned)((nrows + RowBlk<T, K>::RPW - 1) / RowBlk<T, K>::RPW));
 }
 

@@ -124,7 +124,7 @@ __device__ __forceinline__ cx<T> pmul(cx<T> a, cx<T> b) {
 }
 
 #define MFX_ZROW_PROLOGUE                                                           \
-  MFX_ROW_PROLOGUE(T, K)                                                            \
+  MFX_ROW_PROLOGUE1(T, K)                                                           \
   constexpr int EPT = RowCtx<T, K>::E;                                              \
   using Raw = typename ZRow<T, K>::Raw;                                             \
   ZRow<T, K> z{rc, act, rowc * pitch, hin, n, kfac, rowc * (long long)(K)};
@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
 template <typename T, int K>
 void launch_zA_K(Ctx& c, const FieldY<T>& f, cx<T>* S2, cx<T>* const* I, bool three, cx<T>* R, long long rstride,
                  long long pitch, int hin, int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
+  const dim3 gd = row_grid1<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zA", bytes);
   if (three && R)
     hipLaunchKernelGGL((k_lpt_zA<T, K, true, true>), gd, bd, 0, c.stream, f, S2, I[0], I[1], I[2], I[3], R, rstride,
@@ -439,7 +439,7 @@ void launch_zA_K(Ctx& c, const FieldY<T>& f, cx<T>* S2, cx<T>* const* I, bool th
 template <typename T, int K>
 void launch_zC_K(Ctx& c, const FieldY<T>& f2, const cx<T>* R, long long rstride, cx<T>* const* I, long long pitch,
                  int hin, int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
+  const dim3 gd = row_grid1<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zC", bytes);
   hipLaunchKernelGGL((k_lpt_zC<T, K>), gd, bd, 0, c.stream, f2, R, rstride, I[0], I[1], I[2], I[3], pitch, hin, n,
                      kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
@@ -449,7 +449,7 @@ void launch_zC_K(Ctx& c, const FieldY<T>& f2, const cx<T>* R, long long rstride,
 template <typename T, int K>
 void launch_zC1_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, cx<T>* const* I, cx<T>* out, long long pitch,
                   int hin, int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
+  const dim3 gd = row_grid1<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zC1", bytes);
   hipLaunchKernelGGL((k_lpt_zC1<T, K>), gd, bd, 0, c.stream, f1, f2, I[0], I[1], I[2], I[3], out, pitch, hin, n,
                      kfac, nrows, c.twiddles<T>(K), c.twiddles<T>(2 * K));
@@ -459,7 +459,7 @@ void launch_zC1_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, cx<T>* const
 template <typename T, int K>
 void launch_zC2_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, int i, cx<T>* out, long long pitch, int hin,
                   int n, T kfac, long long nrows, double bytes) {
-  const dim3 gd = row_grid<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
+  const dim3 gd = row_grid1<T, K>(nrows), bd(RowBlk<T, K>::THREADS);
   LaunchScope ls(c, "lpt_zC2", bytes);
   if (i == 0)
     hipLaunchKernelGGL((k_lpt_zC2<T, K, 0>), gd, bd, 0, c.stream, f1, f2, out, pitch, hin, n, kfac, nrows,

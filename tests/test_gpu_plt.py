@@ -90,14 +90,15 @@ def lattice_q(n, idim):
     return np.broadcast_to(q.reshape(sh), (n, n, n))
 
 
-@pytest.mark.parametrize("n,order", [(32, 3), (64, 2)])
-def test_icgen_plt_matches_oracle(ctx, orc, oplt, n, order):
-    """3LPT/2LPT FP64 ICs with PLT-corrected assembly vs the oracle pipeline"""
+@pytest.mark.parametrize("n,order,each", [(32, 3, False), (32, 3, True), (64, 2, False)])
+def test_icgen_plt_matches_oracle(ctx, orc, oplt, n, order, each):
+    """3LPT/2LPT FP64 ICs with PLT-corrected assembly vs the oracle pipeline,
+    six-component and one-at-a-time assembly"""
     import monofonic_amd as M
     from test_gpu_lpt import oracle_pipeline
-    print(f"icgen PLT n={n} order={order}", flush=True)
+    print(f"icgen PLT n={n} order={order} assemble_each={each}", flush=True)
     ic = M.ICGenerator(ctx, n, 100.0, 24.0, LPTorder=order, precision="double", generator="MUSIC1", seed=17,
-                       fused=True, PLT=True)
+                       fused=True, PLT=True, assemble_each=each)
     ic.run()
     ref, _, _ = oracle_pipeline(orc, n, 100.0, 24.0, order, 17, orc.Cosmology(), plt_tabs=oplt[0])
     # Tolerance: ~156 table modes on zone-boundary symmetry lines have their
