@@ -69,11 +69,11 @@ __device__ __forceinline__ void static_for(F&& f) {
 // One z row of the fused stages.  Every operand row is fetched into registers
 // (fetch) one step before it is transformed, so that its HBM latency overlaps
 // the previous transform.
-template <typename T, int K>
+template <typename T, int K, bool CW = false>
 struct ZRow {
   static constexpr int E = RowCtx<T, K>::E;
   using Raw = cx<T>[E + 1];
-  RowCtx<T, K> rc;
+  RowCtx<T, K, CW> rc;
   bool act;
   long long off;  // row * pitch
   int hin, n;
@@ -123,11 +123,14 @@ __device__ __forceinline__ cx<T> pmul(cx<T> a, cx<T> b) {
   return mk(a.x * b.x, a.y * b.y);
 }
 
-#define MFX_ZROW_PROLOGUE                                                           \
+#define MFX_ZROW_PROLOGUE_CW(CW)                                                    \
   MFX_ROW_PROLOGUE1(T, K)                                                           \
   constexpr int EPT = RowCtx<T, K>::E;                                              \
   using Raw = typename ZRow<T, K>::Raw;                                             \
-  ZRow<T, K> z{rc, act, rowc * pitch, hin, n, kfac, rowc * (long long)(K)};
+  ZRow<T, K, CW> z{RowCtx<T, K, CW>{rc.A, rc.B, rc.lane, rc.twK, rc.twM}, act, rowc * pitch, hin, n, kfac, \
+                   rowc * (long long)(K)};                                          \
+  z.rc.cache();
+#define MFX_ZROW_PROLOGUE MFX_ZROW_PROLOGUE_CW(false)
 
 // stage A: 2LPT source S2 = sum_{i<j} (phi_ii phi_jj - phi_ij^2) (ic_generator.cc:471-475)
 // and, for 3LPT, the 3a intermediates I1 = phi_xx phi_yy - phi_xy^2, I2 = phi_xy phi_xz,
@@ -334,7 +337,7 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_lpt_zC(FieldY<T> f2, const cx<T>* R, long long rstride, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4,
              long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
              const cx<T>* __restrict__ twM) {
-  MFX_ZROW_PROLOGUE
+  MFX_ZROW_PROLOGUE_CW(sizeof(T) == 4)
   cx<T> h[6][EPT];
   {
     Raw rb[2];

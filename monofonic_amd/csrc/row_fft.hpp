@@ -256,8 +256,10 @@ __device__ __forceinline__ void buf_store(cx<T> v, __amdgpu_buffer_rsrc_t r, uns
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, v), r, off, 0, 0);
 }
 
-// Row context: lane, LDS buffers and twiddles of one row
-template <typename T, int K>
+// Row context: lane, LDS buffers and twiddles of one row.  With CW the
+// lane's E half-angle twiddles twM[lane + P*e] are held in registers (cache()
+// loads them), which saves their LDS reads in every transform of the row.
+template <typename T, int K, bool CW = false>
 struct RowCtx {
   static constexpr int P = RowLanes<K>::P, E = RowLanes<K>::E;
   cx<T>* A;
@@ -265,6 +267,24 @@ struct RowCtx {
   int lane;
   const cx<T>* twK;  // exp(-2 pi i n/K), n < K, in LDS
   const cx<T>* twM;  // exp(-2 pi i n/(2K)), n < K, in LDS
+  cx<T> wm[CW ? E : 1];
+
+  __device__ __forceinline__ void cache() {
+    if constexpr (CW) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int k = lane + P * e;
+        wm[e] = twM[k < K ? k : 0];
+      }
+    }
+  }
+  // twM[k] for k = lane + P*e < K
+  __device__ __forceinline__ cx<T> tm(int e, int k) const {
+    if constexpr (CW)
+      return wm[e < E ? e : E - 1];
+    else
+      return twM[k];
+  }
 
   // the lane's entries k = lane + P*e (e <= E) of a half-spectrum row whose
   // first hin entries are stored; entries k >= hin load as zero
@@ -301,7 +321,7 @@ struct RowCtx {
           Bk.y = T(0);
         }
         cx<T> Ev = Ak + Bk;
-        cx<T> O = twmul<true>(Ak - Bk, twM[k]);
+        cx<T> O = twmul<true>(Ak - Bk, tm(e, k));
         z = mk(Ev.x - O.y, Ev.y + O.x);
       }
       v[e] = z;
@@ -354,7 +374,7 @@ struct RowCtx {
           // X[k] = (Z[k] + conj Z[K-k])/2 - i/2 e^{-2 pi i k/(2K)} (Z[k] - conj Z[K-k])
           cx<T> Zk = z[e < E ? e : E - 1], Zc = conj(F[(K - k) % K]);
           cx<T> a = (Zk + Zc) * T(0.5);
-          cx<T> dd = cmul((Zk - Zc) * T(0.5), twM[k]);
+          cx<T> dd = cmul((Zk - Zc) * T(0.5), tm(e, k));
           X = mk(a.x + dd.y, a.y - dd.x);
         }
         buf_store<T>(X * scale, r, (unsigned)k * sizeof(cx<T>));
