@@ -142,7 +142,8 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_lpt_zA(FieldY<T> f, cx<T>* S2, cx<T>* I1, cx<T>* I2, cx<T>* I3, cx<T>* I4, cx<T>* R, long long rstride,
              long long pitch, int hin, int n, T kfac, long long nrows, const cx<T>* __restrict__ twK,
              const cx<T>* __restrict__ twM) {
-  MFX_ZROW_PROLOGUE
+  // the twiddle cache fits the three-waves-per-SIMD budget at K = 384 only
+  MFX_ZROW_PROLOGUE_CW(sizeof(T) == 4 && K <= 384)
   Raw r0, r1;
   cx<T> dxx[EPT], dyy[EPT], s2[EPT], w[EPT], oxy[EPT];
   auto keep = [&](const cx<T>(&v)[EPT], int c) {
