@@ -21,6 +21,12 @@
 // truncated products; the 3a intermediates trunc(phi_xx phi_yy - phi_xy^2),
 // trunc(phi_xy phi_xz), trunc(phi_yz^2), trunc(phi_xz^2) keep the two-stage
 // truncation of OrszagConvolver::convolve3 (convolution.hh:542-549).
+//
+// Two 3LPT paths: streamed (each z stage transforms the Hessian rows it
+// needs: stages A, C1, C2) and stored (stage A also keeps phi's real-space
+// Hessian rows, and one stage C forms phi3 and A3 from them, phi2's rows and
+// the 3a intermediates).  The intermediates are truncated to the n-grid and
+// padded back by one projection pass per axis on their y/z slot (turn).
 #include "fft_passes.hpp"
 #include "ops.hpp"
 
