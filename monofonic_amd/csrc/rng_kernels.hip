@@ -151,7 +151,8 @@ __device__ __forceinline__ void accept_ranks(bool ok0, bool ok1, int* cnt, int& 
 
 // SUMS: accumulate the per-cube double-double sums for the mean
 template <typename T, bool SUMS>
-__global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, const long* seedp, dd* part) {
+__global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, int lcs, const long* seedp,
+                                                      dd* part) {
   __shared__ unsigned mt[MT_N];
   const long baseseed = *seedp;
   __shared__ int wcnt[8];
@@ -161,9 +162,12 @@ __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, in
   const int ci = (int)(icube / ((long long)nc * nc)), cj = (int)((icube / nc) % nc), ck = (int)(icube % nc);
   if (tid == 0) mt_seed(mt, (unsigned long)(baseseed + icube));
   __syncthreads();
-  const long long cnt = (long long)cs * cs * cs;
+  // the cube size cs = 2^lcs (a power of two, as the grid size is) and a cube
+  // holds cs^3 < 2^31 values (both checked on the host), so the stream
+  // positions are 32-bit and split into cell indices by shifts
+  const unsigned ucs = (unsigned)cs, cnt = ucs * ucs * ucs;
   const long long pitch = n + 2;
-  long long count = 0;
+  unsigned count = 0;
   dd acc{0.0, 0.0};
   while (count < cnt) {
     mt_twist(mt);
@@ -187,16 +191,16 @@ __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, in
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       if (!ok[r]) continue;
-      long long e = count + (r == 0 ? off0 : total0 + off1);
+      const unsigned e = count + (unsigned)(r == 0 ? off0 : total0 + off1);
       if (e < cnt) {
-        int ii = (int)(e / ((long long)cs * cs)), jj = (int)((e / cs) % cs), kk = (int)(e % cs);
+        const unsigned kk = e & (ucs - 1u), jj = (e >> lcs) & (ucs - 1u), ii = e >> (2 * lcs);
         long long gi = (long long)ci * cs + ii, gj = (long long)cj * cs + jj, gk = (long long)ck * cs + kk;
         T sv = T(val[r]);
         data[(gi * n + gj) * pitch + gk] = sv;
         if constexpr (SUMS) acc = dd_add(acc, (double)sv);
       }
     }
-    count += nacc;
+    count += (unsigned)nacc;
     __syncthreads();
   }
   if constexpr (!SUMS) return;
@@ -249,6 +253,7 @@ static void music1_cubes(int n, int cubesize, int& nc, int& cs) {
     cs = n;
   }
   MFX_REQUIRE((long long)nc * cs == n, "MUSIC1 fill: grid size must be a multiple of the cube size");
+  MFX_REQUIRE((cs & (cs - 1)) == 0 && cs <= 1024, "MUSIC1 fill: cube size must be a power of two up to 1024");
 }
 
 size_t music1_scratch_bytes(int n, int cubesize) {
@@ -265,16 +270,18 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
   int nc, cs;
   music1_cubes(n, cubesize, nc, cs);
   const long long ncube = (long long)nc * nc * nc;
+  int lcs = 0;
+  while ((1 << lcs) < cs) ++lcs;
   double* mean = (double*)scratch;
   dd* part = (dd*)((char*)scratch + 256);
   {
     LaunchScope ls(c, "music1_cubes", (double)n * n * n * sizeof(T));
     if (zeromean == 2)
       hipLaunchKernelGGL((k_music1_cubes<T, false>), dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs,
-                         dseed, part);
+                         lcs, dseed, part);
     else
       hipLaunchKernelGGL((k_music1_cubes<T, true>), dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs,
-                         dseed, part);
+                         lcs, dseed, part);
     check_launch("music1_cubes");
   }
   // zeromean 2: the field is only transformed and its k = 0 mode discarded

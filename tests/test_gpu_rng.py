@@ -39,17 +39,18 @@ def test_threefry_uniforms_bit_exact(ctx, orc):
             assert np.array_equal(row, ref), (c, seed)
 
 
-@pytest.mark.parametrize("n", [32, 64])
-def test_music1_fill(ctx, orc, n):
+# cube sizes below, equal to and above the grid size
+@pytest.mark.parametrize("n,cs", [(32, 32), (64, 32), (64, 16), (16, 32)])
+def test_music1_fill(ctx, orc, n, cs):
     import monofonic_amd as M
     g = M.Grid_FFT(ctx, n, 100.0, "double")
-    M.RNG_music(4711).Fill_Grid(g)
+    M.RNG_music(4711, cs).Fill_Grid(g)
     got = g.download()
-    ref = orc.music1_fill(n, 4711)
+    ref = orc.music1_fill(n, 4711, cs)
     assert np.array_equal(got[:, :, :n], ref[:, :, :n])
     gf = M.Grid_FFT(ctx, n, 100.0, "float")
-    M.RNG_music(4711).Fill_Grid(gf)
-    assert np.array_equal(gf.download()[:, :, :n], orc.music1_fill(n, 4711, f32=True)[:, :, :n])
+    M.RNG_music(4711, cs).Fill_Grid(gf)
+    assert np.array_equal(gf.download()[:, :, :n], orc.music1_fill(n, 4711, cs, f32=True)[:, :, :n])
 
 
 @pytest.mark.parametrize("n", [16, 48])
