@@ -5,8 +5,8 @@
 // stream (seed baseseed + icube, GSL seeding rule) is produced 624 words at a
 // time by a parallel twist in LDS; every pair of consecutive uniforms is one
 // attempt of Leva's ratio method (gsl_ran_ugaussian_ratio_method), evaluated
-// for all 312 pairs of a block at once, and the accepted deviates are
-// compacted with a block scan so that they land in the cube in stream order.
+// for all 312 pairs of a block at once, and the accepted deviates are ranked
+// by wave ballots so that they land in the cube in stream order.
 // The global mean is taken with a double-double sum, i.e. the correctly
 // rounded exact sum, and subtracted in double (ctor at :35-66).
 //
