@@ -226,7 +226,9 @@ int mfx_set_particles(mfx_grid *field, double *out_dev, int idim, double lunit, 
 /* ------------------------------------------------------------ RNG plugins
  * RNG_plugin::Fill_Grid (include/random_plugin.hh:26-39) */
 /* MUSIC1 (src/plugins/random_music.cc:92-116, random_music_wnoise_generator.cc
- * :35-66,588-627,747-796): real-space N(0,1), leaves the grid in real space */
+ * :35-66,588-627,747-796): real-space N(0,1), leaves the grid in real space.
+ * The grid size is a power of two; the cube size (the grid size if that is
+ * smaller) must divide it and be at most 1024, else an error is returned. */
 int mfx_rng_music1_fill(mfx_grid *g, long seed, int cubesize);
 /* NGENIC (src/plugins/random_ngenic.cc:221-345): k-space noise, leaves the
  * grid in k space */
