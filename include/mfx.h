@@ -230,8 +230,8 @@ int mfx_set_particles(mfx_grid *field, double *out_dev, int idim, double lunit, 
  * The grid size is a power of two; the cube size (the grid size if that is
  * smaller) must divide it and be at most 1024, else an error is returned. */
 int mfx_rng_music1_fill(mfx_grid *g, long seed, int cubesize);
-/* NGENIC (src/plugins/random_ngenic.cc:221-345): k-space noise, leaves the
- * grid in k space */
+/* NGENIC (src/plugins/random_ngenic.cc:40-75 seed table, :85-152
+ * Fill_Grid): k-space noise, leaves the grid in k space */
 int mfx_rng_ngenic_fill(mfx_grid *g, long seed);
 /* Panphasia-HO Threefry-4x64-20 cell stream (external/panphasia_ho/
  * uniform_rand_threefry4x64.c:254-493): first Box-Muller deviate of each

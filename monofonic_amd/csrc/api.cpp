@@ -1455,7 +1455,7 @@ extern "C" int mfx_icgen_destroy(mfx_icgen* ic) {
 }
 
 // Host half of the noise fill for the next realisation.  The seed (and for
-// NGENIC the per-column seed table of random_ngenic.cc:230-265, a serial
+// NGENIC the per-column seed table of random_ngenic.cc:47-75, a serial
 // RANLUX stream of n^2 draws, computed by a worker thread so that it overlaps
 // the device work already enqueued) is written to one of two pinned staging
 // slots; icgen_commit_seed then enqueues the copy to the device.  A slot is

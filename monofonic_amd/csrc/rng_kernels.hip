@@ -15,7 +15,7 @@
 // restated bit for bit; Box-Muller in double (high_order_panphasia_routines.c
 // :224-228).
 //
-// NGenIC (src/plugins/random_ngenic.cc:275-345): one thread per (i,j) column
+// NGenIC (src/plugins/random_ngenic.cc:85-152): one thread per (i,j) column
 // running its own RANLUX ranlxd1 stream (exact 48-bit arithmetic on doubles).
 //
 // The acceptance tests compare rounded FP64 expressions; contraction into FMA
@@ -530,7 +530,7 @@ __host__ __device__ __forceinline__ double rl_uniform(Ranlxd& s) {
   return s.x[s.ir];
 }
 
-// RNG_ngenic ctor (random_ngenic.cc:230-266): seed table from one ranlxd1
+// RNG_ngenic ctor (random_ngenic.cc:40-75): seed table from one ranlxd1
 // stream in the reference's octant order, evaluated on the host
 void ngenic_seedtable(unsigned* table, int n, long seed) {
   Ranlxd s;

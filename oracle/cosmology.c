@@ -91,6 +91,8 @@ static void eh_onek(const double *eh, double k, double *tfb, double *tfc) {
   *tfc = Tc;
 }
 
+void orc_cosmo_eh_parts(const orc_cosmo_t *c, double k_mpc, double *tb, double *tc) { eh_onek(c->eh, k_mpc, tb, tc); }
+
 /* eisenstein_transfer::at_k (transfer_eisenstein.cc:215-220), k in h/Mpc */
 double orc_cosmo_transfer(const orc_cosmo_t *c, double k) {
   double tb, tc;

@@ -109,6 +109,12 @@ uint32_t orc_mt_next(orc_mt_t *s);
 /* Leva (1992) ratio-of-uniforms normal deviate, as used through
  * gsl_ran_ugaussian_ratio_method (random_music_wnoise_generator.cc:613) */
 double orc_gauss_ratio(orc_mt_t *s);
+/* the same with the constants (s, t, a, b, r1, r2) supplied */
+double orc_gauss_ratio_c(orc_mt_t *s, const double *c6);
+/* count deviates of the stream MT(seed); c6 = NULL: the published constants */
+void orc_gauss_ratio_bulk(unsigned long seed, size_t count, const double *c6, double *out);
+/* the first count words of MT19937(seed) */
+void orc_mt_words(unsigned long seed, size_t count, uint32_t *out);
 /* MUSIC1 white noise fill (random_music_wnoise_generator.cc:35-66,588-627,747-796;
  * random_music.cc:92-116) into the real-space padded layout, FP64 */
 void orc_music1_fill(double *data, int n, long baseseed, int cubesize);
@@ -126,6 +132,8 @@ void orc_panphasia_uniforms(size_t l, size_t j1, size_t j2, size_t j3, size_t se
 void orc_box_muller(const double *u, double *g, int count);
 /* cell white noise: first Box-Muller deviate of the cell's stream, level l = log2 n */
 void orc_threefry_fill(double *data, int n, size_t seed);
+/* the same fill evaluated at m cells (i, j, k) = ijk[3c..3c+2] of an n^3 grid */
+void orc_threefry_cells(const long long *ijk, int m, int n, size_t seed, double *out);
 
 /* NGenIC (random_ngenic.cc): RANLUX ranlxd1 + column fill */
 typedef struct { double xdbl[12]; double carry; unsigned int ir, jr, is, is_old, pr; } orc_ranlxd_t;
@@ -149,6 +157,8 @@ void orc_cosmo_init(orc_cosmo_t *c, double H0, double Omega_m, double Omega_b, d
                     double sigma_8, double Tcmb, int zero_radiation);
 void orc_cosmo_free(orc_cosmo_t *c);
 double orc_cosmo_transfer(const orc_cosmo_t *c, double k);
+/* the baryon and CDM parts T_b, T_c of TFfit_onek at k in Mpc^-1 */
+void orc_cosmo_eh_parts(const orc_cosmo_t *c, double k_mpc, double *tb, double *tc);
 double orc_cosmo_Dplus(const orc_cosmo_t *c, double a);
 double orc_cosmo_f(const orc_cosmo_t *c, double a);
 double orc_cosmo_vfact(const orc_cosmo_t *c, double a);
@@ -160,6 +170,15 @@ double orc_cosmo_sigma8_unnorm(const orc_cosmo_t *c);
 /* real-space dynamical matrix on ng^3 sites incl. the r = 0 background
  * (:161-284); out = 6 padded real grids xx, xy, xz, yy, yz, zz */
 void orc_plt_dynamical_matrix(int ng, double *out);
+/* the two halves of the Ewald split of init_D at the wrapped position ar,
+ * added into D (row-major 3x3), for split parameter eta: the real-space sum
+ * over images |n_i| <= nimg (add_greensftide_sr, particle_plt.hh:175-191) and
+ * the k-space sum over |m_i| <= nk, k = 2 pi m / ng (add_greensftide_lr,
+ * :194-204, with the fft_norm 1/ng^3) */
+void orc_plt_ewald_real(const double ar[3], double eta, int nimg, double D[9]);
+/* planted mistakes for mutation checks: 1 = sign of K1, 2 = k-space exponent */
+extern int orc_plt_mutation;
+void orc_plt_ewald_kspace(const double ar[3], double eta, int nk, int ng, double D[9]);
 /* largest eigenpair of a symmetric 3x3 matrix; eval ascending */
 void orc_sym3_eigen(const double A[3][3], double eval[3], double evec_max[3]);
 /* the same, with the eigenvector of a degenerate largest pair defined as the

@@ -195,6 +195,22 @@ def gauss_ratio(seed, count):
     return np.array([lib().orc_gauss_ratio(ctypes.byref(s)) for _ in range(count)])
 
 
+def gauss_ratio_bulk(seed, count, consts=None):
+    """count deviates of the Leva ratio method on MT19937(seed); consts =
+    (s, t, a, b, r1, r2) replaces the published constants"""
+    out = np.zeros(count)
+    c = None if consts is None else np.ascontiguousarray(consts, dtype=np.float64)
+    lib().orc_gauss_ratio_bulk(ctypes.c_ulong(seed), ctypes.c_size_t(count),
+                               None if c is None else _p(c), _p(out))
+    return out
+
+
+def mt_words(seed, count):
+    out = np.zeros(count, dtype=np.uint32)
+    lib().orc_mt_words(ctypes.c_ulong(seed), ctypes.c_size_t(count), _p(out))
+    return out
+
+
 def music1_fill(n, seed, cubesize=32, f32=False):
     if f32:
         a = np.zeros(real_shape(n), dtype=np.float32)
@@ -224,6 +240,14 @@ def threefry_fill(n, seed):
     a = np.zeros(real_shape(n))
     lib().orc_threefry_fill(_p(a), ctypes.c_int(n), ctypes.c_size_t(seed))
     return a
+
+
+def threefry_cells(n, seed, ijk):
+    """threefry_fill's values at the cells ijk (m x 3) of an n^3 grid"""
+    ijk = np.ascontiguousarray(ijk, dtype=np.int64).reshape(-1, 3)
+    out = np.zeros(ijk.shape[0])
+    lib().orc_threefry_cells(_p(ijk), ctypes.c_int(ijk.shape[0]), ctypes.c_int(n), ctypes.c_size_t(seed), _p(out))
+    return out
 
 
 class Ranlxd(ctypes.Structure):
@@ -281,6 +305,23 @@ class Cosmology:
     def transfer(self, k):
         return self._f("orc_cosmo_transfer", k)
 
+    EH_SLOTS = ("omhh", "obhh", "theta", "zeq", "keq", "zd", "Rd", "Req", "s", "ksilk", "alpha_c", "beta_c",
+                "alpha_b", "beta_b", "beta_node", "h", "fb", "fc")
+
+    def eh(self, name):
+        """a derived Eisenstein & Hu quantity (Mpc units)"""
+        return self.c.eh[self.EH_SLOTS.index(name)]
+
+    def set_eh(self, name, value):
+        """overwrite a derived quantity (plants a mistake for mutation checks)"""
+        self.c.eh[self.EH_SLOTS.index(name)] = value
+
+    def eh_parts(self, k_mpc):
+        """(T_b, T_c) of TFfit_onek at k in Mpc^-1"""
+        tb, tc = ctypes.c_double(), ctypes.c_double()
+        lib().orc_cosmo_eh_parts(ctypes.byref(self.c), ctypes.c_double(k_mpc), ctypes.byref(tb), ctypes.byref(tc))
+        return tb.value, tc.value
+
     def Dplus(self, a):
         return self._f("orc_cosmo_Dplus", a)
 
@@ -316,6 +357,21 @@ def plt_dynamical_matrix(ng):
     out = np.zeros((6, ng, ng, ng + 2))
     lib().orc_plt_dynamical_matrix(ctypes.c_int(ng), _p(out))
     return out
+
+
+def plt_ewald(ar, eta, nimg=3, nk=3, ng=64):
+    """(real-space half, k-space half) of init_D's Ewald sum at ar, 3x3 each"""
+    r = np.ascontiguousarray(ar, dtype=np.float64)
+    Dr = np.zeros(9)
+    Dk = np.zeros(9)
+    lib().orc_plt_ewald_real(_p(r), ctypes.c_double(eta), ctypes.c_int(nimg), _p(Dr))
+    lib().orc_plt_ewald_kspace(_p(r), ctypes.c_double(eta), ctypes.c_int(nk), ctypes.c_int(ng), _p(Dk))
+    return Dr.reshape(3, 3), Dk.reshape(3, 3)
+
+
+def plt_set_mutation(m):
+    """plant a mistake in the Ewald halves (0 = none, 1 = K1 sign, 2 = k-space exponent)"""
+    ctypes.c_int.in_dll(lib(), "orc_plt_mutation").value = int(m)
 
 
 def sym3_eigen(A):

@@ -23,16 +23,68 @@ static size_t rel3(int n, int i, int j, int k) { return ((size_t)i * n + j) * (s
 
 /* ---------------------------------------------------------------- init_D, real space
  * particle_plt.hh:161-276 for lattice_sc (charge_fac 1, Bravais = identity,
- * reciprocal = 2 pi identity), eta = 4, lnumber = knumber = 3 */
-void orc_plt_dynamical_matrix(int ng, double *out) {
+ * reciprocal = 2 pi identity), eta = 4, lnumber = knumber = 3.
+ * The reference takes the real-space images at integer offsets of the wrapped
+ * position ar = (i,j,k)/ng (period 1) and the k-space vectors at 2 pi m / ng,
+ * so its two halves are not the two halves of one Ewald sum: the total
+ * depends on eta (at the 1e-3 relative level).  Parity follows the
+ * reference, not the textbook sum. */
+/* planted mistakes for the mutation checks of the tests (0 = none):
+ * 1 flips the sign of K1, 2 uses exp(-k^2/(2 alpha^2)) in the k-space half */
+int orc_plt_mutation = 0;
+
+void orc_plt_ewald_real(const double ar[3], double eta, int nimg, double D[9]) {
   const double pi = PI, fourpi = 4.0 * PI;
   const double sqrtpi = sqrt(PI), pi32 = pow(PI, 1.5);
-  const double dx = 1.0 / (double)ng;
-  const double eta = 4.0;
   const double alpha = 1.0 / sqrt(2) / eta;
   const double alpha2 = alpha * alpha, alpha3 = alpha2 * alpha;
-  const double charge = 1.0 / pow((double)ng, 3) / 1.0;
+  (void)pi;
+  /* periodic copies (add_greensftide_sr, :175-191) */
+  for (int ix = -nimg; ix <= nimg; ix++)
+    for (int iy = -nimg; iy <= nimg; iy++)
+      for (int iz = -nimg; iz <= nimg; iz++) {
+        const double dr[3] = {ar[0] - (double)ix, ar[1] - (double)iy, ar[2] - (double)iz};
+        const double r = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        if (r < 1e-14) continue;
+        const double r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
+        const double K1 = (orc_plt_mutation == 1 ? -1.0 : 1.0) * (-alpha3 / pi32 * exp(-alpha2 * r2) / r2);
+        const double K2 = (erfc(alpha * r) + 2.0 * alpha / sqrtpi * exp(-alpha2 * r2) * r) / fourpi;
+        for (int mu = 0; mu < 3; ++mu)
+          for (int nu = mu; nu < 3; ++nu) {
+            const double dd = dr[mu] * dr[nu] * K1 + ((mu == nu ? 1.0 : 0.0) / r3 - 3.0 * (dr[mu] * dr[nu]) / r5) * K2;
+            D[mu * 3 + nu] += dd;
+            D[nu * 3 + mu] += (mu != nu) ? dd : 0.0;
+          }
+      }
+}
+
+void orc_plt_ewald_kspace(const double ar[3], double eta, int nk, int ng, double D[9]) {
+  const double pi = PI;
+  const double alpha = 1.0 / sqrt(2) / eta;
+  const double alpha2 = alpha * alpha;
   const double fft_norm = 1.0 / pow((double)ng, 3.0);
+  /* add_greensftide_lr, :194-204 */
+  for (int ix = -nk; ix <= nk; ix++)
+    for (int iy = -nk; iy <= nk; iy++)
+      for (int iz = -nk; iz <= nk; iz++) {
+        if (abs(ix) + abs(iy) + abs(iz) == 0) continue;
+        const double ak[3] = {2.0 * pi * ((double)ix / ng), 2.0 * pi * ((double)iy / ng), 2.0 * pi * ((double)iz / ng)};
+        const double kmod2 = ak[0] * ak[0] + ak[1] * ak[1] + ak[2] * ak[2];
+        const double ex = orc_plt_mutation == 2 ? 2 * alpha2 : 4 * alpha2;
+        const double term = exp(-kmod2 / ex) * cos(ak[0] * ar[0] + ak[1] * ar[1] + ak[2] * ar[2]) / kmod2 * fft_norm;
+        for (int mu = 0; mu < 3; ++mu)
+          for (int nu = mu; nu < 3; ++nu) {
+            const double dd = ak[mu] * ak[nu] * term;
+            D[mu * 3 + nu] += dd;
+            D[nu * 3 + mu] += (mu != nu) ? dd : 0.0;
+          }
+      }
+}
+
+void orc_plt_dynamical_matrix(int ng, double *out) {
+  const double dx = 1.0 / (double)ng;
+  const double eta = 4.0;
+  const double charge = 1.0 / pow((double)ng, 3) / 1.0;
   const size_t nt = (size_t)ng * ng * (ng + 2);
   for (int c = 0; c < 6; ++c) memset(out + c * nt, 0, nt * sizeof(double));
 #pragma omp parallel for collapse(2)
@@ -42,46 +94,16 @@ void orc_plt_dynamical_matrix(int ng, double *out) {
         const double x[3] = {dx * (double)i, dx * (double)j, dx * (double)k};
         double ar[3];
         for (int d = 0; d < 3; ++d) ar[d] = fmod(2.0 * 1.0 + x[d], 1.0); /* vec3::wrap_abs */
-        double D[3][3] = {{0}};
-        /* real-space part, periodic copies (add_greensftide_sr, :175-191) */
-        for (int ix = -3; ix <= 3; ix++)
-          for (int iy = -3; iy <= 3; iy++)
-            for (int iz = -3; iz <= 3; iz++) {
-              const double dr[3] = {ar[0] - (double)ix, ar[1] - (double)iy, ar[2] - (double)iz};
-              const double r = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-              if (r < 1e-14) continue;
-              const double r2 = r * r, r3 = r2 * r, r5 = r3 * r2;
-              const double K1 = -alpha3 / pi32 * exp(-alpha2 * r2) / r2;
-              const double K2 = (erfc(alpha * r) + 2.0 * alpha / sqrtpi * exp(-alpha2 * r2) * r) / fourpi;
-              for (int mu = 0; mu < 3; ++mu)
-                for (int nu = mu; nu < 3; ++nu) {
-                  const double dd = dr[mu] * dr[nu] * K1 + ((mu == nu ? 1.0 : 0.0) / r3 - 3.0 * (dr[mu] * dr[nu]) / r5) * K2;
-                  D[mu][nu] += dd;
-                  D[nu][mu] += (mu != nu) ? dd : 0.0;
-                }
-            }
-        /* k-space part (add_greensftide_lr, :194-204) */
-        for (int ix = -3; ix <= 3; ix++)
-          for (int iy = -3; iy <= 3; iy++)
-            for (int iz = -3; iz <= 3; iz++) {
-              if (abs(ix) + abs(iy) + abs(iz) == 0) continue;
-              const double ak[3] = {2.0 * pi * ((double)ix / ng), 2.0 * pi * ((double)iy / ng), 2.0 * pi * ((double)iz / ng)};
-              const double kmod2 = ak[0] * ak[0] + ak[1] * ak[1] + ak[2] * ak[2];
-              const double term = exp(-kmod2 / (4 * alpha2)) * cos(ak[0] * ar[0] + ak[1] * ar[1] + ak[2] * ar[2]) / kmod2 * fft_norm;
-              for (int mu = 0; mu < 3; ++mu)
-                for (int nu = mu; nu < 3; ++nu) {
-                  const double dd = ak[mu] * ak[nu] * term;
-                  D[mu][nu] += dd;
-                  D[nu][mu] += (mu != nu) ? dd : 0.0;
-                }
-            }
+        double D[9] = {0};
+        orc_plt_ewald_real(ar, eta, 3, D);
+        orc_plt_ewald_kspace(ar, eta, 3, ng, D);
         const size_t q = rel3(ng, i, j, k);
-        out[0 * nt + q] = D[0][0] * charge;
-        out[1 * nt + q] = D[0][1] * charge;
-        out[2 * nt + q] = D[0][2] * charge;
-        out[3 * nt + q] = D[1][1] * charge;
-        out[4 * nt + q] = D[1][2] * charge;
-        out[5 * nt + q] = D[2][2] * charge;
+        out[0 * nt + q] = D[0] * charge;
+        out[1 * nt + q] = D[1] * charge;
+        out[2 * nt + q] = D[2] * charge;
+        out[3 * nt + q] = D[4] * charge;
+        out[4 * nt + q] = D[5] * charge;
+        out[5 * nt + q] = D[8] * charge;
       }
   /* background at r = 0 (:278-284) */
   out[0 * nt] = 1.0 / 3.0;

@@ -54,9 +54,11 @@ uint32_t orc_mt_next(orc_mt_t *s) {
 
 static inline double mt_uniform(orc_mt_t *s) { return orc_mt_next(s) / 4294967296.0; }
 
-/* Leva (1992) ratio method */
-double orc_gauss_ratio(orc_mt_t *s) {
-  const double cs = 0.449871, ct = -0.386595, ca = 0.19600, cb = 0.25472, r1 = 0.27597, r2 = 0.27846;
+/* Leva (1992) ratio method with its six constants (s, t, a, b, r1, r2)
+ * supplied: orc_gauss_ratio passes the published ones, the tests also plant
+ * perturbed ones to show that their checks notice */
+double orc_gauss_ratio_c(orc_mt_t *s, const double *c6) {
+  const double cs = c6[0], ct = c6[1], ca = c6[2], cb = c6[3], r1 = c6[4], r2 = c6[5];
   double u, v, x, y, Q;
   do {
     u = 1 - mt_uniform(s);
@@ -67,6 +69,23 @@ double orc_gauss_ratio(orc_mt_t *s) {
     Q = x * x + y * (ca * y - cb * x);
   } while (Q >= r1 && (Q > r2 || v * v > -4 * u * u * log(u)));
   return v / u;
+}
+
+static const double leva_consts[6] = {0.449871, -0.386595, 0.19600, 0.25472, 0.27597, 0.27846};
+
+double orc_gauss_ratio(orc_mt_t *s) { return orc_gauss_ratio_c(s, leva_consts); }
+
+void orc_gauss_ratio_bulk(unsigned long seed, size_t count, const double *c6, double *out) {
+  orc_mt_t st;
+  orc_mt_seed(&st, seed);
+  if (!c6) c6 = leva_consts;
+  for (size_t i = 0; i < count; ++i) out[i] = orc_gauss_ratio_c(&st, c6);
+}
+
+void orc_mt_words(unsigned long seed, size_t count, uint32_t *out) {
+  orc_mt_t st;
+  orc_mt_seed(&st, seed);
+  for (size_t i = 0; i < count; ++i) out[i] = orc_mt_next(&st);
 }
 
 /* music_wnoise_generator(res,cubesize,seed,zeromean=true):
@@ -289,6 +308,17 @@ void orc_threefry_fill(double *data, int n, size_t seed) {
     }
 }
 
+void orc_threefry_cells(const long long *ijk, int m, int n, size_t seed, double *out) {
+  size_t l = 0;
+  while (((size_t)1 << l) < (size_t)n) ++l;
+  for (int c = 0; c < m; ++c) {
+    double u[2], g[2];
+    orc_panphasia_uniforms(l, (size_t)ijk[3 * c], (size_t)ijk[3 * c + 1], (size_t)ijk[3 * c + 2], seed, u, 2);
+    orc_box_muller(u, g, 2);
+    out[c] = g[0];
+  }
+}
+
 /* ---------------- RANLUX ranlxd ---------------- */
 static const double one_bit = 1.0 / 281474976710656.0; /* 2^-48 */
 static const int nxt[12] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 0};
@@ -377,7 +407,7 @@ double orc_ranlxd_uniform(orc_ranlxd_t *s) {
   return s->xdbl[s->ir];
 }
 
-/* RNG_ngenic ctor (random_ngenic.cc:230-266) */
+/* RNG_ngenic ctor (random_ngenic.cc:40-75) */
 void orc_ngenic_seedtable(unsigned int *table, int n, long seed) {
   orc_ranlxd_t st;
   orc_ranlxd_seed(&st, (unsigned long)seed, 202);
@@ -395,7 +425,7 @@ void orc_ngenic_seedtable(unsigned int *table, int n, long seed) {
   }
 }
 
-/* RNG_ngenic::Fill_Grid, non-distributed branch (random_ngenic.cc:275-345) */
+/* RNG_ngenic::Fill_Grid, non-distributed branch (random_ngenic.cc:85-152) */
 void orc_ngenic_fill(double *cdata, int n, const unsigned int *table) {
   const size_t N = (size_t)n, nh = N / 2 + 1;
   memset(cdata, 0, sizeof(double) * 2 * N * N * nh);

@@ -67,12 +67,15 @@ def test_plane_wave_unitary(ctx):
     assert np.max(np.abs(k)) < 1e-9
 
 
-@pytest.mark.parametrize("shape", [(1536, 16, 1536), (16, 1536, 1536), (1024, 16, 1024), (288, 24, 384)])
+@pytest.mark.parametrize("shape", [(1536, 16, 1536), (16, 1536, 1536), (1024, 16, 1024), (288, 24, 384),
+                                   (768, 16, 768), (16, 768, 768), (512, 16, 512), (384, 16, 384),
+                                   (16, 384, 512), (16, 512, 384)])
 @pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("float", 3e-6)])
 def test_r2c_long_axes_vs_numpy(ctx, shape, prec, tol):
-    """the column (1536, 1024) and row (768, 512 complex) lengths that only the
-    1024^3 configuration uses, on thin non-cubic grids, against numpy's rfftn
-    (pocketfft) with the unitary normalisation"""
+    """column lengths 1536, 1024, 768, 512, 384 and rows of 768, 512, 384,
+    256 and 192 complex (the 1024^3, 512^3 and 256^3 configurations and their
+    padded grids) on thin non-cubic grids, against numpy's rfftn (pocketfft)
+    with the unitary normalisation"""
     import monofonic_amd as M
     n0, n1, n2 = shape
     print(f"r2c {shape} {prec}", flush=True)
@@ -87,5 +90,29 @@ def test_r2c_long_axes_vs_numpy(ctx, shape, prec, tol):
     got = g.kdata()
     assert rel_rms(got, ref) < tol
     assert rel_max(got, ref) < 20 * tol
+    g.FourierTransformBackward()
+    assert rel_max(g.rdata(), x) < 20 * tol
+
+
+@pytest.mark.parametrize("n", [384, 512])
+@pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("float", 3e-6)])
+def test_r2c_cubic_large_vs_numpy(ctx, n, prec, tol):
+    """the cubic 384^3 and 512^3 transforms of the 256^3 and 512^3
+    configurations against numpy's rfftn"""
+    import monofonic_amd as M
+    print(f"r2c {n}^3 {prec}", flush=True)
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((n, n, n))
+    a = np.zeros((n, n, n + 2))
+    a[:, :, :n] = x
+    g = M.Grid_FFT(ctx, n, 100.0, prec)
+    g.upload(a)
+    del a
+    g.FourierTransformForward()
+    ref = np.fft.rfftn(x) / np.sqrt(float(n) ** 3)
+    got = g.kdata()
+    assert rel_rms(got, ref) < tol
+    assert rel_max(got, ref) < 20 * tol
+    del got, ref
     g.FourierTransformBackward()
     assert rel_max(g.rdata(), x) < 20 * tol

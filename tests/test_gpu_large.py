@@ -1,9 +1,10 @@
-"""The 1024^3 configuration (1536^3 padded, FP32 bulk) through a property that
-does not depend on the grid size: for a band-limited potential whose products
-stay below a third of the Nyquist wave number, the Orszag 3/2 products are
-exact, so phi2, phi3 and A3 are the same continuous fields at every
-resolution.  The 1024^3 fields sampled at every 16th point must equal the
-oracle's 64^3 FP64 fields (ic_generator.cc:464-547)."""
+"""The 512^3 (768^3 padded) and 1024^3 (1536^3 padded) FP32 configurations
+through a property that does not depend on the grid size: for a band-limited
+potential whose products stay below a third of the Nyquist wave number, the
+Orszag 3/2 products are exact, so phi2, phi3 and A3 are the same continuous
+fields at every resolution.  The large-grid fields sampled at every 8th
+(512^3) or 16th (1024^3) point must equal the oracle's 64^3 FP64 fields
+(ic_generator.cc:464-547)."""
 import numpy as np
 import pytest
 
@@ -40,12 +41,25 @@ def test_kspace_phi_is_the_cosine_sum(orc):
     assert np.max(np.abs(a[:, :, :n] - f)) < 1e-13
 
 
+@pytest.fixture(scope="module")
+def band_limited_ref(orc):
+    ns = 64
+    r2, r3, rA = orc.lpt_terms(kspace_phi(ns, np.float64), BOX, 3)
+    out = []
+    for r in [r2, r3] + list(rA):
+        ref = r.copy()
+        orc.fft_c2r(ref)
+        out.append(ref[:, :, :ns])
+    return out
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("path", ["streamed", "auto"])
-def test_lpt_sources_1024_band_limited(ctx, orc, path):
+@pytest.mark.parametrize("n,path", [(512, "streamed"), (512, "stored"), (1024, "streamed"), (1024, "auto")])
+def test_lpt_sources_band_limited(ctx, band_limited_ref, n, path):
     import monofonic_amd as M
-    n, ns, step = 1024, 64, 16
-    print(f"band-limited 3LPT sources at 1024^3 FP32 (1536^3 padded), path {path}", flush=True)
+    ns = 64
+    step = n // ns
+    print(f"band-limited 3LPT sources at {n}^3 FP32 ({3 * n // 2}^3 padded), path {path}", flush=True)
     phi = M.Grid_FFT(ctx, n, BOX, "float", initialspace=M.kspace_id)
     phi.upload(kspace_phi(n, np.float32))
     out = [M.Grid_FFT(ctx, n, BOX, "float", initialspace=M.kspace_id) for _ in range(5)]
@@ -55,16 +69,11 @@ def test_lpt_sources_1024_band_limited(ctx, orc, path):
     ctx.synchronize()
     print(f"  sources done on the {conv.lpt_path} path", flush=True)
     del conv
-    r2, r3, rA = orc.lpt_terms(kspace_phi(ns, np.float64), BOX, 3)
-    refs = [r2, r3] + list(rA)
     names = ["phi2", "phi3", "A3x", "A3y", "A3z"]
-    for g, r, name in zip(out, refs, names):
+    for g, ref, name in zip(out, band_limited_ref, names):
         g.FourierTransformBackward()
         got = g.rdata()[::step, ::step, ::step].astype(np.float64)
         g.reset()
-        ref = r.copy()
-        orc.fft_c2r(ref)
-        ref = ref[:, :, :ns]
         scale = np.sqrt(np.mean(ref ** 2))
         err = np.sqrt(np.mean((got - ref) ** 2)) / scale
         mx = np.max(np.abs(got - ref)) / scale

@@ -71,16 +71,27 @@ def test_lpt_sequence_vs_oracle(ctx, orc, n, order):
             assert rel_max(A3[d].kdata(), orc.kview(rA[d])) < 1e-10
 
 
-def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None):
-    """ic_generator::run restated on the CPU: MUSIC1 noise, phi(1..3), A(3),
-    growth scaling and FP64 sc-lattice positions/velocities; with plt_tabs the
-    assembly uses the PLT gradient (ENABLE_PLT builds)"""
+def oracle_noise(orc, n, seed, generator="MUSIC1"):
+    """the white-noise grid after Fill_Grid and wnoise.FourierTransformForward
+    (ic_generator.cc:255-257): NGENIC fills k space directly, so its forward
+    transform is a no-op"""
+    if generator == "NGENIC":
+        return orc.ngenic_fill(n, orc.ngenic_seedtable(n, seed))
+    wn = orc.music1_fill(n, seed) if generator == "MUSIC1" else orc.threefry_fill(n, seed)
+    orc.fft_r2c(wn)
+    return wn
+
+
+def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, generator="MUSIC1"):
+    """ic_generator::run restated on the CPU: white noise (MUSIC1, NGENIC or
+    the Threefry stream), phi(1..3), A(3), growth scaling and FP64 sc-lattice
+    positions/velocities; with plt_tabs the assembly uses the PLT gradient
+    (ENABLE_PLT builds)"""
     a = 1.0 / (1.0 + zstart)
     D = cos.Dplus(a)
     vfac = cos.vfact(a)
     volfac = (boxlen / n / 2.0 / np.pi) ** 1.5
-    wn = orc.music1_fill(n, seed)
-    orc.fft_r2c(wn)
+    wn = oracle_noise(orc, n, seed, generator)
     orc.noise_normalise(wn, volfac)
     phi = np.zeros_like(wn)
     orc.phi1(phi, wn, boxlen, cos.amplitude_table(n, boxlen))
