@@ -230,6 +230,16 @@ int mfx_set_particles(mfx_grid *field, double *out_dev, int idim, double lunit, 
  * The grid size is a power of two; the cube size (the grid size if that is
  * smaller) must divide it and be at most 1024, else an error is returned. */
 int mfx_rng_music1_fill(mfx_grid *g, long seed, int cubesize);
+/* No counterpart in the reference.  The MUSIC1 fill above for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc/hipFree and creates
+ * no events; its seed and scratch (mean, per-cube sums) live in a 1 MiB
+ * file-scope device array, one per GPU, held until the process ends.  At most
+ * 65520 cubes ((n/cubesize)^3, i.e. up to 1280^3 at cube size 32), else an
+ * error is returned.  Asynchronous on the grid's context stream (synchronise
+ * before reading the grid on the host).  Every call on one GPU shares that
+ * GPU's array: calls on different streams of one GPU that overlap race. */
+int mfx_rng_music1_fill_noalloc(mfx_grid *g, long seed, int cubesize);
 /* NGENIC (src/plugins/random_ngenic.cc:40-75 seed table, :85-152
  * Fill_Grid): k-space noise, leaves the grid in k space */
 int mfx_rng_ngenic_fill(mfx_grid *g, long seed);

@@ -28,7 +28,7 @@ EXPORTS = [
     "mfx_conv_hessians3", "mfx_conv_sum_of_hessians", "mfx_conv_difference_of_hessians",
     "mfx_conv_multiply_field", "mfx_lpt_sources", "mfx_conv_set_lpt_path", "mfx_conv_lpt_path", "mfx_plt_create", "mfx_plt_destroy", "mfx_plt_tables",
     "mfx_plt_dmatrix", "mfx_plt_gradient", "mfx_assemble_kspace_plt", "mfx_assemble_kspace", "mfx_set_particles",
-    "mfx_rng_music1_fill", "mfx_rng_ngenic_fill", "mfx_rng_threefry_fill", "mfx_rng_mt19937_raw",
+    "mfx_rng_music1_fill", "mfx_rng_music1_fill_noalloc", "mfx_rng_ngenic_fill", "mfx_rng_threefry_fill", "mfx_rng_mt19937_raw",
     "mfx_rng_ranlxd1_raw", "mfx_rng_threefry_uniforms", "mfx_cosmo_create", "mfx_cosmo_destroy",
     "mfx_cosmo_query", "mfx_cosmo_amplitude_table", "mfx_icgen_create", "mfx_icgen_destroy", "mfx_icgen_run",
     "mfx_icgen_set_seed", "mfx_icgen_capture", "mfx_icgen_launch",
@@ -114,6 +114,7 @@ def load():
         "mfx_plt_gradient": [vp, sz, dbl, vp, i32, vp, vp],
         "mfx_set_particles": [vp, vp, i32, dbl, i32],
         "mfx_rng_music1_fill": [vp, ctypes.c_long, i32],
+        "mfx_rng_music1_fill_noalloc": [vp, ctypes.c_long, i32],
         "mfx_rng_ngenic_fill": [vp, ctypes.c_long],
         "mfx_rng_threefry_fill": [vp, ctypes.c_ulonglong],
         "mfx_rng_mt19937_raw": [vp, vp, i32, i32, vp],

@@ -140,6 +140,7 @@ int mfx_ctx_create(int device, mfx_ctx** out) {
     MFX_HIP(hipGetDeviceCount(&ndev));
     MFX_REQUIRE(device >= 0 && device < ndev, "invalid HIP device index");
     MFX_HIP(hipSetDevice(device));
+    music1_noalloc_init(device);
     auto* c = new mfx_ctx();
     c->c.device = device;
     MFX_HIP(hipStreamCreate(&c->c.stream));
@@ -965,6 +966,19 @@ int mfx_rng_music1_fill(mfx_grid* g, long seed, int cubesize) {
     MFX_HIP(hipStreamSynchronize(c.stream));
     c.release(ds);
     c.release(scr);
+  });
+}
+
+int mfx_rng_music1_fill_noalloc(mfx_grid* g, long seed, int cubesize) {
+  return guard([&] {
+    require_grid(g);
+    check_music1(g, seed, cubesize);
+    Ctx& c = g->ctx->c;
+    if (g->prec == MFX_FLOAT)
+      music1_fill_noalloc<float>(c, (float*)g->d, (int)g->n[0], seed, cubesize);
+    else
+      music1_fill_noalloc<double>(c, (double*)g->d, (int)g->n[0], seed, cubesize);
+    g->space = MFX_RSPACE;
   });
 }
 

@@ -60,7 +60,14 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
 // they can be captured into a graph and replayed with a new seed.
 size_t music1_scratch_bytes(int n, int cubesize);
 template <typename T>
-void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int zeromean, void* scratch);
+void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int zeromean, void* scratch,
+                 bool scoped = true);
+// the same fill with its seed and scratch in a file-scope device array
+// (allocation-free; music1_noalloc_init sets it up once per device)
+void music1_noalloc_init(int device);
+size_t music1_noalloc_limit();
+template <typename T>
+void music1_fill_noalloc(Ctx& c, T* data, int n, long seed, int cubesize);
 template <typename T>
 void threefry_fill(Ctx& c, T* data, int n, const unsigned long long* dseed);
 void threefry_uniforms(Ctx& c, double* out, int ncells, const unsigned long long* cells /*l,j1,j2,j3*/,

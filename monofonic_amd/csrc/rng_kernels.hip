@@ -23,6 +23,8 @@
 #include "ops.hpp"
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #pragma clang fp contract(off)
 
@@ -265,8 +267,11 @@ size_t music1_scratch_bytes(int n, int cubesize) {
 // Stream-ordered and allocation-free: the seed is read from device memory and
 // the per-cube partial sums go to `scratch` (music1_scratch_bytes), so the
 // fill can be captured into a graph and replayed with a new seed.
+// With scoped = false no per-kernel profile events are recorded (the
+// allocation-free entry point creates no events).
 template <typename T>
-void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int zeromean, void* scratch) {
+void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int zeromean, void* scratch,
+                 bool scoped) {
   int nc, cs;
   music1_cubes(n, cubesize, nc, cs);
   const long long ncube = (long long)nc * nc * nc;
@@ -274,6 +279,13 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
   while ((1 << lcs) < cs) ++lcs;
   double* mean = (double*)scratch;
   dd* part = (dd*)((char*)scratch + 256);
+  const bool prof = c.profile;
+  if (!scoped) c.profile = false;
+  struct Restore {
+    Ctx& c;
+    bool p;
+    ~Restore() { c.profile = p; }
+  } restore{c, prof};
   {
     LaunchScope ls(c, "music1_cubes", (double)n * n * n * sizeof(T));
     if (zeromean == 2)
@@ -618,8 +630,59 @@ void ngenic_fill(Ctx& c, cx<T>* data, int n, const unsigned* table) {
   check_launch("ngenic_fill");
 }
 
-template void music1_fill<float>(Ctx&, float*, int, const long*, int, int, void*);
-template void music1_fill<double>(Ctx&, double*, int, const long*, int, int, void*);
+// ------------------------------------------------------------------ allocation-free MUSIC1 fill
+// For callers that cannot allocate: the seed and the fill's scratch (mean and
+// per-cube double-double sums) live in one file-scope device array.  Every
+// GPU holds its own copy, at its own address, until the process ends.  Layout:
+// the seed at byte 0, the fill's scratch (music1_scratch_bytes) from byte 256.
+constexpr size_t kNoAllocScratchBytes = (size_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static void* noalloc_addr[kMaxDevices];  // device address of `scratch` per device
+static std::mutex noalloc_mu;
+
+// Looks up the device address of `scratch` on `device`, which must be the
+// current device.  Called once per device by context creation.
+void music1_noalloc_init(int device) {
+  MFX_REQUIRE(device >= 0 && device < kMaxDevices, "device index beyond the allocation-free scratch table");
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (noalloc_addr[device]) return;
+  void* p = nullptr;
+  MFX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  MFX_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the allocation-free scratch");
+  noalloc_addr[device] = p;
+}
+
+size_t music1_noalloc_limit() { return kNoAllocScratchBytes - 256; }
+
+__global__ void k_put_seed(long* dst, long seed) { *dst = seed; }
+
+// Stream-ordered; nothing in the scratch has to start at zero (every per-cube
+// sum and the mean are written before they are read).
+template <typename T>
+void music1_fill_noalloc(Ctx& c, T* data, int n, long seed, int cubesize) {
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (c.device >= 0 && c.device < kMaxDevices) base = noalloc_addr[c.device];
+  }
+  MFX_REQUIRE(base != nullptr, "allocation-free scratch not set up on this device");
+  const size_t need = music1_scratch_bytes(n, cubesize);
+  MFX_REQUIRE(need <= music1_noalloc_limit(),
+              "MUSIC1 allocation-free fill: " + std::to_string(need) + " bytes of scratch needed for " +
+                  std::to_string(n) + "^3 with cube size " + std::to_string(cubesize) + ", at most " +
+                  std::to_string(music1_noalloc_limit()) + " available (at most 65520 cubes)");
+  long* dseed = (long*)base;
+  hipLaunchKernelGGL(k_put_seed, dim3(1), dim3(1), 0, c.stream, dseed, seed);
+  check_launch("music1_put_seed");
+  music1_fill<T>(c, data, n, dseed, cubesize, 1, (char*)base + 256, false);
+}
+
+template void music1_fill<float>(Ctx&, float*, int, const long*, int, int, void*, bool);
+template void music1_fill<double>(Ctx&, double*, int, const long*, int, int, void*, bool);
+template void music1_fill_noalloc<float>(Ctx&, float*, int, long, int);
+template void music1_fill_noalloc<double>(Ctx&, double*, int, long, int);
 template void threefry_fill<float>(Ctx&, float*, int, const unsigned long long*);
 template void threefry_fill<double>(Ctx&, double*, int, const unsigned long long*);
 template void ngenic_fill<float>(Ctx&, cx<float>*, int, const unsigned*);
