@@ -72,12 +72,15 @@ int mfx_memcpy_h2d(mfx_ctx *c, void *dev, const void *host, size_t bytes);
 
 /* ------------------------------------------------------------ autotuning
  * No counterpart in the reference (FFTW plans are made with FFTW_ESTIMATE,
- * src/grid_fft.cc:53-54).  Times the tile-width candidates of the column
- * passes of lengths n and 3n/2 in `precision` and keeps the faster per
- * length and direction in the context; results never depend on the choice. */
+ * src/grid_fft.cc:53-54).  Times the column-pass variants of lengths n and
+ * 3n/2 in `precision` -- tile width (widest feasible or half) and plain or
+ * persistent pipelined kernel -- and keeps the fastest per length and
+ * direction in the context; results never depend on the choice. */
 int mfx_autotune(mfx_ctx *c, size_t n, int precision);
-/* set the tile width of column passes of length L explicitly (the widest
- * feasible tile or half of it) */
+/* set the column-pass variant of length L explicitly: the tile width (the
+ * widest feasible tile or half of it), or'ed with MFX_COL_PIPELINED for the
+ * persistent pipelined kernel */
+#define MFX_COL_PIPELINED 0x100
 int mfx_set_column_tile(mfx_ctx *c, int L, int precision, int inverse, int tile);
 /* text table of the decisions taken so far */
 int mfx_autotune_report(mfx_ctx *c, char *buf, size_t len);
@@ -264,6 +267,7 @@ int mfx_rng_threefry_uniforms(mfx_ctx *c, const unsigned long long *cells, int n
 #define MFX_COSMO_H 4
 #define MFX_COSMO_AMPLITUDE 5
 #define MFX_COSMO_PNORM 6
+#define MFX_COSMO_OMEGA_DE 7 /* cosmo_param_["Omega_DE"] (cosmology_parameters.hh) */
 int mfx_cosmo_create(double H0, double Omega_m, double Omega_b, double n_s, double sigma_8, double Tcmb,
                      int zero_radiation, mfx_cosmo **out);
 int mfx_cosmo_destroy(mfx_cosmo *cc);
@@ -291,6 +295,14 @@ typedef struct {
   int assemble_each;    /* 1: assemble and transform one component at a time in one grid
                            (least memory); 0: all six components from one read of the
                            potentials when device memory allows */
+  int field_output;     /* 1: generic (field_lagrangian) output (output_generic.cc:62-67):
+                           arrays 0..2 hold the displacements Psi without the lattice
+                           coordinate (DM_dx..DM_dz) and array 6 the 1st-order density
+                           (DM_delta, ic_generator.cc:945-955); 0: particle positions q + Psi */
+  int own_particles;    /* 1: the output arrays get their own allocation, so that a download
+                           of one realisation (mfx_icgen_download_async) overlaps the next
+                           realisation's noise, LPT and assembly stages; 0: they share the
+                           LPT workspace when it is large enough (least memory) */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;
@@ -309,11 +321,27 @@ int mfx_icgen_run(mfx_icgen *ic);
 int mfx_icgen_set_seed(mfx_icgen *ic, long seed);
 int mfx_icgen_capture(mfx_icgen *ic);
 int mfx_icgen_launch(mfx_icgen *ic);
-/* device pointers of the FP64 particle arrays: what 0..2 positions x,y,z,
- * 3..5 velocities x,y,z (n^3 doubles each, order (i,j,k)).  With the fused
- * LPT engine they share device memory with its workspace: valid until the
+/* device pointers of the FP64 particle arrays: what 0..2 positions x,y,z
+ * (displacements with field_output), 3..5 velocities x,y,z, 6 the density
+ * (field_output only) (n^3 doubles each, order (i,j,k)).  With the fused
+ * LPT engine they may share device memory with its workspace: valid until the
  * next mfx_icgen_run */
 int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
+/* Asynchronous download of the output arrays of the last realisation: bit w
+ * of `what` copies array w (0..5 as mfx_icgen_particles, 6 the density of
+ * field output, n^3 doubles each) into host buffer dst[w] on the
+ * generator's copy stream, after the realisation's last kernel.  Host
+ * buffers from mfx_host_alloc (pinned) make the copy asynchronous.  The next
+ * mfx_icgen_run waits for these copies before its first kernel that
+ * overwrites the arrays (the assembly, with own_particles; its first kernel
+ * otherwise); mfx_icgen_launch waits before the replay.  No counterpart in
+ * the reference, whose output plugins read host memory (output_plugin.hh). */
+int mfx_icgen_download_async(mfx_icgen *ic, unsigned what, void *const dst[7]);
+/* blocks until the copies of mfx_icgen_download_async are complete */
+int mfx_icgen_download_wait(mfx_icgen *ic);
+/* pinned host memory (hipHostMalloc) for the downloads */
+int mfx_host_alloc(size_t bytes, void **ptr);
+int mfx_host_free(void *ptr);
 /* grids of the realisation: 0 wnoise, 1 phi, 2 phi2, 3 phi3, 4..6 A3, 7 tmp.
  * After a run the noise holds the k-space field before its normalisation
  * (ic_generator.cc:341-346 normalises in place; here the normalisation is

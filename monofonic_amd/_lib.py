@@ -32,7 +32,8 @@ EXPORTS = [
     "mfx_rng_ranlxd1_raw", "mfx_rng_threefry_uniforms", "mfx_cosmo_create", "mfx_cosmo_destroy",
     "mfx_cosmo_query", "mfx_cosmo_amplitude_table", "mfx_icgen_create", "mfx_icgen_destroy", "mfx_icgen_run",
     "mfx_icgen_set_seed", "mfx_icgen_capture", "mfx_icgen_launch",
-    "mfx_icgen_particles", "mfx_icgen_grid", "mfx_icgen_scalars",
+    "mfx_icgen_particles", "mfx_icgen_grid", "mfx_icgen_scalars", "mfx_icgen_download_async",
+    "mfx_icgen_download_wait", "mfx_host_alloc", "mfx_host_free",
 ]
 
 
@@ -54,7 +55,7 @@ class IcgenParams(ctypes.Structure):
                 ("n_s", ctypes.c_double), ("sigma_8", ctypes.c_double), ("Tcmb", ctypes.c_double),
                 ("zero_radiation", ctypes.c_int), ("use_fused_lpt", ctypes.c_int), ("use_plt", ctypes.c_int),
                 ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int),
-                ("assemble_each", ctypes.c_int)]
+                ("assemble_each", ctypes.c_int), ("field_output", ctypes.c_int), ("own_particles", ctypes.c_int)]
 
 
 def load():
@@ -127,6 +128,8 @@ def load():
         "mfx_icgen_create": [vp, vp, pp], "mfx_icgen_destroy": [vp], "mfx_icgen_run": [vp],
         "mfx_icgen_set_seed": [vp, ctypes.c_long], "mfx_icgen_capture": [vp], "mfx_icgen_launch": [vp],
         "mfx_icgen_particles": [vp, i32, pp], "mfx_icgen_grid": [vp, i32, pp],
+        "mfx_icgen_download_async": [vp, ctypes.c_uint, ctypes.POINTER(vp)], "mfx_icgen_download_wait": [vp],
+        "mfx_host_alloc": [sz, pp], "mfx_host_free": [vp],
         "mfx_icgen_scalars": [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)],
     }
     for name, args in sig.items():

@@ -388,7 +388,7 @@ def threefry_uniforms(ctx, cells, seed, count):
 # ------------------------------------------------------------------ cosmology (host FP64)
 class CosmologyCalculator:
     """cosmology::calculator with the Eisenstein & Hu transfer function"""
-    TRANSFER, DPLUS, F, VFACT, H, AMPLITUDE, PNORM = range(7)
+    TRANSFER, DPLUS, F, VFACT, H, AMPLITUDE, PNORM, OMEGA_DE = range(8)
 
     def __init__(self, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96, sigma_8=0.81, Tcmb=2.7255,
                  zero_radiation=False):
@@ -430,6 +430,9 @@ class CosmologyCalculator:
     @property
     def pnorm(self):
         return self._q(self.PNORM)
+
+    def omega_de(self):
+        return self._q(self.OMEGA_DE)
 
     def amplitude_table(self, n, boxlen):
         nt = 3 * (n // 2) ** 2 + 1
@@ -516,9 +519,14 @@ class ICGenerator:
     def __init__(self, ctx, GridRes, BoxLength, zstart, LPTorder=3, precision="double", generator="MUSIC1",
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
-                 autotune=True, assemble_each=False):
+                 autotune=True, assemble_each=False, field_output=False, own_particles=False):
         self.ctx = ctx
         self.n = int(GridRes)
+        self.boxlen, self.zstart, self.lpt_order = float(BoxLength), float(zstart), int(LPTorder)
+        self.cosmo = dict(H0=H0, Omega_m=Omega_m, Omega_b=Omega_b)
+        self.generator, self.plt = generator, bool(PLT)
+        self.field_output = bool(field_output)
+        self._host = [None] * 7
         p = _lib.IcgenParams()
         p.n, p.boxlen, p.zstart, p.lpt_order = self.n, float(BoxLength), float(zstart), int(LPTorder)
         p.precision, p.generator, p.seed = _prec(precision), GENERATORS[generator], int(seed)
@@ -528,6 +536,7 @@ class ICGenerator:
         p.use_plt, p.plt_ngrid = int(PLT), int(plt_ngrid)
         p.autotune = int(autotune)
         p.assemble_each = int(assemble_each)
+        p.field_output, p.own_particles = int(field_output), int(own_particles)
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 
@@ -535,8 +544,77 @@ class ICGenerator:
         try:
             if self._h:
                 call("mfx_icgen_destroy", self._h)
+                self._h = ctypes.c_void_p()
+            for b in self._host:
+                if b is not None:
+                    call("mfx_host_free", ctypes.c_void_p(b[0]))
+            self._host = [None] * 7
         except Exception:
             pass
+
+    def _pinned(self, w):
+        """pinned host buffer of output array w (n^3 doubles), as a numpy view"""
+        if self._host[w] is None:
+            nb = self.n ** 3 * 8
+            ptr = ctypes.c_void_p()
+            call("mfx_host_alloc", nb, ctypes.byref(ptr))
+            arr = np.ctypeslib.as_array((ctypes.c_double * (self.n ** 3)).from_address(ptr.value))
+            self._host[w] = (ptr.value, arr.reshape(self.n, self.n, self.n))
+        return self._host[w][1]
+
+    def download_async(self, what=range(6)):
+        """start copying output arrays `what` (0..5, 6 = density of the field
+        output) of the last realisation into pinned host buffers; the next
+        run overlaps these copies until it overwrites the arrays"""
+        mask = 0
+        dst = (ctypes.c_void_p * 7)()
+        for w in what:
+            self._pinned(w)
+            dst[w] = self._host[w][0]
+            mask |= 1 << w
+        call("mfx_icgen_download_async", self._h, mask, dst)
+
+    def download_wait(self):
+        """wait for download_async; returns the pinned host arrays (views
+        that the next download overwrites)"""
+        call("mfx_icgen_download_wait", self._h)
+        return [h[1] if h is not None else None for h in self._host]
+
+    def write_generic(self, fname, fields=None):
+        """generic HDF5 output (src/plugins/output_generic.cc): DM_dx..DM_vz
+        and, with field_output, DM_delta, from the downloaded arrays (field
+        output only: displacements without the lattice, lunit = vunit = 1)"""
+        from .io import GenericOutput
+        if not self.field_output:
+            raise ValueError("generic output needs field_output=True")
+        arrs = fields if fields is not None else self.download_wait()
+        names = ["DM_dx", "DM_dy", "DM_dz", "DM_vx", "DM_vy", "DM_vz", "DM_delta"]
+        astart = 1.0 / (1.0 + self.zstart)
+        Om, Ob = self.cosmo["Omega_m"], self.cosmo["Omega_b"]
+        with GenericOutput(fname, self.boxlen, astart, Ob, Om, self._omega_de()) as out:
+            for w, name in enumerate(names):
+                if arrs[w] is not None:
+                    out.write_field(name, np.ascontiguousarray(arrs[w]))
+
+    def write_gadget(self, fname, double_prec=True, long_ids=False, arrays=None):
+        """Gadget-HDF5 particle output (src/plugins/output_gadget_hdf5.cc):
+        positions in Mpc/h (lunit = BoxLength), velocities with vunit =
+        BoxLength / sqrt(a), from the downloaded arrays 0..5"""
+        from .io import GadgetOutput
+        if self.field_output:
+            raise ValueError("particle output needs field_output=False")
+        arrs = arrays if arrays is not None else self.download_wait()
+        a = 1.0 / (1.0 + self.zstart)
+        L = self.boxlen
+        h = self.cosmo["H0"] / 100.0
+        with GadgetOutput(fname, self.zstart, L, self.cosmo["Omega_m"], self._omega_de(), self.cosmo["Omega_b"], h,
+                          self.lpt_order, self.n, double_prec=double_prec, long_ids=long_ids, rng=self.generator,
+                          plt="yes" if self.plt else "no") as out:
+            out.write_particles(arrs[0:3], arrs[3:6], self.n, self.cosmo["Omega_m"], L, L / np.sqrt(a))
+
+    def _omega_de(self):
+        c = CosmologyCalculator(H0=self.cosmo["H0"], Omega_m=self.cosmo["Omega_m"], Omega_b=self.cosmo["Omega_b"])
+        return c.omega_de()
 
     def run(self):
         call("mfx_icgen_run", self._h)
@@ -562,6 +640,11 @@ class ICGenerator:
         p = ctypes.c_void_p()
         call("mfx_icgen_particles", self._h, what, ctypes.byref(p))
         return p
+
+    def density(self):
+        """host copy of the field output's 1st-order density (n,n,n)"""
+        n = self.n
+        return self.ctx.d2h(self.particles_dev(6), n ** 3 * 8, np.float64).reshape(n, n, n)
 
     def particles(self, what):
         """host copy of positions (0..2) or velocities (3..5), shape (n,n,n)"""

@@ -78,6 +78,15 @@ const cx<T>* Ctx::twiddles(int L) {
 template const cx<float>* Ctx::twiddles<float>(int);
 template const cx<double>* Ctx::twiddles<double>(int);
 
+int Ctx::num_cus() {
+  if (cus == 0) {
+    int n = 0;
+    MFX_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device));
+    cus = n > 0 ? n : 1;
+  }
+  return cus;
+}
+
 void* Ctx::alloc(size_t bytes) {
   void* p = nullptr;
   if (bytes == 0) bytes = 16;
@@ -1098,6 +1107,7 @@ int mfx_cosmo_query(const mfx_cosmo* cc, int what, double x, double* out) {
       case MFX_COSMO_H: *out = c.H(x); break;
       case MFX_COSMO_AMPLITUDE: *out = c.amplitude(x); break;
       case MFX_COSMO_PNORM: *out = c.pnorm(); break;
+      case MFX_COSMO_OMEGA_DE: *out = c.Omega_DE(); break;
       default: throw Error(MFX_EINVAL, "unknown cosmology query");
     }
   });
@@ -1149,8 +1159,10 @@ int mfx_set_column_tile(mfx_ctx* c, int L, int precision, int inverse, int tile)
     MFX_REQUIRE(column_length_supported(L), "unsupported column length");
     MFX_REQUIRE(precision == MFX_FLOAT || precision == MFX_DOUBLE, "invalid precision");
     const int cm = column_tile_max(L, precision);
-    MFX_REQUIRE(tile == cm || (cm >= 2 && tile == cm / 2),
-                "tile must be " + std::to_string(cm) + (cm >= 2 ? " or " + std::to_string(cm / 2) : std::string()));
+    const int w = tile & ~COL_PIPELINED;
+    MFX_REQUIRE((tile & ~(COL_PIPELINED | 0xff)) == 0 && (w == cm || (cm >= 2 && w == cm / 2)),
+                "tile must be " + std::to_string(cm) + (cm >= 2 ? " or " + std::to_string(cm / 2) : std::string()) +
+                    ", optionally or'ed with MFX_COL_PIPELINED");
     c->c.coltile[std::make_tuple(L, precision, inverse ? 1 : 0)] = tile;
   });
 }
@@ -1160,11 +1172,18 @@ int mfx_autotune_report(mfx_ctx* c, char* buf, size_t len) {
     MFX_REQUIRE(c && buf && len > 0, "invalid argument");
     std::string out;
     char line[192];
+    auto name = [](int v) {
+      return "C=" + std::to_string(v & ~COL_PIPELINED) + ((v & COL_PIPELINED) ? "p" : "");
+    };
     for (const auto& r : c->c.tune_log) {
-      std::snprintf(line, sizeof line, "L=%d %s %s: C=%d %.4f ms, C=%d %.4f ms -> C=%d\n", r.L,
-                    r.tsize == 4 ? "f32" : "f64", r.inv ? "inverse" : "forward", r.cand[0], r.ms[0], r.cand[1],
-                    r.ms[1], r.chosen);
+      std::snprintf(line, sizeof line, "L=%d %s %s:", r.L, r.tsize == 4 ? "f32" : "f64",
+                    r.inv ? "inverse" : "forward");
       out += line;
+      for (int q = 0; q < r.ncand; ++q) {
+        std::snprintf(line, sizeof line, " %s %.4f ms%s", name(r.cand[q]).c_str(), r.ms[q], q + 1 < r.ncand ? "," : "");
+        out += line;
+      }
+      out += " -> " + name(r.chosen) + "\n";
     }
     std::snprintf(buf, len, "%s", out.c_str());
   });
@@ -1329,6 +1348,14 @@ struct mfx_icgen {
   std::future<void> table_job;       // NGENIC seed table being computed
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  bool capturing = false;
+  // field output: the 1st-order density (n^3 doubles)
+  double* delta = nullptr;
+  // downloads: a copy stream, the end of the last realisation and of the
+  // copies of its arrays
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t run_done = nullptr, copy_done = nullptr;
+  bool copy_pending = false;
 };
 
 static void icgen_free(mfx_icgen* ic) {
@@ -1349,6 +1376,13 @@ static void icgen_free(mfx_icgen* ic) {
   if (ic->plt) mfx_plt_destroy(ic->plt);
   if (ic->gexec) (void)hipGraphExecDestroy(ic->gexec);
   if (ic->graph) (void)hipGraphDestroy(ic->graph);
+  if (ic->delta) ic->ctx->c.release(ic->delta);
+  if (ic->copy_stream) {
+    (void)hipStreamSynchronize(ic->copy_stream);
+    (void)hipStreamDestroy(ic->copy_stream);
+  }
+  if (ic->run_done) (void)hipEventDestroy(ic->run_done);
+  if (ic->copy_done) (void)hipEventDestroy(ic->copy_done);
   if (ic->dev_seed) ic->ctx->c.release(ic->dev_seed);
   if (ic->dev_table) ic->ctx->c.release(ic->dev_table);
   if (ic->rng_scratch) ic->ctx->c.release(ic->rng_scratch);
@@ -1389,7 +1423,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       // FP32 this keeps the realisation within one GPU's HBM); they stay valid
       // until the next run.
       const size_t npart = p->n * p->n * p->n;
-      if (ic->conv && p->use_fused_lpt) {
+      if (ic->conv && p->use_fused_lpt && !p->own_particles) {
         lpt_reserve(ic->conv, p->lpt_order);
         if (ic->conv->lpt_bytes >= 6 * npart * sizeof(double)) {
           ic->part_in_ws = true;
@@ -1411,6 +1445,10 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
         MFX_HIP(hipMemGetInfo(&fr, &tot));
         if (comp_bytes + ((size_t)8 << 30) <= fr) ic->comp6 = c->c.alloc(comp_bytes);
       }
+      if (p->field_output) ic->delta = (double*)c->c.alloc(sizeof(double) * npart);
+      MFX_HIP(hipStreamCreate(&ic->copy_stream));
+      MFX_HIP(hipEventCreateWithFlags(&ic->run_done, hipEventDisableTiming));
+      MFX_HIP(hipEventCreateWithFlags(&ic->copy_done, hipEventDisableTiming));
       const size_t nt = 3 * (p->n / 2) * (p->n / 2) + 1;
       std::vector<double> t(nt);
       amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);
@@ -1510,8 +1548,18 @@ static void icgen_commit_seed(mfx_icgen* ic) {
 // The device sequence of one realisation, from the staged seed to the FP64
 // particles: stream-ordered kernels only (no allocation, host copy, memset or
 // synchronisation once the generator has run once), so it can be captured.
+// a download of the previous realisation's arrays must end before they are
+// overwritten (eager runs; a replay waits before its launch)
+static void icgen_wait_copies(mfx_icgen* ic) {
+  if (ic->copy_pending && !ic->capturing) {
+    MFX_HIP(hipStreamWaitEvent(ic->ctx->c.stream, ic->copy_done, 0));
+    ic->copy_pending = false;
+  }
+}
+
 static void icgen_device_run(mfx_icgen* ic) {
   const mfx_icgen_params& p = ic->p;
+  if (ic->part_in_ws) icgen_wait_copies(ic);
   mfx_grid *wn = ic->g[0], *phi = ic->g[1], *phi2 = ic->g[2], *phi3 = ic->g[3], *tmp = ic->g[7];
   mfx_grid* A3[3] = {ic->g[4], ic->g[5], ic->g[6]};
   const double D = ic->dplus;
@@ -1550,6 +1598,22 @@ static void icgen_device_run(mfx_icgen* ic) {
   auto P = [](mfx_grid* g) { return g ? g->d : nullptr; };
   const double* plt_tab = ic->plt ? ic->plt->tab : nullptr;
   const int plt_ng = ic->plt ? ic->plt->ng : 0;
+  icgen_wait_copies(ic);
+  // output kind per array: positions add the lattice coordinate, the field
+  // output's displacements and the velocities do not
+  auto okind = [&](int w) { return (w < 3 && !p.field_output) ? 0 : 1; };
+  // the field output's density, from the unscaled phi and D+ (tmp is free)
+  auto density = [&] {
+    if (!p.field_output) return;
+    if (tmp->prec == MFX_FLOAT) {
+      density_k<float>(ic->ctx->c, (cx<float>*)tmp->d, (const cx<float>*)phi->d, (int)p.n, p.boxlen, D);
+      fft_c2r_particles<float>(ic->ctx->c, (float*)tmp->d, (int)p.n, (float)norm_fac(tmp), ic->delta, 0, 1.0, 1);
+    } else {
+      density_k<double>(ic->ctx->c, (cx<double>*)tmp->d, (const cx<double>*)phi->d, (int)p.n, p.boxlen, D);
+      fft_c2r_particles<double>(ic->ctx->c, (double*)tmp->d, (int)p.n, norm_fac(tmp), ic->delta, 0, 1.0, 1);
+    }
+    tmp->space = MFX_RSPACE;
+  };
   if (ic->comp6) {
     const long long ws = (long long)p.n * p.n * (p.n / 2 + 1);
     if (tmp->prec == MFX_FLOAT) {
@@ -1559,7 +1623,7 @@ static void icgen_device_run(mfx_icgen* ic) {
                        gr);
       for (int w = 0; w < 6; ++w)
         fft_c2r_particles<float>(ic->ctx->c, (float*)((cx<float>*)ic->comp6 + w * ws), (int)p.n,
-                                 (float)norm_fac(tmp), ic->part[w], w % 3, 1.0, w / 3);
+                                 (float)norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
     } else {
       assemble6<double>(ic->ctx->c, (cx<double>*)ic->comp6, ws, (const cx<double>*)phi->d,
                         (const cx<double>*)P(phi2), (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]),
@@ -1567,8 +1631,9 @@ static void icgen_device_run(mfx_icgen* ic) {
                         ic->vfact, 1.0, plt_tab, plt_ng, gr);
       for (int w = 0; w < 6; ++w)
         fft_c2r_particles<double>(ic->ctx->c, (double*)((cx<double>*)ic->comp6 + w * ws), (int)p.n, norm_fac(tmp),
-                                  ic->part[w], w % 3, 1.0, w / 3);
+                                  ic->part[w], w % 3, 1.0, okind(w));
     }
+    density();
     return;
   }
   for (int kind = 0; kind < 2; ++kind)
@@ -1585,12 +1650,14 @@ static void icgen_device_run(mfx_icgen* ic) {
                          (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, kind, idim, ic->vfact,
                          1.0, plt_tab, plt_ng, gr);
       double* out = ic->part[3 * kind + idim];
+      const int ok = okind(3 * kind + idim);
       if (tmp->prec == MFX_FLOAT)
-        fft_c2r_particles<float>(ic->ctx->c, (float*)tmp->d, (int)p.n, (float)norm_fac(tmp), out, idim, 1.0, kind);
+        fft_c2r_particles<float>(ic->ctx->c, (float*)tmp->d, (int)p.n, (float)norm_fac(tmp), out, idim, 1.0, ok);
       else
-        fft_c2r_particles<double>(ic->ctx->c, (double*)tmp->d, (int)p.n, norm_fac(tmp), out, idim, 1.0, kind);
+        fft_c2r_particles<double>(ic->ctx->c, (double*)tmp->d, (int)p.n, norm_fac(tmp), out, idim, 1.0, ok);
       tmp->space = MFX_RSPACE;
     }
+  density();
 }
 
 extern "C" int mfx_icgen_run(mfx_icgen* ic) {
@@ -1598,6 +1665,7 @@ extern "C" int mfx_icgen_run(mfx_icgen* ic) {
     MFX_REQUIRE(ic, "null generator");
     icgen_commit_seed(ic);
     icgen_device_run(ic);
+    MFX_HIP(hipEventRecord(ic->run_done, ic->ctx->c.stream));
   });
 }
 
@@ -1628,15 +1696,18 @@ extern "C" int mfx_icgen_capture(mfx_icgen* ic) {
     const bool prof = c.profile;
     c.profile = false;
     MFX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    ic->capturing = true;
     try {
       icgen_device_run(ic);
     } catch (...) {
+      ic->capturing = false;
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(c.stream, &g);
       if (g) (void)hipGraphDestroy(g);
       c.profile = prof;
       throw;
     }
+    ic->capturing = false;
     MFX_HIP(hipStreamEndCapture(c.stream, &ic->graph));
     c.profile = prof;
     MFX_HIP(hipGraphInstantiate(&ic->gexec, ic->graph, nullptr, nullptr, 0));
@@ -1647,14 +1718,56 @@ extern "C" int mfx_icgen_launch(mfx_icgen* ic) {
   return guard([&] {
     MFX_REQUIRE(ic && ic->gexec, "generator has no captured graph (mfx_icgen_capture)");
     icgen_commit_seed(ic);
+    icgen_wait_copies(ic);
     MFX_HIP(hipGraphLaunch(ic->gexec, ic->ctx->c.stream));
+    MFX_HIP(hipEventRecord(ic->run_done, ic->ctx->c.stream));
   });
 }
 
 extern "C" int mfx_icgen_particles(mfx_icgen* ic, int what, double** dev_ptr) {
   return guard([&] {
-    MFX_REQUIRE(ic && dev_ptr && what >= 0 && what < 6, "invalid argument");
-    *dev_ptr = ic->part[what];
+    MFX_REQUIRE(ic && dev_ptr && what >= 0 && what <= 6, "invalid argument");
+    MFX_REQUIRE(what < 6 || ic->delta, "the density is written with field_output only");
+    *dev_ptr = what < 6 ? ic->part[what] : ic->delta;
+  });
+}
+
+extern "C" int mfx_icgen_download_async(mfx_icgen* ic, unsigned what, void* const dst[7]) {
+  return guard([&] {
+    MFX_REQUIRE(ic && dst, "null argument");
+    MFX_REQUIRE((what >> 7) == 0, "what: bits 0..6 only");
+    MFX_REQUIRE(!(what & 64u) || ic->delta, "the density is written with field_output only");
+    const size_t bytes = sizeof(double) * ic->p.n * ic->p.n * ic->p.n;
+    MFX_HIP(hipStreamWaitEvent(ic->copy_stream, ic->run_done, 0));
+    for (int w = 0; w < 7; ++w)
+      if (what & (1u << w)) {
+        MFX_REQUIRE(dst[w], "null destination buffer");
+        MFX_HIP(hipMemcpyAsync(dst[w], w < 6 ? ic->part[w] : ic->delta, bytes, hipMemcpyDeviceToHost,
+                               ic->copy_stream));
+      }
+    MFX_HIP(hipEventRecord(ic->copy_done, ic->copy_stream));
+    ic->copy_pending = true;
+  });
+}
+
+extern "C" int mfx_icgen_download_wait(mfx_icgen* ic) {
+  return guard([&] {
+    MFX_REQUIRE(ic, "null generator");
+    MFX_HIP(hipEventSynchronize(ic->copy_done));
+  });
+}
+
+extern "C" int mfx_host_alloc(size_t bytes, void** ptr) {
+  return guard([&] {
+    MFX_REQUIRE(ptr, "null argument");
+    *ptr = nullptr;
+    MFX_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  });
+}
+
+extern "C" int mfx_host_free(void* ptr) {
+  return guard([&] {
+    if (ptr) MFX_HIP(hipHostFree(ptr));
   });
 }
 

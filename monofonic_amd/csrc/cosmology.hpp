@@ -25,6 +25,7 @@ class Cosmology {
   double vfact(double a) const;
   double amplitude(double k) const;
   double pnorm() const { return pnorm_; }
+  double Omega_DE() const { return Omega_DE_; }
   double sigma8_unnormalised() const;
 
  private:

@@ -130,12 +130,14 @@ void conv2(Ctx& c, const cx<T>* fa, const MultSpec& ma, const cx<T>* fb, const M
 }
 
 // ------------------------------------------------------------------ autotuning
-// Column-pass tile width: the widest tile whose exchange area fits the LDS
+// Column-pass variant: the widest tile whose exchange area fits the LDS
 // budget keeps 8-16 adjacent columns per line access (wider HBM segments),
-// half of it doubles the workgroups per CU.  Which is faster depends on the
-// length and precision, so both are timed on a block of the library's own
-// shape -- L/2+1 adjacent columns, up to 256 batches -- in both directions,
-// five launches each after two warm-ups, and the faster is kept.
+// half of it doubles the workgroups per CU; the persistent pipelined kernel
+// keeps the next tile's loads in flight while a tile is transformed.  Which
+// is faster depends on the length and precision, so the (up to) four
+// combinations are timed on a block of the library's own shape -- L/2+1
+// adjacent columns, up to 256 batches -- in both directions, five launches
+// each after two warm-ups, and the fastest is kept.
 template <typename T>
 void autotune_columns(Ctx& c, int L) {
   const int cm = col_tile_max<T>(L);
@@ -152,8 +154,10 @@ void autotune_columns(Ctx& c, int L) {
     MFX_HIP(hipEventCreate(&e1));
     const long long sb = (long long)L * ncols, sp = ncols;
     for (int inv = 0; inv < 2; ++inv) {
-      TuneRec rec{L, (int)sizeof(T), inv, {cm, cm / 2}, {0.0, 0.0}, cm};
-      const int ncand = cm >= 2 ? 2 : 1;
+      TuneRec rec{L, (int)sizeof(T), inv, 0, {}, {}, cm};
+      for (int pipe = 0; pipe < 2; ++pipe)
+        for (int h = 0; h < (cm >= 2 ? 2 : 1); ++h) rec.cand[rec.ncand++] = (h ? cm / 2 : cm) | (pipe ? COL_PIPELINED : 0);
+      const int ncand = rec.ncand;
       double best = 1e300;
       for (int q = 0; q < ncand; ++q) {
         c.coltile[std::make_tuple(L, (int)sizeof(T), inv)] = rec.cand[q];

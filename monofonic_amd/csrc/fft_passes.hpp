@@ -247,6 +247,62 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
   }
 }
 
+// Persistent, software-pipelined variant of k_col: gridDim.x workgroups (a
+// multiple of 8, as many as are resident at once) each walk a strided subset
+// of the work items (batch, tile), and the loads of a workgroup's next item
+// are issued before the transform and stores of its current one, so that a
+// workgroup keeps a tile's worth of loads in flight while it computes.  Work
+// items are split into 8 contiguous ranges, one per XCD (workgroups are dealt
+// to the XCDs round-robin), and the workgroups of an XCD interleave within its
+// range, so that the tiles in flight on one XCD are neighbours in memory.
+__device__ __forceinline__ bool pipe_range(long long nitems, long long& w, long long& hi, int& step) {
+  constexpr int NX = 8;
+  const int xcd = blockIdx.x % NX, slot = blockIdx.x / NX;
+  step = gridDim.x / NX;
+  const long long per = nitems / NX, rem = nitems % NX;
+  const long long lo = xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per;
+  hi = lo + per + (xcd < rem ? 1 : 0);
+  w = lo + slot;
+  return w < hi;
+}
+
+template <typename T, int L, bool INV, class Src, class Dst, int C>
+__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
+    k_col_pipe(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
+  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  __shared__ cx<T> lds[L * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  long long w, hi;
+  int step;
+  if (!pipe_range(nitems, w, hi, step)) return;
+  cx<T> nx[EPT];
+  auto fetch = [&](long long item, cx<T>(&v)[EPT]) {
+    const int b = (int)(item / ntiles), col = (int)(item % ntiles) * C + c;
+    const int colc = col < ncols ? col : ncols - 1;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
+  };
+  fetch(w, nx);
+  for (; w < hi; w += step) {
+    cx<T> v[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) v[e] = nx[e];
+    if (w + step < hi) fetch(w + step, nx);
+    // every stage exchange of the previous item ended with a barrier after
+    // its last LDS read, so the first exchange of this one may write
+    fft_exec<T, L, INV>(v, lds + c, C, t, tw);
+    const int b = (int)(w / ntiles), col = (int)(w % ntiles) * C + c;
+    const int colc = col < ncols ? col : ncols - 1;
+    cx<T> old[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) old[e] = dst.load_old(b, colc, t + TPS * e);
+    if (col < ncols) {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) dst.store(b, col, t + TPS * e, v[e], old[e]);
+    }
+  }
+}
+
 // The table pointer as a value the compiler cannot see through: a second
 // transform in one kernel then loads its twiddles again rather than keeping
 // the first transform's twiddles in registers across it.
@@ -308,31 +364,110 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
   }
 }
 
+// persistent, software-pipelined k_col_turn (see k_col_pipe)
+template <typename T, int L, int C>
+__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
+    k_col_turn_pipe(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
+  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  __shared__ cx<T> lds[L * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  const int nh = a.n >> 1;
+  long long w, hi;
+  int step;
+  if (!pipe_range(nitems, w, hi, step)) return;
+  cx<T> nx[EPT];
+  auto fetch = [&](long long item, cx<T>(&v)[EPT]) {
+    const int b = (int)(item / ntiles), col = (int)(item % ntiles) * C + c;
+    const int colc = col < ncols ? col : ncols - 1;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) v[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
+  };
+  fetch(w, nx);
+  for (; w < hi; w += step) {
+    cx<T> x[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) x[e] = nx[e];
+    if (w + step < hi) fetch(w + step, nx);
+    const int b = (int)(w / ntiles), col = (int)(w % ntiles) * C + c;
+    const int colc = col < ncols ? col : ncols - 1;
+    fft_exec<T, L, false>(x, lds + c, C, t, tw);
+    const bool nyqc = colc == nh;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int pos = t + TPS * e;
+      const bool zero = ((pos > nh) & (pos <= a.n)) | (pos == nh) | nyqc;
+      const cx<T> o = x[e] * a.fscale;
+      x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
+    }
+    __syncthreads();  // the forward transform's exchange reads are done
+    fft_exec<T, L, true>(x, lds + c, C, t, opaque(tw));
+    if (col < ncols) {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(t + TPS * e) * a.osp + col] = x[e];
+    }
+  }
+}
+
 #define MFX_COL_LENGTHS(X) \
   X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(768) X(1024) X(1536)
 #define MFX_ROW_LENGTHS(X) \
   X(8) X(12) X(16) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(768)
 
-template <typename T, int L, int C, bool INV, class Src, class Dst>
+// Workgroups of a persistent kernel: as many as are resident at once on the
+// device (occupancy x compute units), a multiple of the 8 XCDs, at most the
+// number of work items rounded up to 8.
+template <class K>
+unsigned pipe_grid(Ctx& c, K kernel, int block, long long nitems) {
+  int& per_cu = c.occupancy[(const void*)kernel];  // one query per kernel
+  if (per_cu == 0) {
+    int n = 0;
+    MFX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, 0));
+    per_cu = n > 0 ? n : 1;
+  }
+  long long g = (long long)per_cu * c.num_cus();
+  const long long cap = (nitems + 7) / 8 * 8;
+  if (g > cap) g = cap;
+  return (unsigned)((g + 7) / 8 * 8);
+}
+
+template <typename T, int L, int C, bool INV, bool PIPE, class Src, class Dst>
 void launch_col_LC(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
-  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
+  const dim3 block(C * PlanInfo<L>::TPS);
+  const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst, C>), grid, block, 0, c.stream, s, d, c.twiddles<T>(L), ncols);
+  if constexpr (PIPE) {
+    const long long nitems = (long long)ntiles * nbatch;
+    auto k = k_col_pipe<T, L, INV, Src, Dst, C>;
+    hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems)), block, 0, c.stream, s, d, c.twiddles<T>(L),
+                       ncols, ntiles, nitems);
+  } else {
+    hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst, C>), dim3(ntiles, nbatch), block, 0, c.stream, s, d,
+                       c.twiddles<T>(L), ncols);
+  }
   check_launch(name);
 }
 
-// Tile width candidates: the widest tile whose exchange area fits 64 KiB of
-// LDS (ColCfg) and half of it; the context's tuning table picks one.
+// Column-pass variants: tile width (the widest tile whose exchange area fits
+// 64 KiB of LDS, ColCfg, or half of it) and plain or persistent-pipelined
+// kernel; the context's tuning table (col_tile) picks one.
 template <typename T, int L, bool INV, class Src, class Dst>
 void launch_col_L(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
   constexpr int CM = ColCfg<T, L>::C;
+  const int v = c.col_tile(L, (int)sizeof(T), INV, CM);
+  const bool pipe = (v & COL_PIPELINED) != 0;
   if constexpr (CM >= 2) {
-    if (c.col_tile(L, (int)sizeof(T), INV, CM) == CM / 2) {
-      launch_col_LC<T, L, CM / 2, INV>(c, s, d, ncols, nbatch, name, bytes);
+    if ((v & ~COL_PIPELINED) == CM / 2) {
+      if (pipe)
+        launch_col_LC<T, L, CM / 2, INV, true>(c, s, d, ncols, nbatch, name, bytes);
+      else
+        launch_col_LC<T, L, CM / 2, INV, false>(c, s, d, ncols, nbatch, name, bytes);
       return;
     }
   }
-  launch_col_LC<T, L, CM, INV>(c, s, d, ncols, nbatch, name, bytes);
+  if (pipe)
+    launch_col_LC<T, L, CM, INV, true>(c, s, d, ncols, nbatch, name, bytes);
+  else
+    launch_col_LC<T, L, CM, INV, false>(c, s, d, ncols, nbatch, name, bytes);
 }
 
 // widest tile for length L, precision T
@@ -383,25 +518,41 @@ void launch_col_pad(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const ch
   }
 }
 
-template <typename T, int L, int C>
+template <typename T, int L, int C, bool PIPE>
 void launch_turn_LC(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
-  dim3 grid((ncols + C - 1) / C, nbatch), block(C * PlanInfo<L>::TPS);
+  const dim3 block(C * PlanInfo<L>::TPS);
+  const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_col_turn<T, L, C>), grid, block, 0, c.stream, a, c.twiddles<T>(L), ncols);
+  if constexpr (PIPE) {
+    const long long nitems = (long long)ntiles * nbatch;
+    auto k = k_col_turn_pipe<T, L, C>;
+    hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems)), block, 0, c.stream, a, c.twiddles<T>(L), ncols,
+                       ntiles, nitems);
+  } else {
+    hipLaunchKernelGGL((k_col_turn<T, L, C>), dim3(ntiles, nbatch), block, 0, c.stream, a, c.twiddles<T>(L), ncols);
+  }
   check_launch(name);
 }
 
-// at the tile width tuned for the inverse transforms of (L, T)
+// at the variant tuned for the inverse transforms of (L, T)
 template <typename T, int L>
 void launch_turn_L(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
   constexpr int CM = ColCfg<T, L>::C;
+  const int v = c.col_tile(L, (int)sizeof(T), true, CM);
+  const bool pipe = (v & COL_PIPELINED) != 0;
   if constexpr (CM >= 2) {
-    if (c.col_tile(L, (int)sizeof(T), true, CM) == CM / 2) {
-      launch_turn_LC<T, L, CM / 2>(c, a, ncols, nbatch, name, bytes);
+    if ((v & ~COL_PIPELINED) == CM / 2) {
+      if (pipe)
+        launch_turn_LC<T, L, CM / 2, true>(c, a, ncols, nbatch, name, bytes);
+      else
+        launch_turn_LC<T, L, CM / 2, false>(c, a, ncols, nbatch, name, bytes);
       return;
     }
   }
-  launch_turn_LC<T, L, CM>(c, a, ncols, nbatch, name, bytes);
+  if (pipe)
+    launch_turn_LC<T, L, CM, true>(c, a, ncols, nbatch, name, bytes);
+  else
+    launch_turn_LC<T, L, CM, false>(c, a, ncols, nbatch, name, bytes);
 }
 
 template <typename T>
@@ -487,7 +638,7 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
               long long nrows, T scale, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
   constexpr int E = RowCtx<T, K>::E;
-  const __amdgpu_buffer_rsrc_t r = row_rsrc(src + rowc * spitch, K * sizeof(cx<T>));
+  const __amdgpu_buffer_rsrc_t r = row_rsrc_u<RowLanes<K>::RPW == 1>(src + rowc * spitch, K * sizeof(cx<T>));
   cx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = buf_load<T>(r, (unsigned)(lane + P_ * e) * sizeof(cx<T>));
@@ -506,7 +657,7 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
   // K packed pairs, and the two padding reals of an in-place row set to zero;
   // an inactive row stores nothing (zero-length resource)
   const unsigned nst = act ? (unsigned)(dpitch >= 2 * K + 2 ? K + 1 : K) : 0u;
-  const __amdgpu_buffer_rsrc_t r = row_rsrc(dst + rowc * dpitch, nst * sizeof(cx<T>));
+  const __amdgpu_buffer_rsrc_t r = row_rsrc_u<RowLanes<K>::RPW == 1>(dst + rowc * dpitch, nst * sizeof(cx<T>));
 #pragma unroll
   for (int e = 0; e <= E; ++e) {
     const int n = lane + P_ * e;
@@ -530,7 +681,8 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
   rc.c2r(v, src + rowc * spitch, hin);
   const int i = (int)(rowc / n), j = (int)(rowc % n);
   const double qrow = (double)(idim == 0 ? i : j) / (double)n * lunit;
-  const __amdgpu_buffer_rsrc_t r = row_rsrc(out + rowc * n, act ? (unsigned)(n * sizeof(double)) : 0u);
+  const __amdgpu_buffer_rsrc_t r =
+      row_rsrc_u<RowLanes<K>::RPW == 1>(out + rowc * n, act ? (unsigned)(n * sizeof(double)) : 0u);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int m = lane + P_ * e;

@@ -113,12 +113,12 @@ struct ZRow {
   // real-space row of a stored Hessian component: K packed pairs per row,
   // rows contiguous; an inactive row stores nothing
   __device__ __forceinline__ void store_real(const cx<T> (&v)[E], cx<T>* p) const {
-    const __amdgpu_buffer_rsrc_t r = row_rsrc(p + roff, act ? (unsigned)(K * sizeof(cx<T>)) : 0u);
+    const __amdgpu_buffer_rsrc_t r = row_rsrc_u<RowLanes<K>::RPW == 1>(p + roff, act ? (unsigned)(K * sizeof(cx<T>)) : 0u);
 #pragma unroll
     for (int e = 0; e < E; ++e) buf_store<T>(v[e], r, (unsigned)(rc.lane + RowCtx<T, K>::P * e) * sizeof(cx<T>));
   }
   __device__ __forceinline__ void load_real(cx<T> (&v)[E], const cx<T>* p) const {
-    const __amdgpu_buffer_rsrc_t r = row_rsrc(p + roff, (unsigned)(K * sizeof(cx<T>)));
+    const __amdgpu_buffer_rsrc_t r = row_rsrc_u<RowLanes<K>::RPW == 1>(p + roff, (unsigned)(K * sizeof(cx<T>)));
 #pragma unroll
     for (int e = 0; e < E; ++e) v[e] = buf_load<T>(r, (unsigned)(rc.lane + RowCtx<T, K>::P * e) * sizeof(cx<T>));
   }

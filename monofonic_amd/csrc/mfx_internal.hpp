@@ -33,10 +33,15 @@ struct Error : std::runtime_error {
 
 // one autotuning decision: candidate tile widths of a column pass and their
 // measured times
+// column-pass variant: the tile width, or'ed with COL_PIPELINED for the
+// persistent software-pipelined kernel
+constexpr int COL_PIPELINED = 0x100;
+
 struct TuneRec {
   int L, tsize, inv;
-  int cand[2];
-  double ms[2];
+  int ncand;
+  int cand[4];
+  double ms[4];
   int chosen;
 };
 
@@ -67,6 +72,9 @@ struct Ctx {
     auto it = coltile.find(std::make_tuple(L, tsize, (int)inv));
     return it == coltile.end() ? cmax : it->second;
   }
+  int cus = 0;  // compute units of the device (queried once)
+  int num_cus();
+  std::map<const void*, int> occupancy;  // resident workgroups per CU, per kernel
   ~Ctx();
   template <typename T>
   const cx<T>* twiddles(int L);

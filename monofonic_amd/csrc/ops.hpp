@@ -47,6 +47,9 @@ void assemble6(Ctx& c, cx<T>* out, long long wstride, const cx<T>* phi, const cx
                double unit, const double* plt_tab, int plt_ng, Growth gr);
 template <typename T>
 void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind);
+// out = dplus |k|^2 phi (the field output's 1st-order density in k space)
+template <typename T>
+void density_k(Ctx& c, cx<T>* out, const cx<T>* phi, int n, double boxlen, double dplus);
 template <typename T>
 void add_const(Ctx& c, T* f, int n0, int n1, int n2, long long pitch, double v);
 template <typename T>

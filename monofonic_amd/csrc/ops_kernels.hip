@@ -83,6 +83,23 @@ __global__ void k_invlap(cx<T>* f, int n, T kfac) {
   f[idx] = (idx == 0) ? mk(T(0), T(0)) : mk(-v.x / k2, -v.y / k2);
 }
 
+// -k^2 phi of a growth-scaled potential phi_s = g1 phi with g1 = -D: the
+// 1st-order SPT density of the field output, tmp = -|k|^2 phi_s = D |k|^2
+// phi (ic_generator.cc:949-954, get_k wave numbers)
+template <typename T>
+__global__ void k_density_k(cx<T>* out, const cx<T>* phi, int n, double kfac, double d) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  int k = (int)(idx % nh);
+  long long ij = idx / nh;
+  int j = (int)(ij % n), i = (int)(ij / n);
+  const double kx = kd_of(i, n, kfac), ky = kd_of(j, n, kfac), kz = kd_of(k, n, kfac);
+  const double f = d * (kx * kx + ky * ky + kz * kz);
+  const cx<T> v = phi[idx];
+  out[idx] = mk(T((double)v.x * f), T((double)v.y * f));
+}
+
 template <typename T>
 __global__ void k_scale(T* f, long long cnt, double s) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,6 +418,15 @@ void assemble6(Ctx& c, cx<T>* out, long long wstride, const cx<T>* phi, const cx
 }
 
 template <typename T>
+void density_k(Ctx& c, cx<T>* out, const cx<T>* phi, int n, double boxlen, double dplus) {
+  const long long cnt = (long long)n * n * (n / 2 + 1);
+  LaunchScope ls(c, "density_k", 2.0 * (double)cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_density_k<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, out, phi, n, 2.0 * M_PI / boxlen,
+                     dplus);
+  check_launch("density_k");
+}
+
+template <typename T>
 void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind) {
   long long cnt = (long long)n * n * n;
   LaunchScope ls(c, "set_particles", (double)cnt * (sizeof(T) + sizeof(double)));
@@ -479,6 +505,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
   template void assemble6<T>(Ctx&, cx<T>*, long long, const cx<T>*, const cx<T>*, const cx<T>*, const cx<T>*,     \
                              const cx<T>*, const cx<T>*, int, double, int, double, double, const double*, int, Growth); \
   template void set_particles<T>(Ctx&, const T*, double*, int, int, double, int);                                 \
+  template void density_k<T>(Ctx&, cx<T>*, const cx<T>*, int, double, double);                                  \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \
   template int power_spectrum<T>(Ctx&, const cx<T>*, int, double, double*, double*, double*, double*, int);

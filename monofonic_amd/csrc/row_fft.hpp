@@ -240,6 +240,23 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* base, uns
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
 
+// The same for a row that every lane of the wave addresses (one row per
+// workgroup, RowLanes<K>::RPW == 1): base and length are read from the first
+// lane, so that the descriptor is built in scalar registers.  A descriptor
+// the compiler cannot prove uniform is kept in vector registers, and every
+// buffer instruction then runs in a loop over its distinct values.
+template <bool UNIFORM>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc_u(const void* base, unsigned bytes) {
+  if constexpr (UNIFORM) {
+    const unsigned long long a = (unsigned long long)base;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    return row_rsrc((const void*)(((unsigned long long)hi << 32) | lo), __builtin_amdgcn_readfirstlane(bytes));
+  } else {
+    return row_rsrc(base, bytes);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
   if constexpr (sizeof(cx<T>) == 8)
@@ -289,7 +306,7 @@ struct RowCtx {
   // the lane's entries k = lane + P*e (e <= E) of a half-spectrum row whose
   // first hin entries are stored; entries k >= hin load as zero
   __device__ __forceinline__ void load_raw(cx<T> (&x)[E + 1], const cx<T>* s, int hin) const {
-    const __amdgpu_buffer_rsrc_t r = row_rsrc(s, (unsigned)hin * sizeof(cx<T>));
+    const __amdgpu_buffer_rsrc_t r = row_rsrc_u<RowLanes<K>::RPW == 1>(s, (unsigned)hin * sizeof(cx<T>));
 #pragma unroll
     for (int e = 0; e <= E; ++e) x[e] = buf_load<T>(r, (unsigned)(lane + P * e) * sizeof(cx<T>));
   }
@@ -361,7 +378,7 @@ struct RowCtx {
     wave_sync();
     // entries k >= hout (all of them for an inactive row) are dropped by the
     // range check of the row's buffer resource
-    const __amdgpu_buffer_rsrc_t r = row_rsrc(d, act ? (unsigned)hout * sizeof(cx<T>) : 0u);
+    const __amdgpu_buffer_rsrc_t r = row_rsrc_u<RowLanes<K>::RPW == 1>(d, act ? (unsigned)hout * sizeof(cx<T>) : 0u);
 #pragma unroll
     for (int e = 0; e <= E; ++e) {
       const int k = lane + P * e;
