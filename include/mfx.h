@@ -339,6 +339,21 @@ int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
 int mfx_icgen_download_async(mfx_icgen *ic, unsigned what, void *const dst[7]);
 /* blocks until the copies of mfx_icgen_download_async are complete */
 int mfx_icgen_download_wait(mfx_icgen *ic);
+/* Batched realisations (no counterpart in the reference, which generates
+ * one realisation per run): the device sequences of `count` generators, of
+ * any mix of grid sizes, captured into ONE HIP graph whose branches run on
+ * min(nstreams, count) concurrent streams (generator i on branch i mod
+ * nstreams), forked from and joined to the batch's own stream.  Creation
+ * runs each generator once.  mfx_batch_launch copies every generator's
+ * staged seed (mfx_icgen_set_seed) and replays the graph: one launch for all
+ * realisations; each generator's results equal mfx_icgen_run with its seed.
+ * mfx_batch_synchronize waits for it.  A generator in a batch cannot be
+ * destroyed before the batch. */
+typedef struct mfx_batch mfx_batch;
+int mfx_batch_create(mfx_icgen *const *gens, int count, int nstreams, mfx_batch **out);
+int mfx_batch_launch(mfx_batch *b);
+int mfx_batch_synchronize(mfx_batch *b);
+int mfx_batch_destroy(mfx_batch *b);
 /* pinned host memory (hipHostMalloc) for the downloads */
 int mfx_host_alloc(size_t bytes, void **ptr);
 int mfx_host_free(void *ptr);

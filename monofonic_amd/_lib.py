@@ -33,7 +33,8 @@ EXPORTS = [
     "mfx_cosmo_query", "mfx_cosmo_amplitude_table", "mfx_icgen_create", "mfx_icgen_destroy", "mfx_icgen_run",
     "mfx_icgen_set_seed", "mfx_icgen_capture", "mfx_icgen_launch",
     "mfx_icgen_particles", "mfx_icgen_grid", "mfx_icgen_scalars", "mfx_icgen_download_async",
-    "mfx_icgen_download_wait", "mfx_host_alloc", "mfx_host_free",
+    "mfx_icgen_download_wait", "mfx_host_alloc", "mfx_host_free", "mfx_batch_create", "mfx_batch_launch",
+    "mfx_batch_synchronize", "mfx_batch_destroy",
 ]
 
 
@@ -130,6 +131,8 @@ def load():
         "mfx_icgen_particles": [vp, i32, pp], "mfx_icgen_grid": [vp, i32, pp],
         "mfx_icgen_download_async": [vp, ctypes.c_uint, ctypes.POINTER(vp)], "mfx_icgen_download_wait": [vp],
         "mfx_host_alloc": [sz, pp], "mfx_host_free": [vp],
+        "mfx_batch_create": [ctypes.POINTER(vp), i32, i32, pp], "mfx_batch_launch": [vp],
+        "mfx_batch_synchronize": [vp], "mfx_batch_destroy": [vp],
         "mfx_icgen_scalars": [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)],
     }
     for name, args in sig.items():

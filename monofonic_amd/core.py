@@ -665,3 +665,31 @@ class ICGenerator:
         a = np.empty(nt.value, dtype=dt)
         call("mfx_grid_download", h, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
         return a.reshape(self.n, self.n, self.n + 2)
+
+
+class Batch:
+    """many realisations (any mix of grid sizes) as one captured HIP graph:
+    launch() replays every generator's realisation for its staged seed"""
+
+    def __init__(self, generators, nstreams=8):
+        self.gens = list(generators)
+        arr = (ctypes.c_void_p * len(self.gens))(*[g._h.value for g in self.gens])
+        self._h = ctypes.c_void_p()
+        call("mfx_batch_create", arr, len(self.gens), int(nstreams), ctypes.byref(self._h))
+
+    def launch(self):
+        call("mfx_batch_launch", self._h)
+
+    def synchronize(self):
+        call("mfx_batch_synchronize", self._h)
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, ctypes.c_void_p()
+            call("mfx_batch_destroy", h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

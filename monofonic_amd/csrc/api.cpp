@@ -545,6 +545,8 @@ int mfx_conv_create(mfx_ctx* c, size_t n, double L, int precision, mfx_conv** ou
   return guard([&] {
     MFX_REQUIRE(c && out, "null argument");
     MFX_REQUIRE(n % 2 == 0 && n >= 16, "Orszag convolver requires an even grid size >= 16");
+    MFX_REQUIRE(n % 4 == 0, "Orszag convolver: the grid size must be divisible by 4 (padded z rows of even length "
+                            "3n/2)");
     MFX_REQUIRE(precision == MFX_FLOAT || precision == MFX_DOUBLE, "invalid precision");
     auto* cv = new mfx_conv();
     cv->ctx = c;
@@ -837,6 +839,12 @@ static void lpt_reserve(mfx_conv* cv, int order) {
 }
 
 static void lpt_fused(mfx_conv* cv, mfx_grid* phi, mfx_grid* phi2, mfx_grid* phi3, mfx_grid* A3[3], int order) {
+  if (!lpt_fused_supported(cv->n)) {
+    // a grid size the engine is not compiled for: the same fields from the
+    // convolve sequence
+    lpt_sequence(cv, phi, phi2, phi3, A3, order);
+    return;
+  }
   lpt_reserve(cv, order);
   grid_forward(phi, true);
   phi2->space = MFX_KSPACE;
@@ -1149,7 +1157,7 @@ int mfx_autotune(mfx_ctx* c, size_t n, int precision) {
         autotune_columns<double>(c->c, L);
       ++tuned;
     }
-    MFX_REQUIRE(tuned > 0, "no supported column length for this grid size");
+    // lengths without tuned kernels run the generic kernels: nothing to tune
   });
 }
 
@@ -1356,6 +1364,7 @@ struct mfx_icgen {
   hipStream_t copy_stream = nullptr;
   hipEvent_t run_done = nullptr, copy_done = nullptr;
   bool copy_pending = false;
+  int in_batches = 0;  // batches whose graph records this generator
 };
 
 static void icgen_free(mfx_icgen* ic) {
@@ -1398,6 +1407,8 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
     MFX_REQUIRE(p->lpt_order >= 1 && p->lpt_order <= 3, "LPTorder must be 1, 2 or 3");
     // ic_generator.cc:118-122
     MFX_REQUIRE(!(p->n % 2 != 0 && p->lpt_order > 1), "Orszag convolver for LPTorder>1 requires even grid size!");
+    MFX_REQUIRE(!(p->n % 4 != 0 && p->lpt_order > 1),
+                "LPTorder>1 needs a grid size divisible by 4 (padded z rows of even length 3n/2)");
     MFX_REQUIRE(p->generator >= 0 && p->generator <= 2, "generator must be 0 (MUSIC1), 1 (NGENIC) or 2 (Threefry)");
     auto* ic = new mfx_icgen();
     ic->ctx = c;
@@ -1423,7 +1434,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       // FP32 this keeps the realisation within one GPU's HBM); they stay valid
       // until the next run.
       const size_t npart = p->n * p->n * p->n;
-      if (ic->conv && p->use_fused_lpt && !p->own_particles) {
+      if (ic->conv && p->use_fused_lpt && !p->own_particles && lpt_fused_supported((int)p->n)) {
         lpt_reserve(ic->conv, p->lpt_order);
         if (ic->conv->lpt_bytes >= 6 * npart * sizeof(double)) {
           ic->part_in_ws = true;
@@ -1499,6 +1510,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
 extern "C" int mfx_icgen_destroy(mfx_icgen* ic) {
   return guard([&] {
     if (!ic) return;
+    MFX_REQUIRE(ic->in_batches == 0, "generator is part of a batch; destroy the batch first");
     MFX_HIP(hipStreamSynchronize(ic->ctx->c.stream));
     icgen_free(ic);
     ic->ctx->children--;
@@ -1531,17 +1543,17 @@ static void icgen_stage_seed(mfx_icgen* ic, long seed) {
   ic->have_staged = true;
 }
 
-static void icgen_commit_seed(mfx_icgen* ic) {
+// on `st` (the context's stream unless a batch launches the generator)
+static void icgen_commit_seed(mfx_icgen* ic, hipStream_t st = nullptr) {
   if (!ic->have_staged) icgen_stage_seed(ic, ic->p.seed);
   if (ic->table_job.valid()) ic->table_job.get();
-  Ctx& c = ic->ctx->c;
+  if (!st) st = ic->ctx->c.stream;
   const int q = ic->slot;
   char* h = (char*)ic->host_stage[q];
-  MFX_HIP(hipMemcpyAsync(ic->dev_seed, h, 8, hipMemcpyHostToDevice, c.stream));
+  MFX_HIP(hipMemcpyAsync(ic->dev_seed, h, 8, hipMemcpyHostToDevice, st));
   if (ic->p.generator == 1)
-    MFX_HIP(hipMemcpyAsync(ic->dev_table, h + 16, sizeof(unsigned) * ic->p.n * ic->p.n, hipMemcpyHostToDevice,
-                           c.stream));
-  MFX_HIP(hipEventRecord(ic->staged[q], c.stream));
+    MFX_HIP(hipMemcpyAsync(ic->dev_table, h + 16, sizeof(unsigned) * ic->p.n * ic->p.n, hipMemcpyHostToDevice, st));
+  MFX_HIP(hipEventRecord(ic->staged[q], st));
   ic->have_staged = false;
 }
 
@@ -1784,5 +1796,138 @@ extern "C" int mfx_icgen_scalars(mfx_icgen* ic, double* dplus, double* vfact) {
     MFX_REQUIRE(ic, "null generator");
     if (dplus) *dplus = ic->dplus;
     if (vfact) *vfact = ic->vfact;
+  });
+}
+
+// ------------------------------------------------------------------ batched realisations
+// Many realisations (of any mix of grid sizes) as ONE captured HIP graph: the
+// device sequences of `count` generators are recorded on `nstreams` branch
+// streams forked from the batch's origin stream and joined back to it, so a
+// single graph launch runs them all, the branches concurrently.
+struct mfx_batch {
+  std::vector<mfx_icgen*> gens;
+  hipStream_t origin = nullptr;
+  std::vector<hipStream_t> branch;
+  std::vector<hipEvent_t> ev;  // fork, then one join per branch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+static void batch_free(mfx_batch* b) {
+  if (b->origin) (void)hipStreamSynchronize(b->origin);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto e : b->ev) (void)hipEventDestroy(e);
+  for (auto s : b->branch) (void)hipStreamDestroy(s);
+  if (b->origin) (void)hipStreamDestroy(b->origin);
+  for (auto* g : b->gens) g->in_batches--;
+}
+
+extern "C" int mfx_batch_create(mfx_icgen* const* gens, int count, int nstreams, mfx_batch** out) {
+  return guard([&] {
+    MFX_REQUIRE(gens && out && count > 0 && nstreams > 0, "invalid argument");
+    for (int i = 0; i < count; ++i) MFX_REQUIRE(gens[i], "null generator");
+    for (int i = 0; i < count; ++i)
+      for (int j = 0; j < i; ++j) MFX_REQUIRE(gens[i] != gens[j], "a generator appears twice in the batch");
+    auto* b = new mfx_batch();
+    try {
+      b->gens.assign(gens, gens + count);
+      for (auto* g : b->gens) g->in_batches++;
+      const int ns = nstreams < count ? nstreams : count;
+      MFX_HIP(hipStreamCreate(&b->origin));
+      b->branch.resize(ns);
+      for (auto& s : b->branch) MFX_HIP(hipStreamCreate(&s));
+      b->ev.resize(ns + 1);
+      for (auto& e : b->ev) MFX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      // one eager realisation per generator creates every lazily built table
+      for (auto* g : b->gens) {
+        icgen_commit_seed(g);
+        icgen_device_run(g);
+      }
+      for (auto* g : b->gens) MFX_HIP(hipStreamSynchronize(g->ctx->c.stream));
+      MFX_HIP(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+      std::vector<hipStream_t> saved(count);
+      std::vector<bool> prof(count);
+      try {
+        MFX_HIP(hipEventRecord(b->ev[0], b->origin));
+        for (auto s : b->branch) MFX_HIP(hipStreamWaitEvent(s, b->ev[0], 0));
+        for (int i = 0; i < count; ++i) {
+          mfx_icgen* g = b->gens[i];
+          Ctx& c = g->ctx->c;
+          saved[i] = c.stream;
+          prof[i] = c.profile;
+          c.stream = b->branch[i % ns];
+          c.profile = false;
+          g->capturing = true;
+          icgen_device_run(g);
+          g->capturing = false;
+          c.stream = saved[i];
+          c.profile = prof[i];
+        }
+        for (int q = 0; q < ns; ++q) {
+          MFX_HIP(hipEventRecord(b->ev[1 + q], b->branch[q]));
+          MFX_HIP(hipStreamWaitEvent(b->origin, b->ev[1 + q], 0));
+        }
+      } catch (...) {
+        for (int i = 0; i < count; ++i)
+          if (saved[i]) {
+            b->gens[i]->ctx->c.stream = saved[i];
+            b->gens[i]->ctx->c.profile = prof[i];
+            b->gens[i]->capturing = false;
+          }
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->origin, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      MFX_HIP(hipStreamEndCapture(b->origin, &b->graph));
+      MFX_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      batch_free(b);
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+// the staged seeds of every generator are copied on the origin stream, then
+// the graph runs; the generators' outputs are valid once the origin stream
+// (mfx_batch_synchronize) has finished
+extern "C" int mfx_batch_launch(mfx_batch* b) {
+  return guard([&] {
+    MFX_REQUIRE(b && b->gexec, "invalid batch");
+    for (auto* g : b->gens) {
+      // copies of earlier downloads must end before the arrays are rewritten
+      if (g->copy_pending) {
+        MFX_HIP(hipStreamWaitEvent(b->origin, g->copy_done, 0));
+        g->copy_pending = false;
+      }
+      // earlier work of the generator's own stream comes first
+      MFX_HIP(hipEventRecord(g->run_done, g->ctx->c.stream));
+      MFX_HIP(hipStreamWaitEvent(b->origin, g->run_done, 0));
+      icgen_commit_seed(g, b->origin);
+    }
+    MFX_HIP(hipGraphLaunch(b->gexec, b->origin));
+    for (auto* g : b->gens) {
+      MFX_HIP(hipEventRecord(g->run_done, b->origin));
+      // later work on the generator's stream waits for the batch
+      MFX_HIP(hipStreamWaitEvent(g->ctx->c.stream, g->run_done, 0));
+    }
+  });
+}
+
+extern "C" int mfx_batch_synchronize(mfx_batch* b) {
+  return guard([&] {
+    MFX_REQUIRE(b, "invalid batch");
+    MFX_HIP(hipStreamSynchronize(b->origin));
+  });
+}
+
+extern "C" int mfx_batch_destroy(mfx_batch* b) {
+  return guard([&] {
+    if (!b) return;
+    batch_free(b);
+    delete b;
   });
 }

@@ -4,6 +4,7 @@
 // the column kernel over tiles of adjacent columns and the z-row kernels
 // (r2c, c2r, fused c2r-product-r2c).
 #pragma once
+#include "fft_generic.hpp"
 #include "mfx_internal.hpp"
 #include "row_fft.hpp"
 
@@ -484,6 +485,32 @@ int col_tile_max(int L) {
   }
 }
 
+// ------------------------------------------------------------------ lengths without tuned kernels
+// A kernel whose dynamic LDS exceeds the default 64 KiB is allowed more once.
+template <class K>
+void gen_lds_prep(Ctx& c, K kernel, size_t lds) {
+  MFX_REQUIRE(lds <= (size_t)GEN_LDS_MAX, "transform length too long for one compute unit's LDS");
+  if (lds <= 65536) return;
+  int& done = c.occupancy[(const void*)kernel];
+  if (done < (int)lds) {
+    MFX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    done = (int)lds;
+  }
+}
+
+template <typename T, bool INV, class Src, class Dst>
+void launch_col_gen(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  const GenPlan plan = gen_plan(L);
+  const int C = gen_col_tile<T>(L);
+  const size_t lds = 2 * (size_t)L * C * sizeof(cx<T>);
+  auto k = k_col_gen<T, INV, Src, Dst>;
+  gen_lds_prep(c, k, lds);
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL(k, dim3((ncols + C - 1) / C, nbatch), dim3(GEN_THREADS), lds, c.stream, s, d, c.twiddles<T>(L),
+                     ncols, plan, C);
+  check_launch(name);
+}
+
 template <typename T, bool INV, class Src, class Dst>
 void launch_col(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
   switch (L) {
@@ -494,7 +521,7 @@ void launch_col(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const char* 
     MFX_COL_LENGTHS(MFX_CASE)
 #undef MFX_CASE
     default:
-      throw Error(1, "unsupported transform length " + std::to_string(L));
+      launch_col_gen<T, INV>(c, L, s, d, ncols, nbatch, name, bytes);
   }
 }
 
@@ -514,7 +541,7 @@ void launch_col_pad(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const ch
     MFX_PAD_LENGTHS(MFX_CASE)
 #undef MFX_CASE
     default:
-      throw Error(1, "unsupported padded transform length " + std::to_string(L));
+      launch_col_gen<T, INV>(c, L, s, d, ncols, nbatch, name, bytes);
   }
 }
 
@@ -564,8 +591,17 @@ void launch_turn(Ctx& c, int L, const TurnArgs<T>& a, int ncols, int nbatch, con
     return;
     MFX_PAD_LENGTHS(MFX_CASE)
 #undef MFX_CASE
-    default:
-      throw Error(1, "unsupported padded transform length " + std::to_string(L));
+    default: {
+      const GenPlan plan = gen_plan(L);
+      const int C = gen_col_tile<T>(L);
+      const size_t lds = 2 * (size_t)L * C * sizeof(cx<T>);
+      auto k = k_col_turn_gen<T, TurnArgs<T>>;
+      gen_lds_prep(c, k, lds);
+      LaunchScope ls(c, name, bytes);
+      hipLaunchKernelGGL(k, dim3((ncols + C - 1) / C, nbatch), dim3(GEN_THREADS), lds, c.stream, a,
+                         c.twiddles<T>(L), ncols, plan, C);
+      check_launch(name);
+    }
   }
 }
 
@@ -764,12 +800,27 @@ void launch_row_prod_K(Ctx& c, const cx<T>* a, const cx<T>* b, long long spitch,
   check_launch(name);
 }
 
-#define MFX_ROW_SWITCH(K, CALL)                                         \
-  switch (K) {                                                          \
-    MFX_ROW_LENGTHS(CALL)                                               \
-    default:                                                            \
-      throw Error(1, "unsupported row length " + std::to_string(2 * (K))); \
+// rows of a length without tuned kernels go to the generic kernel (GEN)
+#define MFX_ROW_SWITCH(K, CALL, GEN) \
+  switch (K) {                       \
+    MFX_ROW_LENGTHS(CALL)            \
+    default:                         \
+      GEN;                           \
   }
+
+// one generic row kernel launch: one workgroup per row
+#define MFX_GEN_ROW_LAUNCH(KERNEL, NBUF, ...)                                                       \
+  do {                                                                                              \
+    const GenPlan plan_ = gen_plan(K);                                                              \
+    const size_t lds_ = gen_row_lds<T>(K, NBUF);                                                    \
+    auto k_ = KERNEL<T>;                                                                            \
+    gen_lds_prep(c, k_, lds_);                                                                      \
+    MFX_REQUIRE(nrows < (1LL << 31), "too many rows for one launch");                               \
+    LaunchScope ls_(c, name, bytes);                                                                \
+    hipLaunchKernelGGL(k_, dim3((unsigned)nrows), dim3(GEN_THREADS), lds_, c.stream, __VA_ARGS__, plan_, \
+                       c.twiddles<T>(K), c.twiddles<T>(2 * K));                                     \
+    check_launch(name);                                                                             \
+  } while (0)
 
 template <typename T>
 void launch_row_r2c(Ctx& c, int K, const T* src, long long spitch, cx<T>* dst, long long dpitch, int hout,
@@ -778,7 +829,7 @@ void launch_row_r2c(Ctx& c, int K, const T* src, long long spitch, cx<T>* dst, l
   case KK:                                                                                        \
     launch_row_r2c_K<T, KK>(c, src, spitch, dst, dpitch, hout, nrows, scale, name, bytes);       \
     return;
-  MFX_ROW_SWITCH(K, MFX_CASE)
+  MFX_ROW_SWITCH(K, MFX_CASE, MFX_GEN_ROW_LAUNCH(k_row_r2c_gen, 2, src, spitch, dst, dpitch, hout, scale))
 #undef MFX_CASE
 }
 
@@ -789,7 +840,7 @@ void launch_row_c2r(Ctx& c, int K, const cx<T>* src, long long spitch, int hin, 
   case KK:                                                                                        \
     launch_row_c2r_K<T, KK>(c, src, spitch, hin, dst, dpitch, nrows, scale, name, bytes);        \
     return;
-  MFX_ROW_SWITCH(K, MFX_CASE)
+  MFX_ROW_SWITCH(K, MFX_CASE, MFX_GEN_ROW_LAUNCH(k_row_c2r_gen, 2, src, spitch, hin, dst, dpitch, scale))
 #undef MFX_CASE
 }
 
@@ -810,7 +861,8 @@ void launch_row_c2r_part(Ctx& c, int K, const cx<T>* src, long long spitch, int 
   case KK:                                                                                                \
     launch_row_c2r_part_K<T, KK>(c, src, spitch, hin, out, nrows, scale, idim, lunit, kind, name, bytes); \
     return;
-  MFX_ROW_SWITCH(K, MFX_CASE)
+  MFX_ROW_SWITCH(K, MFX_CASE,
+                 MFX_GEN_ROW_LAUNCH(k_row_c2r_part_gen, 2, src, spitch, hin, out, scale, idim, lunit, kind))
 #undef MFX_CASE
 }
 
@@ -821,7 +873,7 @@ void launch_row_prod(Ctx& c, int K, const cx<T>* a, const cx<T>* b, long long sp
   case KK:                                                                                          \
     launch_row_prod_K<T, KK>(c, a, b, spitch, hin, dst, dpitch, hout, nrows, name, bytes);        \
     return;
-  MFX_ROW_SWITCH(K, MFX_CASE)
+  MFX_ROW_SWITCH(K, MFX_CASE, MFX_GEN_ROW_LAUNCH(k_row_prod_gen, 4, a, b, spitch, hin, dst, dpitch, hout))
 #undef MFX_CASE
 }
 

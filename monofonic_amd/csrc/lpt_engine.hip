@@ -490,6 +490,21 @@ void launch_zC2_K(Ctx& c, const FieldY<T>& f1, const FieldY<T>& f2, int i, cx<T>
       throw Error(1, "unsupported padded length " + std::to_string(2 * (K))); \
   }
 
+// the fused engine is compiled for the padded row lengths K = 3n/4 of
+// MFX_PAD_K_LENGTHS; other grid sizes take the convolve sequence
+bool lpt_fused_supported(int n) {
+  if (n % 4 != 0) return false;
+  switch (3 * n / 4) {
+#define MFX_CASE(KK) \
+  case KK:           \
+    return true;
+    MFX_PAD_K_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      return false;
+  }
+}
+
 // ------------------------------------------------------------------ host orchestration
 struct Layout {
   int n, m, h, hp;

@@ -83,6 +83,8 @@ void ngenic_seedtable(unsigned* table, int n, long seed);
 
 // fused LPT source engine
 size_t lpt_workspace_bytes(int n, int prec, int order, bool stored);
+// whether the fused engine is compiled for grid size n
+bool lpt_fused_supported(int n);
 template <typename T>
 void lpt_sources(Ctx& c, const cx<T>* phi, cx<T>* phi2, cx<T>* phi3, cx<T>* a3x, cx<T>* a3y, cx<T>* a3z, int n,
                  double boxlen, int order, bool stored, void* ws);
