@@ -24,6 +24,13 @@ Also reported on the same line:
   accuracy      per output field, FP32 bulk vs FP64 bulk: RMS and max error
                 relative to the field RMS, and the P(k) ratio deviation
   cpu_baseline  the FP64 CPU oracle (OpenMP) on a 256^3 3LPT sample (box 500)
+  accuracy_vs_oracle  per output field of that 256^3 sample, FP32-bulk and
+                FP64 GPU paths vs the FP64 CPU oracle: RMS and max error
+                relative to the oracle field's RMS
+
+--config batched reports the 48 uneven 2LPT realisations (16 each at 96^3,
+128^3, 192^3) run sequentially, as per-generator graphs on streams, and as
+one captured graph holding all 48.
 """
 import argparse
 import json
@@ -167,17 +174,48 @@ def accuracy(M, ctx, ic32, ic64, n, boxlen):
 
 
 def cpu_baseline(n=256):
-    """FP64 CPU oracle (OpenMP) on an n^3 3LPT sample of the same workload"""
+    """FP64 CPU oracle (OpenMP) on an n^3 3LPT sample of the same workload;
+    also returns the oracle's six fields for the accuracy check"""
     from oracle import oracle as O
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from test_gpu_lpt import oracle_pipeline
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cos = O.Cosmology()
     t0 = time.perf_counter()
-    oracle_pipeline(O, n, 1000.0 * n / 512, 24.0, 3, 12345, cos)
+    fields, _, _ = oracle_pipeline(O, n, 1000.0 * n / 512, 24.0, 3, 12345, cos)
     dt = time.perf_counter() - t0
     return {"value": round(n ** 3 / dt, 1), "unit": "particles/s", "cores": cores, "kind": "port",
-            "sample": f"3LPT {n}^3 sc, box {1000.0 * n / 512:g} Mpc/h, MUSIC1, FP64 oracle (OpenMP), {dt:.2f} s"}
+            "sample": f"3LPT {n}^3 sc, box {1000.0 * n / 512:g} Mpc/h, MUSIC1, FP64 oracle (OpenMP), {dt:.2f} s"}, fields
+
+
+def accuracy_vs_oracle(M, ctx, n, fields):
+    """per field RMS and max |GPU - oracle| relative to the oracle field's RMS
+    (sigma_Psi for displacements) for the FP32-bulk and FP64-bulk GPU paths,
+    same seed and configuration as the oracle sample"""
+    import numpy as np
+    names = ["psi_x", "psi_y", "psi_z", "v_x", "v_y", "v_z"]
+    q1 = np.arange(n, dtype=np.float64) / n
+    out = {}
+    for prec in ("float", "double"):
+        ic = M.ICGenerator(ctx, n, 1000.0 * n / 512, 24.0, LPTorder=3, precision=prec, generator="MUSIC1", seed=12345)
+        ic.run()
+        res = {}
+        for w in range(6):
+            a = ic.particles(w)
+            b = fields[w].copy()
+            if w < 3:
+                sh = [1, 1, 1]
+                sh[w] = n
+                a -= q1.reshape(sh)
+                b -= q1.reshape(sh)
+            sig = float(np.sqrt(np.mean(b * b)))
+            d = a - b
+            res[names[w]] = {"rms_rel": float(f"{np.sqrt(np.mean(d * d)) / sig:.3e}"),
+                             "max_rel": float(f"{np.max(np.abs(d)) / sig:.3e}")}
+        out["fp32_bulk" if prec == "float" else "fp64"] = res
+        del ic
+        log(f"[bench] {prec} vs oracle at {n}^3: max rms {max(v['rms_rel'] for v in res.values()):.2e}")
+    return out
 
 
 BATCH_CLASSES = [(96, 150.0), (128, 200.0), (192, 300.0)]  # (grid, box Mpc/h), 16 realisations each
@@ -231,15 +269,69 @@ def run_batched(M, reps=16, streams=4, seed0=1000):
     ggens.clear()
     for q in range(nq):
         ctxs[q].close()
+    # (c) all 48 realisations, one generator each, captured into ONE graph
+    # whose branches run on `bstreams` streams; seeds staged per generator
+    import numpy as np
+    bctx = M.Context(0)
+    order = [ci for r in range(reps) for ci in range(len(BATCH_CLASSES))]  # interleave the sizes
+    bgens = [M.ICGenerator(bctx, BATCH_CLASSES[ci][0], BATCH_CLASSES[ci][1], seed=seed0, **kw) for ci in order]
+    bstreams = 16
+    batch = M.Batch(bgens, nstreams=bstreams)
+    seeds = [seed0 + r for r in range(len(bgens))]
+    for g, sd in zip(bgens, seeds):
+        g.set_seed(sd)
+    batch.launch()  # warm-up
+    batch.synchronize()
+    for g, sd in zip(bgens, seeds):
+        g.set_seed(sd)
+    t0 = time.perf_counter()
+    batch.launch()
+    batch.synchronize()
+    one_s = time.perf_counter() - t0
+    log(f"[bench] batched: one graph, {len(bgens)} realisations on {bstreams} streams {one_s * 1e3:.1f} ms")
+    # check: one realisation of each size equals an eager FP32 run with its
+    # seed bit for bit; its RMS distance to the eager FP64-bulk run
+    check = []
+    for i in range(len(BATCH_CLASSES)):
+        n, L = BATCH_CLASSES[order[i]]
+        got = [bgens[i].particles(w) for w in range(6)]
+        eq = True
+        for prec in ("float", "double"):
+            e = M.ICGenerator(bctx, n, L, seed=seeds[i], **dict(kw, precision=prec))
+            e.run()
+            ref = [e.particles(w) for w in range(6)]
+            if prec == "float":
+                eq = all(np.array_equal(a, b) for a, b in zip(got, ref))
+            else:
+                q1 = np.arange(n) / n
+                errs = []
+                for w in range(6):
+                    a, b = got[w], ref[w]
+                    if w < 3:
+                        sh = [1, 1, 1]
+                        sh[w] = n
+                        a, b = a - q1.reshape(sh), b - q1.reshape(sh)
+                    errs.append(float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))))
+            del e
+        check.append({"n": n, "seed": seeds[i], "equals_eager_fp32": bool(eq),
+                      "max_rms_rel_vs_eager_fp64": float(f"{max(errs):.3e}")})
+        log(f"[bench] batched check {n}^3: equal to eager {eq}, vs FP64 {max(errs):.2e}")
+    batch.close()
+    del g, got, batch
+    bgens.clear()
+    bctx.close()
     return {
-        "metric": "batched uneven 2LPT particles/s (16 x 96^3, 16 x 128^3, 16 x 192^3), graphs on streams",
-        "value": round(npart / graph_s, 1), "unit": "particles/s", "n_gpus": 1, "higher_is_better": True,
+        "metric": "batched uneven 2LPT particles/s (16 x 96^3, 16 x 128^3, 16 x 192^3), one captured graph",
+        "value": round(npart / one_s, 1), "unit": "particles/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "48 realisations, 2LPT sc, NGENIC seeds s0+r, FP32 bulk + FP64 assembly",
-                   "realisations": 3 * reps, "particles": npart, "streams": nq},
+                   "realisations": 3 * reps, "particles": npart, "graph_branches": bstreams},
+        "one_graph_ms": round(one_s * 1e3, 3),
         "sequential_ms": round(seq_s * 1e3, 3), "graph_streams_ms": round(graph_s * 1e3, 3),
-        "speedup_vs_sequential": round(seq_s / graph_s, 3),
+        "graph_streams": nq, "speedup_vs_sequential": round(seq_s / one_s, 3),
         "sequential_particles_per_s": round(npart / seq_s, 1),
+        "graph_streams_particles_per_s": round(npart / graph_s, 1),
+        "check": check,
     }
 
 
@@ -304,10 +396,15 @@ def main():
     del ic32
 
     cpu = None
+    acc_oracle = None
     if not args.no_cpu:
         log("[bench] CPU oracle baseline ...")
-        cpu = cpu_baseline(args.cpu_n)
+        cpu, ofields = cpu_baseline(args.cpu_n)
         log(f"[bench] CPU oracle baseline done: {cpu['value']} particles/s")
+        acc_oracle = {"n": args.cpu_n, "box": 1000.0 * args.cpu_n / 512,
+                      "normalisation": "|GPU - FP64 CPU oracle| / RMS of the oracle field",
+                      "fields": accuracy_vs_oracle(M, ctx, args.cpu_n, ofields)}
+        del ofields
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
@@ -321,7 +418,7 @@ def main():
                    "column_tiles": ctx.autotune_report().strip().splitlines(), "wall_s": round(wall, 4)},
         "roofline": roof, "fft_gbps": round(fft_gbps, 1) if fft_gbps else None,
         "fft_frac": round(fft_gbps / HBM_PEAK_GBPS, 4) if fft_gbps else None,
-        "fp64": fp64, "accuracy": acc, "cpu_baseline": cpu,
+        "fp64": fp64, "accuracy": acc, "accuracy_vs_oracle": acc_oracle, "cpu_baseline": cpu,
         "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
                         "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
                     for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])},
