@@ -687,6 +687,7 @@ class Batch:
         if self._h:
             h, self._h = self._h, ctypes.c_void_p()
             call("mfx_batch_destroy", h)
+        self.gens = []  # the generators may now be destroyed
 
     def __del__(self):
         try:
