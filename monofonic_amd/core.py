@@ -82,8 +82,8 @@ class Context:
         call("mfx_set_column_tile", self._h, int(L), _prec(precision), int(inverse), int(tile))
 
     def autotune_report(self):
-        buf = ctypes.create_string_buffer(8192)
-        call("mfx_autotune_report", self._h, buf, 8192)
+        buf = ctypes.create_string_buffer(65536)
+        call("mfx_autotune_report", self._h, buf, 65536)
         return buf.value.decode()
 
     def d2h(self, dev, nbytes, dtype):

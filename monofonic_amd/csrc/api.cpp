@@ -1167,11 +1167,13 @@ int mfx_set_column_tile(mfx_ctx* c, int L, int precision, int inverse, int tile)
     MFX_REQUIRE(column_length_supported(L), "unsupported column length");
     MFX_REQUIRE(precision == MFX_FLOAT || precision == MFX_DOUBLE, "invalid precision");
     const int cm = column_tile_max(L, precision);
-    const int w = tile & ~COL_PIPELINED;
-    MFX_REQUIRE((tile & ~(COL_PIPELINED | 0xff)) == 0 && (w == cm || (cm >= 2 && w == cm / 2)),
+    const int w = tile & COL_WIDTH;
+    const int flags = COL_PIPELINED | (column_plan_alt(L) ? COL_PLAN_ALT : 0);
+    MFX_REQUIRE((tile & ~(flags | COL_WIDTH)) == 0 && (w == cm || (cm >= 2 && w == cm / 2)),
                 "tile must be " + std::to_string(cm) + (cm >= 2 ? " or " + std::to_string(cm / 2) : std::string()) +
-                    ", optionally or'ed with MFX_COL_PIPELINED");
-    c->c.coltile[std::make_tuple(L, precision, inverse ? 1 : 0)] = tile;
+                    ", optionally or'ed with MFX_COL_PIPELINED" +
+                    (column_plan_alt(L) ? " and MFX_COL_PLAN_ALT" : " (this length has one radix plan)"));
+    for (int cls = 0; cls < 2; ++cls) c->c.coltile[std::make_tuple(L, precision, inverse ? 1 : 0, cls)] = tile;
   });
 }
 
@@ -1181,11 +1183,11 @@ int mfx_autotune_report(mfx_ctx* c, char* buf, size_t len) {
     std::string out;
     char line[192];
     auto name = [](int v) {
-      return "C=" + std::to_string(v & ~COL_PIPELINED) + ((v & COL_PIPELINED) ? "p" : "");
+      return "C=" + std::to_string(v & COL_WIDTH) + ((v & COL_PIPELINED) ? "p" : "") + ((v & COL_PLAN_ALT) ? "a" : "");
     };
     for (const auto& r : c->c.tune_log) {
-      std::snprintf(line, sizeof line, "L=%d %s %s:", r.L, r.tsize == 4 ? "f32" : "f64",
-                    r.inv ? "inverse" : "forward");
+      std::snprintf(line, sizeof line, "L=%d %s %s%s:", r.L, r.tsize == 4 ? "f32" : "f64",
+                    r.inv ? "inverse" : "forward", r.cls ? " plane-strided" : "");
       out += line;
       for (int q = 0; q < r.ncand; ++q) {
         std::snprintf(line, sizeof line, " %s %.4f ms%s", name(r.cand[q]).c_str(), r.ms[q], q + 1 < r.ncand ? "," : "");

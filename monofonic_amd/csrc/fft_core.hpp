@@ -167,15 +167,34 @@ __device__ __forceinline__ cx<T> ld(gcx<T>* p, int i) {
 }
 
 // ---------------------------------------------------------------- plans
-// Radix schedule per length: EPT must be a multiple of every radix.
-template <int L>
+// Radix schedule per length and variant V: EPT must be a multiple of every
+// radix.  Variant 0 exists for every length; some lengths have a variant 1
+// with fewer, larger stages (fewer LDS exchanges, more elements per thread),
+// and the column-tile autotuner picks between them per precision and
+// direction.
+template <int L, int V = 0>
 struct Plan;
+template <int L>
+struct PlanAlt {
+  static constexpr bool value = false;
+};
 #define MFX_PLAN(L_, EPT_, ...)                            \
   template <>                                              \
-  struct Plan<L_> {                                        \
+  struct Plan<L_, 0> {                                     \
     static constexpr int EPT = EPT_;                       \
     static constexpr int R[] = {__VA_ARGS__};              \
     static constexpr int NR = sizeof(R) / sizeof(int);     \
+  };
+#define MFX_PLAN_ALT(L_, EPT_, ...)                        \
+  template <>                                              \
+  struct Plan<L_, 1> {                                     \
+    static constexpr int EPT = EPT_;                       \
+    static constexpr int R[] = {__VA_ARGS__};              \
+    static constexpr int NR = sizeof(R) / sizeof(int);     \
+  };                                                       \
+  template <>                                              \
+  struct PlanAlt<L_> {                                     \
+    static constexpr bool value = true;                    \
   };
 
 MFX_PLAN(8, 8, 8)
@@ -198,31 +217,38 @@ MFX_PLAN(512, 8, 8, 8, 8)
 MFX_PLAN(768, 12, 12, 4, 4, 4)
 MFX_PLAN(1024, 8, 8, 8, 4, 4)
 MFX_PLAN(1536, 12, 12, 4, 4, 4, 2)
+// two exchanges instead of three (384, 768, 1024) or three instead of four (1536)
+MFX_PLAN_ALT(384, 24, 12, 8, 4)
+MFX_PLAN_ALT(768, 24, 12, 8, 8)
+MFX_PLAN_ALT(1024, 16, 16, 16, 4)
+MFX_PLAN_ALT(1536, 24, 12, 8, 8, 2)
 #undef MFX_PLAN
+#undef MFX_PLAN_ALT
 
-template <int L>
+template <int L, int V = 0>
 struct PlanInfo {
-  static constexpr int EPT = Plan<L>::EPT;
+  static constexpr int EPT = Plan<L, V>::EPT;
   static constexpr int TPS = L / EPT;
 };
 
 // product of the radices of the stages before stage S
-template <int L, int S>
+template <int L, int S, int V>
 struct StageNS {
-  static constexpr int value = S == 0 ? 1 : Plan<L>::R[S - 1] * StageNS<L, (S > 0 ? S - 1 : 0)>::value;
+  static constexpr int value = S == 0 ? 1 : Plan<L, V>::R[S - 1] * StageNS<L, (S > 0 ? S - 1 : 0), V>::value;
 };
-template <int L>
-struct StageNS<L, 0> {
+template <int L, int V>
+struct StageNS<L, 0, V> {
   static constexpr int value = 1;
 };
 
 // Twiddles of stage S for thread t: w[m*(R-1) + q-1] = tw[q k L/(NS R)] for
 // butterfly m (k = (t + TPS m) mod NS) and q = 1..R-1; none for stage 0.
-template <typename T, int L, int S>
-__device__ __forceinline__ void stage_twiddles(cx<T> (&w)[Plan<L>::EPT], gcx<T>* tw, int t) {
-  if constexpr (S > 0 && S < Plan<L>::NR) {
-    constexpr int EPT = Plan<L>::EPT, TPS = L / EPT, R = Plan<L>::R[S], NB = EPT / R;
-    constexpr int NS = StageNS<L, S>::value;
+template <typename T, int L, int S, int V, int EPT>
+__device__ __forceinline__ void stage_twiddles(cx<T> (&w)[EPT], gcx<T>* tw, int t) {
+  static_assert(EPT == Plan<L, V>::EPT, "register array of the plan");
+  if constexpr (S > 0 && S < Plan<L, V>::NR) {
+    constexpr int TPS = L / EPT, R = Plan<L, V>::R[S], NB = EPT / R;
+    constexpr int NS = StageNS<L, S, V>::value;
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
       const int k = (t + TPS * m) % NS;
@@ -275,26 +301,27 @@ __device__ __forceinline__ void fft_stage(cx<T> (&v)[EPT], cx<T>* __restrict__ l
 // Stage S runs with its twiddles already in registers (wcur) and first issues
 // the loads of stage S+1's twiddles, whose latency then overlaps stage S's
 // arithmetic and LDS exchange.
-template <typename T, int L, bool INV, int S>
-__device__ __forceinline__ void stage_loop(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t, gcx<T>* tw,
-                                           const cx<T> (&wcur)[Plan<L>::EPT]) {
-  if constexpr (S < Plan<L>::NR) {
-    constexpr int R = Plan<L>::R[S];
-    cx<T> wnext[Plan<L>::EPT];
-    stage_twiddles<T, L, S + 1>(wnext, tw, t);
-    fft_stage<T, L, Plan<L>::EPT, R, StageNS<L, S>::value, INV, S == Plan<L>::NR - 1>(v, lds, ls, t, wcur);
-    stage_loop<T, L, INV, S + 1>(v, lds, ls, t, tw, wnext);
+template <typename T, int L, bool INV, int S, int V, int EPT>
+__device__ __forceinline__ void stage_loop(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t, gcx<T>* tw,
+                                           const cx<T> (&wcur)[EPT]) {
+  if constexpr (S < Plan<L, V>::NR) {
+    constexpr int R = Plan<L, V>::R[S];
+    cx<T> wnext[EPT];
+    stage_twiddles<T, L, S + 1, V>(wnext, tw, t);
+    fft_stage<T, L, EPT, R, StageNS<L, S, V>::value, INV, S == Plan<L, V>::NR - 1>(v, lds, ls, t, wcur);
+    stage_loop<T, L, INV, S + 1, V>(v, lds, ls, t, tw, wnext);
   }
 }
 
-// Full length-L transform of the registers v (positions t + TPS*e).
-// lds points at this sequence's L-element exchange area with element stride ls.
-// All threads of the block must call it (it contains barriers).
-template <typename T, int L, bool INV>
-__device__ __forceinline__ void fft_exec(cx<T> (&v)[Plan<L>::EPT], cx<T>* lds, int ls, int t,
-                                         const cx<T>* __restrict__ tw) {
-  cx<T> w0[Plan<L>::EPT];
-  stage_loop<T, L, INV, 0>(v, lds, ls, t, global_table(tw), w0);
+// Full length-L transform (plan variant V) of the registers v (positions
+// t + TPS*e).  lds points at this sequence's L-element exchange area with
+// element stride ls.  All threads of the block must call it (it contains
+// barriers).
+template <typename T, int L, bool INV, int V = 0, int EPT>
+__device__ __forceinline__ void fft_exec(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t, const cx<T>* __restrict__ tw) {
+  static_assert(EPT == Plan<L, V>::EPT, "register array of the plan");
+  cx<T> w0[EPT];
+  stage_loop<T, L, INV, 0, V>(v, lds, ls, t, global_table(tw), w0);
 }
 
 }  // namespace mfx

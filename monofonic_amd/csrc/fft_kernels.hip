@@ -133,9 +133,10 @@ void conv2(Ctx& c, const cx<T>* fa, const MultSpec& ma, const cx<T>* fb, const M
 // Column-pass variant: the widest tile whose exchange area fits the LDS
 // budget keeps 8-16 adjacent columns per line access (wider HBM segments),
 // half of it doubles the workgroups per CU; the persistent pipelined kernel
-// keeps the next tile's loads in flight while a tile is transformed.  Which
-// is faster depends on the length and precision, so the (up to) four
-// combinations are timed on a block of the library's own shape -- L/2+1
+// keeps the next tile's loads in flight while a tile is transformed; a
+// length's second radix plan (where it has one) trades an LDS exchange for
+// more elements per thread.  Which is faster depends on the length and
+// precision, so the (up to) eight combinations are timed on a block of the library's own shape -- L/2+1
 // adjacent columns, up to 256 batches -- in both directions, five launches
 // each after two warm-ups, and the fastest is kept.
 template <typename T>
@@ -152,37 +153,47 @@ void autotune_columns(Ctx& c, int L) {
     MFX_HIP(hipMemsetAsync(buf, 0, sizeof(cx<T>) * cnt, c.stream));
     MFX_HIP(hipEventCreate(&e0));
     MFX_HIP(hipEventCreate(&e1));
-    const long long sb = (long long)L * ncols, sp = ncols;
-    for (int inv = 0; inv < 2; ++inv) {
-      TuneRec rec{L, (int)sizeof(T), inv, 0, {}, {}, cm};
-      for (int pipe = 0; pipe < 2; ++pipe)
-        for (int h = 0; h < (cm >= 2 ? 2 : 1); ++h) rec.cand[rec.ncand++] = (h ? cm / 2 : cm) | (pipe ? COL_PIPELINED : 0);
-      const int ncand = rec.ncand;
-      double best = 1e300;
-      for (int q = 0; q < ncand; ++q) {
-        c.coltile[std::make_tuple(L, (int)sizeof(T), inv)] = rec.cand[q];
-        auto pass = [&] {
-          if (inv)
-            launch_col<T, true>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune", 0);
-          else
-            launch_col<T, false>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune", 0);
-        };
-        pass();
-        pass();
-        MFX_HIP(hipEventRecord(e0, c.stream));
-        for (int r = 0; r < 5; ++r) pass();
-        MFX_HIP(hipEventRecord(e1, c.stream));
-        MFX_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        MFX_HIP(hipEventElapsedTime(&ms, e0, e1));
-        rec.ms[q] = ms / 5.0;
-        if (rec.ms[q] < best) {
-          best = rec.ms[q];
-          rec.chosen = rec.cand[q];
+    for (int cls = 0; cls < 2; ++cls) {
+      // class 0: lines of L elements a row of columns apart ([batch][line][col]);
+      // class 1: a plane apart ([line][batch][col]); only where the plane is
+      // large enough to be class 1 at all
+      const long long sb = cls ? ncols : (long long)L * ncols, sp = cls ? (long long)nb * ncols : ncols;
+      if (stride_class(sp, (int)sizeof(cx<T>)) != cls) continue;
+      for (int inv = 0; inv < 2; ++inv) {
+        TuneRec rec{L, (int)sizeof(T), inv, cls, 0, {}, {}, cm};
+        for (int alt = 0; alt < (col_plan_alt(L) ? 2 : 1); ++alt)
+          for (int pipe = 0; pipe < 2; ++pipe)
+            for (int h = 0; h < (cm >= 2 ? 2 : 1); ++h)
+              rec.cand[rec.ncand++] = (h ? cm / 2 : cm) | (pipe ? COL_PIPELINED : 0) | (alt ? COL_PLAN_ALT : 0);
+        const int ncand = rec.ncand;
+        double best = 1e300;
+        for (int q = 0; q < ncand; ++q) {
+          c.coltile[std::make_tuple(L, (int)sizeof(T), inv, cls)] = rec.cand[q];
+          auto pass = [&] {
+            if (inv)
+              launch_col<T, true>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune",
+                                  0);
+            else
+              launch_col<T, false>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune",
+                                   0);
+          };
+          pass();
+          pass();
+          MFX_HIP(hipEventRecord(e0, c.stream));
+          for (int r = 0; r < 5; ++r) pass();
+          MFX_HIP(hipEventRecord(e1, c.stream));
+          MFX_HIP(hipEventSynchronize(e1));
+          float ms = 0;
+          MFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+          rec.ms[q] = ms / 5.0;
+          if (rec.ms[q] < best) {
+            best = rec.ms[q];
+            rec.chosen = rec.cand[q];
+          }
         }
+        c.coltile[std::make_tuple(L, (int)sizeof(T), inv, cls)] = rec.chosen;
+        c.tune_log.push_back(rec);
       }
-      c.coltile[std::make_tuple(L, (int)sizeof(T), inv)] = rec.chosen;
-      c.tune_log.push_back(rec);
     }
   } catch (...) {
     c.profile = prof;
@@ -200,6 +211,8 @@ void autotune_columns(Ctx& c, int L) {
 
 template void autotune_columns<float>(Ctx&, int);
 template void autotune_columns<double>(Ctx&, int);
+
+int column_plan_alt(int L) { return col_plan_alt(L) ? 1 : 0; }
 
 int column_tile_max(int L, int tsize) {
   return tsize == 4 ? col_tile_max<float>(L) : col_tile_max<double>(L);

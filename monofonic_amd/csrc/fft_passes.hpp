@@ -14,9 +14,9 @@
 namespace mfx {
 
 // ------------------------------------------------------------------ config
-template <typename T, int L>
+template <typename T, int L, int V = 0>
 struct ColCfg {
-  static constexpr int TPS = PlanInfo<L>::TPS;
+  static constexpr int TPS = PlanInfo<L, V>::TPS;
   static constexpr int lim_lds = 65536 / (L * (int)sizeof(cx<T>));
   static constexpr int lim_thr = 1024 / TPS;
   static constexpr int lim = (lim_lds < lim_thr ? lim_lds : lim_thr) < 16 ? (lim_lds < lim_thr ? lim_lds : lim_thr) : 16;
@@ -223,10 +223,10 @@ __device__ __forceinline__ void xcd_tile(int& tile, int& b) {
 }
 
 // C adjacent columns per workgroup (the tile width chosen by the autotuner)
-template <typename T, int L, bool INV, class Src, class Dst, int C>
-__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
+template <typename T, int L, bool INV, class Src, class Dst, int C, int V = 0>
+__global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
-  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
   cx<T> v[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
-  fft_exec<T, L, INV>(v, lds + c, C, t, tw);
+  fft_exec<T, L, INV, V>(v, lds + c, C, t, tw);
   cx<T> old[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) old[e] = dst.load_old(b, colc, t + TPS * e);
@@ -267,10 +267,10 @@ __device__ __forceinline__ bool pipe_range(long long nitems, long long& w, long 
   return w < hi;
 }
 
-template <typename T, int L, bool INV, class Src, class Dst, int C>
-__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
+template <typename T, int L, bool INV, class Src, class Dst, int C, int V = 0>
+__global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_pipe(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
-  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   long long w, hi;
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
     if (w + step < hi) fetch(w + step, nx);
     // every stage exchange of the previous item ended with a barrier after
     // its last LDS read, so the first exchange of this one may write
-    fft_exec<T, L, INV>(v, lds + c, C, t, tw);
+    fft_exec<T, L, INV, V>(v, lds + c, C, t, tw);
     const int b = (int)(w / ntiles), col = (int)(w % ntiles) * C + c;
     const int colc = col < ncols ? col : ncols - 1;
     cx<T> old[EPT];
@@ -313,6 +313,53 @@ __device__ __forceinline__ const cx<T>* opaque(const cx<T>* p) {
   return p;
 }
 
+// Up to three outputs of one source tile (the x and y stages of the Hessian
+// rows): output o transforms the zero-padded source line times the power
+// pw[o] of its wavenumber factor (SrcPadPow) into out[o] (DstPlain strides).
+// The tile is read again from memory for each output instead of being held
+// in registers, which would cost occupancy; a re-read follows the first read
+// of the same workgroup by one transform and is served by the L2 or the
+// MALL, so HBM sees the source once.  o is uniform: the selects below pick
+// kernel-argument fields without indexing into private memory.
+template <typename T>
+struct MultiPow {
+  SrcPadPow<T> src;
+  DstPlain<T> dst;
+  cx<T>* out[3];
+  int pw[3];
+  int no;
+};
+
+template <typename T, int L, bool INV, int C, int V = 0>
+__global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
+    k_col_multi(MultiPow<T> a, const cx<T>* __restrict__ tw, int ncols) {
+  constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
+  __shared__ cx<T> lds[L * C];
+  const int c = threadIdx.x % C, t = threadIdx.x / C;
+  int tile, b;
+  xcd_tile(tile, b);
+  const int col = tile * C + c;
+  const bool act = col < ncols;
+  const int colc = act ? col : ncols - 1;
+#pragma unroll 1
+  for (int o = 0; o < a.no; ++o) {
+    SrcPadPow<T> s = a.src;
+    s.pw = o == 0 ? a.pw[0] : (o == 1 ? a.pw[1] : a.pw[2]);
+    DstPlain<T> d = a.dst;
+    d.p = o == 0 ? a.out[0] : (o == 1 ? a.out[1] : a.out[2]);
+    cx<T> v[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) v[e] = s.load(b, colc, t + TPS * e);
+    // the previous output's last exchange ended with a barrier after its
+    // last LDS read, so this transform's first exchange may write
+    fft_exec<T, L, INV, V>(v, lds + c, C, t, opaque(tw));
+    if (act) {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) d.store(b, col, t + TPS * e, v[e], v[e]);
+    }
+  }
+}
+
 // Projection of a padded line onto the modes of the n-grid: forward
 // transform, truncation to the n kept modes with the Nyquist mode and the
 // Nyquist column (kz = n/2) zeroed and the scale fscale (the truncating pass
@@ -332,10 +379,10 @@ struct TurnArgs {
   long long osb, osp;
 };
 
-template <typename T, int L, int C>
-__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
+template <typename T, int L, int C, int V = 0>
+__global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols) {
-  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
@@ -347,7 +394,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
   cx<T> x[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) x[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
-  fft_exec<T, L, false>(x, lds + c, C, t, tw);
+  fft_exec<T, L, false, V>(x, lds + c, C, t, tw);
   const bool nyqc = colc == nh;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
@@ -358,7 +405,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
     x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
   }
   __syncthreads();  // the forward transform's exchange reads are done
-  fft_exec<T, L, true>(x, lds + c, C, t, opaque(tw));
+  fft_exec<T, L, true, V>(x, lds + c, C, t, opaque(tw));
   if (act) {
 #pragma unroll
     for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(t + TPS * e) * a.osp + col] = x[e];
@@ -366,10 +413,10 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
 }
 
 // persistent, software-pipelined k_col_turn (see k_col_pipe)
-template <typename T, int L, int C>
-__global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
+template <typename T, int L, int C, int V = 0>
+__global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn_pipe(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
-  constexpr int EPT = PlanInfo<L>::EPT, TPS = PlanInfo<L>::TPS;
+  constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   const int nh = a.n >> 1;
@@ -391,7 +438,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
     if (w + step < hi) fetch(w + step, nx);
     const int b = (int)(w / ntiles), col = (int)(w % ntiles) * C + c;
     const int colc = col < ncols ? col : ncols - 1;
-    fft_exec<T, L, false>(x, lds + c, C, t, tw);
+    fft_exec<T, L, false, V>(x, lds + c, C, t, tw);
     const bool nyqc = colc == nh;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -401,7 +448,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L>::TPS))
       x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
     }
     __syncthreads();  // the forward transform's exchange reads are done
-    fft_exec<T, L, true>(x, lds + c, C, t, opaque(tw));
+    fft_exec<T, L, true, V>(x, lds + c, C, t, opaque(tw));
     if (col < ncols) {
 #pragma unroll
       for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(t + TPS * e) * a.osp + col] = x[e];
@@ -431,44 +478,70 @@ unsigned pipe_grid(Ctx& c, K kernel, int block, long long nitems) {
   return (unsigned)((g + 7) / 8 * 8);
 }
 
-template <typename T, int L, int C, bool INV, bool PIPE, class Src, class Dst>
+template <typename T, int L, int C, bool INV, bool PIPE, int V, class Src, class Dst>
 void launch_col_LC(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
-  const dim3 block(C * PlanInfo<L>::TPS);
+  const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
   if constexpr (PIPE) {
     const long long nitems = (long long)ntiles * nbatch;
-    auto k = k_col_pipe<T, L, INV, Src, Dst, C>;
+    auto k = k_col_pipe<T, L, INV, Src, Dst, C, V>;
     hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems)), block, 0, c.stream, s, d, c.twiddles<T>(L),
                        ncols, ntiles, nitems);
   } else {
-    hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst, C>), dim3(ntiles, nbatch), block, 0, c.stream, s, d,
+    hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst, C, V>), dim3(ntiles, nbatch), block, 0, c.stream, s, d,
                        c.twiddles<T>(L), ncols);
   }
   check_launch(name);
 }
 
 // Column-pass variants: tile width (the widest tile whose exchange area fits
-// 64 KiB of LDS, ColCfg, or half of it) and plain or persistent-pipelined
-// kernel; the context's tuning table (col_tile) picks one.
-template <typename T, int L, bool INV, class Src, class Dst>
-void launch_col_L(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
-  constexpr int CM = ColCfg<T, L>::C;
-  const int v = c.col_tile(L, (int)sizeof(T), INV, CM);
+// 64 KiB of LDS, ColCfg, or half of it), plain or persistent-pipelined
+// kernel, and radix plan (variant 1 where the length has one); the context's
+// tuning table (col_tile) picks one.
+template <typename T, int L, bool INV, int V, class Src, class Dst>
+void launch_col_LV(Ctx& c, int v, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  constexpr int CM = ColCfg<T, L, V>::C;
   const bool pipe = (v & COL_PIPELINED) != 0;
   if constexpr (CM >= 2) {
-    if ((v & ~COL_PIPELINED) == CM / 2) {
+    if ((v & COL_WIDTH) == CM / 2) {
       if (pipe)
-        launch_col_LC<T, L, CM / 2, INV, true>(c, s, d, ncols, nbatch, name, bytes);
+        launch_col_LC<T, L, CM / 2, INV, true, V>(c, s, d, ncols, nbatch, name, bytes);
       else
-        launch_col_LC<T, L, CM / 2, INV, false>(c, s, d, ncols, nbatch, name, bytes);
+        launch_col_LC<T, L, CM / 2, INV, false, V>(c, s, d, ncols, nbatch, name, bytes);
       return;
     }
   }
   if (pipe)
-    launch_col_LC<T, L, CM, INV, true>(c, s, d, ncols, nbatch, name, bytes);
+    launch_col_LC<T, L, CM, INV, true, V>(c, s, d, ncols, nbatch, name, bytes);
   else
-    launch_col_LC<T, L, CM, INV, false>(c, s, d, ncols, nbatch, name, bytes);
+    launch_col_LC<T, L, CM, INV, false, V>(c, s, d, ncols, nbatch, name, bytes);
+}
+
+// Destinations whose store path needs many registers per element: with the
+// second radix plan's 24 elements per thread the final x pass (DstTruncOp)
+// compiles to 172 VGPRs (2 waves per SIMD) instead of 88 and measured slower
+// (2.33 vs 2.11 ms for the five 768-line x passes of 3LPT at 512^3), so such
+// passes keep plan 0 whatever the tuner chose.
+template <class D>
+struct HeavyDst {
+  static constexpr bool value = false;
+};
+template <typename T>
+struct HeavyDst<DstTruncOp<T>> {
+  static constexpr bool value = true;
+};
+
+template <typename T, int L, bool INV, class Src, class Dst>
+void launch_col_L(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
+  const int v = c.col_tile(L, (int)sizeof(T), INV, ColCfg<T, L>::C, stride_class(s.sp, (int)sizeof(cx<T>)));
+  if constexpr (PlanAlt<L>::value && !HeavyDst<Dst>::value) {
+    if (v & COL_PLAN_ALT) {
+      launch_col_LV<T, L, INV, 1>(c, v, s, d, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
+  launch_col_LV<T, L, INV, 0>(c, v, s, d, ncols, nbatch, name, bytes);
 }
 
 // widest tile for length L, precision T
@@ -482,6 +555,19 @@ int col_tile_max(int L) {
 #undef MFX_CASE
     default:
       throw Error(1, "unsupported transform length " + std::to_string(L));
+  }
+}
+
+// whether length L has a second radix plan
+inline bool col_plan_alt(int L) {
+  switch (L) {
+#define MFX_CASE(LL) \
+  case LL:           \
+    return PlanAlt<LL>::value;
+    MFX_COL_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      return false;
   }
 }
 
@@ -545,41 +631,53 @@ void launch_col_pad(Ctx& c, int L, Src s, Dst d, int ncols, int nbatch, const ch
   }
 }
 
-template <typename T, int L, int C, bool PIPE>
+template <typename T, int L, int C, bool PIPE, int V>
 void launch_turn_LC(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
-  const dim3 block(C * PlanInfo<L>::TPS);
+  const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
   if constexpr (PIPE) {
     const long long nitems = (long long)ntiles * nbatch;
-    auto k = k_col_turn_pipe<T, L, C>;
+    auto k = k_col_turn_pipe<T, L, C, V>;
     hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems)), block, 0, c.stream, a, c.twiddles<T>(L), ncols,
                        ntiles, nitems);
   } else {
-    hipLaunchKernelGGL((k_col_turn<T, L, C>), dim3(ntiles, nbatch), block, 0, c.stream, a, c.twiddles<T>(L), ncols);
+    hipLaunchKernelGGL((k_col_turn<T, L, C, V>), dim3(ntiles, nbatch), block, 0, c.stream, a, c.twiddles<T>(L),
+                       ncols);
   }
   check_launch(name);
+}
+
+template <typename T, int L, int V>
+void launch_turn_LV(Ctx& c, int v, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+  constexpr int CM = ColCfg<T, L, V>::C;
+  const bool pipe = (v & COL_PIPELINED) != 0;
+  if constexpr (CM >= 2) {
+    if ((v & COL_WIDTH) == CM / 2) {
+      if (pipe)
+        launch_turn_LC<T, L, CM / 2, true, V>(c, a, ncols, nbatch, name, bytes);
+      else
+        launch_turn_LC<T, L, CM / 2, false, V>(c, a, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
+  if (pipe)
+    launch_turn_LC<T, L, CM, true, V>(c, a, ncols, nbatch, name, bytes);
+  else
+    launch_turn_LC<T, L, CM, false, V>(c, a, ncols, nbatch, name, bytes);
 }
 
 // at the variant tuned for the inverse transforms of (L, T)
 template <typename T, int L>
 void launch_turn_L(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
-  constexpr int CM = ColCfg<T, L>::C;
-  const int v = c.col_tile(L, (int)sizeof(T), true, CM);
-  const bool pipe = (v & COL_PIPELINED) != 0;
-  if constexpr (CM >= 2) {
-    if ((v & ~COL_PIPELINED) == CM / 2) {
-      if (pipe)
-        launch_turn_LC<T, L, CM / 2, true>(c, a, ncols, nbatch, name, bytes);
-      else
-        launch_turn_LC<T, L, CM / 2, false>(c, a, ncols, nbatch, name, bytes);
+  const int v = c.col_tile(L, (int)sizeof(T), true, ColCfg<T, L>::C, stride_class(a.sp, (int)sizeof(cx<T>)));
+  if constexpr (PlanAlt<L>::value) {
+    if (v & COL_PLAN_ALT) {
+      launch_turn_LV<T, L, 1>(c, v, a, ncols, nbatch, name, bytes);
       return;
     }
   }
-  if (pipe)
-    launch_turn_LC<T, L, CM, true>(c, a, ncols, nbatch, name, bytes);
-  else
-    launch_turn_LC<T, L, CM, false>(c, a, ncols, nbatch, name, bytes);
+  launch_turn_LV<T, L, 0>(c, v, a, ncols, nbatch, name, bytes);
 }
 
 template <typename T>
@@ -602,6 +700,66 @@ void launch_turn(Ctx& c, int L, const TurnArgs<T>& a, int ncols, int nbatch, con
                          c.twiddles<T>(L), ncols, plan, C);
       check_launch(name);
     }
+  }
+}
+
+template <typename T, int L, int C, bool INV, int V>
+void launch_multi_LC(Ctx& c, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+  const dim3 block(C * PlanInfo<L, V>::TPS);
+  const int ntiles = (ncols + C - 1) / C;
+  LaunchScope ls(c, name, bytes);
+  hipLaunchKernelGGL((k_col_multi<T, L, INV, C, V>), dim3(ntiles, nbatch), block, 0, c.stream, a, c.twiddles<T>(L),
+                     ncols);
+  check_launch(name);
+}
+
+template <typename T, int L, bool INV, int V>
+void launch_multi_LV(Ctx& c, int v, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+  constexpr int CM = ColCfg<T, L, V>::C;
+  if constexpr (CM >= 2) {
+    if ((v & COL_WIDTH) == CM / 2) {
+      launch_multi_LC<T, L, CM / 2, INV, V>(c, a, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
+  launch_multi_LC<T, L, CM, INV, V>(c, a, ncols, nbatch, name, bytes);
+}
+
+// at the tile width and plan tuned for (L, T, INV); the multi-output kernel
+// has no pipelined form
+template <typename T, int L, bool INV>
+void launch_multi_L(Ctx& c, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+  const int v = c.col_tile(L, (int)sizeof(T), INV, ColCfg<T, L>::C, stride_class(a.src.sp, (int)sizeof(cx<T>)));
+  if constexpr (PlanAlt<L>::value) {
+    if (v & COL_PLAN_ALT) {
+      launch_multi_LV<T, L, INV, 1>(c, v, a, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
+  launch_multi_LV<T, L, INV, 0>(c, v, a, ncols, nbatch, name, bytes);
+}
+
+// a.no outputs of one source; bytes: the source read once plus a.no outputs.
+// Lengths without tuned kernels run one generic pass per output.
+template <typename T, bool INV>
+void launch_col_multi(Ctx& c, int L, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bsrc,
+                      double bdst) {
+  MFX_REQUIRE(a.no >= 1 && a.no <= 3, "multi-output pass: 1 to 3 outputs");
+  switch (L) {
+#define MFX_CASE(LL)                                                                          \
+  case LL:                                                                                    \
+    launch_multi_L<T, LL, INV>(c, a, ncols, nbatch, name, bsrc + a.no * bdst);                \
+    return;
+    MFX_PAD_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      for (int o = 0; o < a.no; ++o) {
+        SrcPadPow<T> s = a.src;
+        s.pw = a.pw[o];
+        DstPlain<T> d = a.dst;
+        d.p = a.out[o];
+        launch_col_gen<T, INV>(c, L, s, d, ncols, nbatch, name, bsrc + bdst);
+      }
   }
 }
 

@@ -36,12 +36,19 @@ struct Error : std::runtime_error {
 // column-pass variant: the tile width, or'ed with COL_PIPELINED for the
 // persistent software-pipelined kernel
 constexpr int COL_PIPELINED = 0x100;
+constexpr int COL_PLAN_ALT = 0x200;  // the length's second radix plan
+constexpr int COL_WIDTH = 0xff;       // tile width bits of a column variant
+// Stride class of a column pass: 0 when consecutive line elements are close
+// (the y passes, a row of kz columns apart), 1 when they are 64 KiB or more
+// apart (the x passes, a whole plane apart).  The two access patterns load
+// HBM differently and are tuned separately.
+inline int stride_class(long long line_stride, int csize) { return line_stride * csize >= 65536 ? 1 : 0; }
 
 struct TuneRec {
-  int L, tsize, inv;
+  int L, tsize, inv, cls;
   int ncand;
-  int cand[4];
-  double ms[4];
+  int cand[8];
+  double ms[8];
   int chosen;
 };
 
@@ -66,10 +73,11 @@ struct Ctx {
   std::vector<ProfRec> pending;
   std::map<std::string, KStat> stats;
   // column-pass tile width per (length, sizeof(T), inverse): autotuned
-  std::map<std::tuple<int, int, int>, int> coltile;
+  // column-pass variant per (length, element size, direction, stride class)
+  std::map<std::tuple<int, int, int, int>, int> coltile;
   std::vector<TuneRec> tune_log;
-  int col_tile(int L, int tsize, bool inv, int cmax) const {
-    auto it = coltile.find(std::make_tuple(L, tsize, (int)inv));
+  int col_tile(int L, int tsize, bool inv, int cmax, int cls = 0) const {
+    auto it = coltile.find(std::make_tuple(L, tsize, (int)inv, cls));
     return it == coltile.end() ? cmax : it->second;
   }
   int cus = 0;  // compute units of the device (queried once)
@@ -143,5 +151,6 @@ void autotune_columns(Ctx& c, int L);
 int column_length_supported(int L);
 // widest tile of length L for element size tsize (4: FP32, 8: FP64)
 int column_tile_max(int L, int tsize);
+int column_plan_alt(int L);  // 1 if length L has a second radix plan
 
 }  // namespace mfx
