@@ -165,6 +165,21 @@ template <typename T>
 __device__ __forceinline__ cx<T> ld(gcx<T>* p, int i) {
   return mk(p[i].x, p[i].y);
 }
+// the same for a copy of the table in LDS: the asm statement keeps the
+// loads of each transform local to it (as global_table does) and the
+// address space lets them compile to LDS reads
+template <typename T>
+using lcx = __attribute__((address_space(3))) const cx<T>;
+template <typename T>
+__device__ __forceinline__ lcx<T>* lds_table(const cx<T>* p) {
+  lcx<T>* q = (lcx<T>*)p;
+  asm volatile("" : "+s"(q));
+  return q;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> ld(lcx<T>* p, int i) {
+  return mk(p[i].x, p[i].y);
+}
 
 // ---------------------------------------------------------------- plans
 // Radix schedule per length and variant V: EPT must be a multiple of every
@@ -243,8 +258,8 @@ struct StageNS<L, 0, V> {
 
 // Twiddles of stage S for thread t: w[m*(R-1) + q-1] = tw[q k L/(NS R)] for
 // butterfly m (k = (t + TPS m) mod NS) and q = 1..R-1; none for stage 0.
-template <typename T, int L, int S, int V, int EPT>
-__device__ __forceinline__ void stage_twiddles(cx<T> (&w)[EPT], gcx<T>* tw, int t) {
+template <typename T, int L, int S, int V, int EPT, class TP>
+__device__ __forceinline__ void stage_twiddles(cx<T> (&w)[EPT], TP tw, int t) {
   static_assert(EPT == Plan<L, V>::EPT, "register array of the plan");
   if constexpr (S > 0 && S < Plan<L, V>::NR) {
     constexpr int TPS = L / EPT, R = Plan<L, V>::R[S], NB = EPT / R;
@@ -313,6 +328,20 @@ __device__ __forceinline__ void stage_loop(cx<T> (&v)[EPT], cx<T>* lds, int ls, 
   }
 }
 
+// The same with the twiddle table in LDS: each stage reads its own
+// twiddles as it starts (an LDS read is short enough not to be prefetched a
+// stage ahead), which keeps one stage's twiddles in registers instead of two.
+template <typename T, int L, bool INV, int S, int V, int EPT>
+__device__ __forceinline__ void stage_loop_lt(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t, lcx<T>* tw) {
+  if constexpr (S < Plan<L, V>::NR) {
+    constexpr int R = Plan<L, V>::R[S];
+    cx<T> w[EPT];
+    stage_twiddles<T, L, S, V>(w, tw, t);
+    fft_stage<T, L, EPT, R, StageNS<L, S, V>::value, INV, S == Plan<L, V>::NR - 1>(v, lds, ls, t, w);
+    stage_loop_lt<T, L, INV, S + 1, V>(v, lds, ls, t, tw);
+  }
+}
+
 // Full length-L transform (plan variant V) of the registers v (positions
 // t + TPS*e).  lds points at this sequence's L-element exchange area with
 // element stride ls.  All threads of the block must call it (it contains
@@ -323,5 +352,43 @@ __device__ __forceinline__ void fft_exec(cx<T> (&v)[EPT], cx<T>* lds, int ls, in
   cx<T> w0[EPT];
   stage_loop<T, L, INV, 0, V>(v, lds, ls, t, global_table(tw), w0);
 }
+
+// fft_exec with the twiddle table copied to LDS at twl (L entries, visible
+// to every thread: the copy is followed by a barrier before the first call)
+template <typename T, int L, bool INV, int V = 0, int EPT>
+__device__ __forceinline__ void fft_exec_lt(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t, const cx<T>* twl) {
+  static_assert(EPT == Plan<L, V>::EPT, "register array of the plan");
+  stage_loop_lt<T, L, INV, 0, V>(v, lds, ls, t, lds_table(twl));
+}
+
+// Whether a column workgroup of C lines of length L keeps the twiddle table
+// in LDS beside its exchange area (within the 64 KiB of static LDS)
+template <typename T, int L, int C>
+struct TwLds {
+  static constexpr bool value = (long long)(C + 1) * L * (long long)sizeof(cx<T>) <= 65536;
+};
+
+// Copies the twiddle table into LDS (when TwLds holds) and runs the
+// transforms through it; otherwise reads the global table.  Construct once
+// per workgroup, before any transform; the constructor ends with a barrier.
+template <typename T, int L, int C, int V>
+struct ColTw {
+  static constexpr bool LT = TwLds<T, L, C>::value;
+  const cx<T>* g;
+  cx<T>* s;
+  __device__ __forceinline__ ColTw(const cx<T>* tw, cx<T>* lt) : g(tw), s(lt) {
+    if constexpr (LT) {
+      for (int i = threadIdx.x; i < L; i += blockDim.x) s[i] = g[i];
+      __syncthreads();
+    }
+  }
+  template <bool INV, int EPT>
+  __device__ __forceinline__ void exec(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t) const {
+    if constexpr (LT)
+      fft_exec_lt<T, L, INV, V>(v, lds, ls, t, s);
+    else
+      fft_exec<T, L, INV, V>(v, lds, ls, t, g);
+  }
+};
 
 }  // namespace mfx

@@ -228,6 +228,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
+  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
@@ -238,7 +239,8 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   cx<T> v[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
-  fft_exec<T, L, INV, V>(v, lds + c, C, t, tw);
+  const ColTw<T, L, C, V> ft(tw, twl);  // after the loads: their latency overlaps the table copy
+  ft.template exec<INV>(v, lds + c, C, t);
   cx<T> old[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) old[e] = dst.load_old(b, colc, t + TPS * e);
@@ -272,6 +274,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_pipe(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
+  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   long long w, hi;
   int step;
@@ -284,6 +287,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
   };
   fetch(w, nx);
+  const ColTw<T, L, C, V> ft(tw, twl);
   for (; w < hi; w += step) {
     cx<T> v[EPT];
 #pragma unroll
@@ -291,7 +295,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     if (w + step < hi) fetch(w + step, nx);
     // every stage exchange of the previous item ended with a barrier after
     // its last LDS read, so the first exchange of this one may write
-    fft_exec<T, L, INV, V>(v, lds + c, C, t, tw);
+    ft.template exec<INV>(v, lds + c, C, t);
     const int b = (int)(w / ntiles), col = (int)(w % ntiles) * C + c;
     const int colc = col < ncols ? col : ncols - 1;
     cx<T> old[EPT];
@@ -302,15 +306,6 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
       for (int e = 0; e < EPT; ++e) dst.store(b, col, t + TPS * e, v[e], old[e]);
     }
   }
-}
-
-// The table pointer as a value the compiler cannot see through: a second
-// transform in one kernel then loads its twiddles again rather than keeping
-// the first transform's twiddles in registers across it.
-template <typename T>
-__device__ __forceinline__ const cx<T>* opaque(const cx<T>* p) {
-  asm volatile("" : "+s"(p));
-  return p;
 }
 
 // Up to three outputs of one source tile (the x and y stages of the Hessian
@@ -335,12 +330,14 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_multi(MultiPow<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
+  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
   const int col = tile * C + c;
   const bool act = col < ncols;
   const int colc = act ? col : ncols - 1;
+  const ColTw<T, L, C, V> ft(tw, twl);
 #pragma unroll 1
   for (int o = 0; o < a.no; ++o) {
     SrcPadPow<T> s = a.src;
@@ -352,7 +349,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     for (int e = 0; e < EPT; ++e) v[e] = s.load(b, colc, t + TPS * e);
     // the previous output's last exchange ended with a barrier after its
     // last LDS read, so this transform's first exchange may write
-    fft_exec<T, L, INV, V>(v, lds + c, C, t, opaque(tw));
+    ft.template exec<INV>(v, lds + c, C, t);
     if (act) {
 #pragma unroll
       for (int e = 0; e < EPT; ++e) d.store(b, col, t + TPS * e, v[e], v[e]);
@@ -384,6 +381,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
+  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
@@ -394,7 +392,8 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   cx<T> x[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) x[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
-  fft_exec<T, L, false, V>(x, lds + c, C, t, tw);
+  const ColTw<T, L, C, V> ft(tw, twl);
+  ft.template exec<false>(x, lds + c, C, t);
   const bool nyqc = colc == nh;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
@@ -405,7 +404,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
   }
   __syncthreads();  // the forward transform's exchange reads are done
-  fft_exec<T, L, true, V>(x, lds + c, C, t, opaque(tw));
+  ft.template exec<true>(x, lds + c, C, t);
   if (act) {
 #pragma unroll
     for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(t + TPS * e) * a.osp + col] = x[e];
@@ -418,6 +417,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn_pipe(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
+  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   const int nh = a.n >> 1;
   long long w, hi;
@@ -431,6 +431,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     for (int e = 0; e < EPT; ++e) v[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
   };
   fetch(w, nx);
+  const ColTw<T, L, C, V> ft(tw, twl);
   for (; w < hi; w += step) {
     cx<T> x[EPT];
 #pragma unroll
@@ -438,7 +439,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     if (w + step < hi) fetch(w + step, nx);
     const int b = (int)(w / ntiles), col = (int)(w % ntiles) * C + c;
     const int colc = col < ncols ? col : ncols - 1;
-    fft_exec<T, L, false, V>(x, lds + c, C, t, tw);
+    ft.template exec<false>(x, lds + c, C, t);
     const bool nyqc = colc == nh;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -448,7 +449,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
       x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
     }
     __syncthreads();  // the forward transform's exchange reads are done
-    fft_exec<T, L, true, V>(x, lds + c, C, t, opaque(tw));
+    ft.template exec<true>(x, lds + c, C, t);
     if (col < ncols) {
 #pragma unroll
       for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(t + TPS * e) * a.osp + col] = x[e];
