@@ -361,33 +361,28 @@ __device__ __forceinline__ void fft_exec_lt(cx<T> (&v)[EPT], cx<T>* lds, int ls,
   stage_loop_lt<T, L, INV, 0, V>(v, lds, ls, t, lds_table(twl));
 }
 
-// Whether a column workgroup of C lines of length L keeps the twiddle table
-// in LDS beside its exchange area (within the 64 KiB of static LDS)
+// Whether the twiddle table of a column workgroup of C lines of length L fits
+// in static LDS beside its exchange area (64 KiB); otherwise the table is
+// placed in dynamic LDS, for which the launch passes L complex values.
 template <typename T, int L, int C>
 struct TwLds {
   static constexpr bool value = (long long)(C + 1) * L * (long long)sizeof(cx<T>) <= 65536;
+  static constexpr size_t dyn = value ? 0 : (size_t)L * sizeof(cx<T>);
 };
 
-// Copies the twiddle table into LDS (when TwLds holds) and runs the
-// transforms through it; otherwise reads the global table.  Construct once
-// per workgroup, before any transform; the constructor ends with a barrier.
-template <typename T, int L, int C, int V>
+// Copies the twiddle table into LDS at lt and runs the transforms through
+// it.  Construct once per workgroup, before any transform; the constructor
+// ends with a barrier.
+template <typename T, int L, int V>
 struct ColTw {
-  static constexpr bool LT = TwLds<T, L, C>::value;
-  const cx<T>* g;
   cx<T>* s;
-  __device__ __forceinline__ ColTw(const cx<T>* tw, cx<T>* lt) : g(tw), s(lt) {
-    if constexpr (LT) {
-      for (int i = threadIdx.x; i < L; i += blockDim.x) s[i] = g[i];
-      __syncthreads();
-    }
+  __device__ __forceinline__ ColTw(const cx<T>* tw, cx<T>* lt) : s(lt) {
+    for (int i = threadIdx.x; i < L; i += blockDim.x) s[i] = tw[i];
+    __syncthreads();
   }
   template <bool INV, int EPT>
   __device__ __forceinline__ void exec(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t) const {
-    if constexpr (LT)
-      fft_exec_lt<T, L, INV, V>(v, lds, ls, t, s);
-    else
-      fft_exec<T, L, INV, V>(v, lds, ls, t, g);
+    fft_exec_lt<T, L, INV, V>(v, lds, ls, t, s);
   }
 };
 

@@ -228,7 +228,9 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];
+  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   cx<T> v[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
-  const ColTw<T, L, C, V> ft(tw, twl);  // after the loads: their latency overlaps the table copy
+  const ColTw<T, L, V> ft(tw, twl);  // after the loads: their latency overlaps the table copy
   ft.template exec<INV>(v, lds + c, C, t);
   cx<T> old[EPT];
 #pragma unroll
@@ -274,7 +276,9 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_pipe(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];
+  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   long long w, hi;
   int step;
@@ -287,7 +291,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
   };
   fetch(w, nx);
-  const ColTw<T, L, C, V> ft(tw, twl);
+  const ColTw<T, L, V> ft(tw, twl);
   for (; w < hi; w += step) {
     cx<T> v[EPT];
 #pragma unroll
@@ -330,14 +334,16 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_multi(MultiPow<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];
+  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
   const int col = tile * C + c;
   const bool act = col < ncols;
   const int colc = act ? col : ncols - 1;
-  const ColTw<T, L, C, V> ft(tw, twl);
+  const ColTw<T, L, V> ft(tw, twl);
 #pragma unroll 1
   for (int o = 0; o < a.no; ++o) {
     SrcPadPow<T> s = a.src;
@@ -381,7 +387,9 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];
+  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
@@ -392,7 +400,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   cx<T> x[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) x[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
-  const ColTw<T, L, C, V> ft(tw, twl);
+  const ColTw<T, L, V> ft(tw, twl);
   ft.template exec<false>(x, lds + c, C, t);
   const bool nyqc = colc == nh;
 #pragma unroll
@@ -417,7 +425,9 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn_pipe(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];
+  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   const int nh = a.n >> 1;
   long long w, hi;
@@ -431,7 +441,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     for (int e = 0; e < EPT; ++e) v[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
   };
   fetch(w, nx);
-  const ColTw<T, L, C, V> ft(tw, twl);
+  const ColTw<T, L, V> ft(tw, twl);
   for (; w < hi; w += step) {
     cx<T> x[EPT];
 #pragma unroll
@@ -466,11 +476,11 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
 // device (occupancy x compute units), a multiple of the 8 XCDs, at most the
 // number of work items rounded up to 8.
 template <class K>
-unsigned pipe_grid(Ctx& c, K kernel, int block, long long nitems) {
+unsigned pipe_grid(Ctx& c, K kernel, int block, long long nitems, size_t dyn = 0) {
   int& per_cu = c.occupancy[(const void*)kernel];  // one query per kernel
   if (per_cu == 0) {
     int n = 0;
-    MFX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, 0));
+    MFX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, dyn));
     per_cu = n > 0 ? n : 1;
   }
   long long g = (long long)per_cu * c.num_cus();
@@ -479,19 +489,41 @@ unsigned pipe_grid(Ctx& c, K kernel, int block, long long nitems) {
   return (unsigned)((g + 7) / 8 * 8);
 }
 
+// A column kernel whose twiddle table goes to dynamic LDS (TwLds) may need
+// more than the default 64 KiB per workgroup in total; allow it once per
+// kernel.
+template <class K>
+void col_lds_prep(Ctx& c, K kernel, size_t stat, size_t dyn) {
+  if (dyn == 0 || stat + dyn <= 65536) return;
+  MFX_REQUIRE(stat + dyn <= (size_t)GEN_LDS_MAX, "column tile too large for one compute unit's LDS");
+  int& done = c.lds_attr[(const void*)kernel];
+  if (done < (int)dyn) {
+    MFX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    done = (int)dyn;
+  }
+}
+
+template <typename T, int L, int C>
+constexpr size_t col_static_lds() {
+  return (size_t)(L * C + (TwLds<T, L, C>::value ? L : 1)) * sizeof(cx<T>);
+}
+
 template <typename T, int L, int C, bool INV, bool PIPE, int V, class Src, class Dst>
 void launch_col_LC(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
   const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
+  constexpr size_t dyn = TwLds<T, L, C>::dyn;
   if constexpr (PIPE) {
     const long long nitems = (long long)ntiles * nbatch;
     auto k = k_col_pipe<T, L, INV, Src, Dst, C, V>;
-    hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems)), block, 0, c.stream, s, d, c.twiddles<T>(L),
-                       ncols, ntiles, nitems);
+    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems, dyn)), block, dyn, c.stream, s, d,
+                       c.twiddles<T>(L), ncols, ntiles, nitems);
   } else {
-    hipLaunchKernelGGL((k_col<T, L, INV, Src, Dst, C, V>), dim3(ntiles, nbatch), block, 0, c.stream, s, d,
-                       c.twiddles<T>(L), ncols);
+    auto k = k_col<T, L, INV, Src, Dst, C, V>;
+    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    hipLaunchKernelGGL(k, dim3(ntiles, nbatch), block, dyn, c.stream, s, d, c.twiddles<T>(L), ncols);
   }
   check_launch(name);
 }
@@ -578,7 +610,7 @@ template <class K>
 void gen_lds_prep(Ctx& c, K kernel, size_t lds) {
   MFX_REQUIRE(lds <= (size_t)GEN_LDS_MAX, "transform length too long for one compute unit's LDS");
   if (lds <= 65536) return;
-  int& done = c.occupancy[(const void*)kernel];
+  int& done = c.lds_attr[(const void*)kernel];
   if (done < (int)lds) {
     MFX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     done = (int)lds;
@@ -637,14 +669,17 @@ void launch_turn_LC(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const c
   const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
+  constexpr size_t dyn = TwLds<T, L, C>::dyn;
   if constexpr (PIPE) {
     const long long nitems = (long long)ntiles * nbatch;
     auto k = k_col_turn_pipe<T, L, C, V>;
-    hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems)), block, 0, c.stream, a, c.twiddles<T>(L), ncols,
-                       ntiles, nitems);
+    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems, dyn)), block, dyn, c.stream, a,
+                       c.twiddles<T>(L), ncols, ntiles, nitems);
   } else {
-    hipLaunchKernelGGL((k_col_turn<T, L, C, V>), dim3(ntiles, nbatch), block, 0, c.stream, a, c.twiddles<T>(L),
-                       ncols);
+    auto k = k_col_turn<T, L, C, V>;
+    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    hipLaunchKernelGGL(k, dim3(ntiles, nbatch), block, dyn, c.stream, a, c.twiddles<T>(L), ncols);
   }
   check_launch(name);
 }
@@ -709,8 +744,10 @@ void launch_multi_LC(Ctx& c, const MultiPow<T>& a, int ncols, int nbatch, const 
   const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
-  hipLaunchKernelGGL((k_col_multi<T, L, INV, C, V>), dim3(ntiles, nbatch), block, 0, c.stream, a, c.twiddles<T>(L),
-                     ncols);
+  constexpr size_t dyn = TwLds<T, L, C>::dyn;
+  auto k = k_col_multi<T, L, INV, C, V>;
+  col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+  hipLaunchKernelGGL(k, dim3(ntiles, nbatch), block, dyn, c.stream, a, c.twiddles<T>(L), ncols);
   check_launch(name);
 }
 

@@ -83,6 +83,7 @@ struct Ctx {
   int cus = 0;  // compute units of the device (queried once)
   int num_cus();
   std::map<const void*, int> occupancy;  // resident workgroups per CU, per kernel
+  std::map<const void*, int> lds_attr;   // dynamic LDS bytes allowed, per kernel
   ~Ctx();
   template <typename T>
   const cx<T>* twiddles(int L);
