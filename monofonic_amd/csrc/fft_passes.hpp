@@ -350,15 +350,21 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     s.pw = o == 0 ? a.pw[0] : (o == 1 ? a.pw[1] : a.pw[2]);
     DstPlain<T> d = a.dst;
     d.p = o == 0 ? a.out[0] : (o == 1 ? a.out[1] : a.out[2]);
+    // the line position and column pass through an empty asm statement per
+    // output, so that the element addresses and wavenumber factors are
+    // recomputed for each output instead of being held in registers across
+    // the loop (about 60 VGPRs, which would halve the resident workgroups)
+    int tq = t, cq = colc, colq = col;
+    asm volatile("" : "+v"(tq), "+v"(cq), "+v"(colq));
     cx<T> v[EPT];
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) v[e] = s.load(b, colc, t + TPS * e);
+    for (int e = 0; e < EPT; ++e) v[e] = s.load(b, cq, tq + TPS * e);
     // the previous output's last exchange ended with a barrier after its
     // last LDS read, so this transform's first exchange may write
     ft.template exec<INV>(v, lds + c, C, t);
     if (act) {
 #pragma unroll
-      for (int e = 0; e < EPT; ++e) d.store(b, col, t + TPS * e, v[e], v[e]);
+      for (int e = 0; e < EPT; ++e) d.store(b, colq, tq + TPS * e, v[e], v[e]);
     }
   }
 }
