@@ -284,19 +284,23 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   int step;
   if (!pipe_range(nitems, w, hi, step)) return;
   cx<T> nx[EPT];
-  auto fetch = [&](long long item, cx<T>(&v)[EPT]) {
+  auto fetch = [&](long long item, int tq, cx<T>(&v)[EPT]) {
     const int b = (int)(item / ntiles), col = (int)(item % ntiles) * C + c;
     const int colc = col < ncols ? col : ncols - 1;
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
+    for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, tq + TPS * e);
   };
-  fetch(w, nx);
+  fetch(w, t, nx);
   const ColTw<T, L, V> ft(tw, twl);
   for (; w < hi; w += step) {
+    // per-element positions recomputed each item (an empty asm statement
+    // hides that they do not change), not held in registers across the loop
+    int tq = t;
+    asm volatile("" : "+v"(tq));
     cx<T> v[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) v[e] = nx[e];
-    if (w + step < hi) fetch(w + step, nx);
+    if (w + step < hi) fetch(w + step, tq, nx);
     // every stage exchange of the previous item ended with a barrier after
     // its last LDS read, so the first exchange of this one may write
     ft.template exec<INV>(v, lds + c, C, t);
@@ -304,10 +308,10 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     const int colc = col < ncols ? col : ncols - 1;
     cx<T> old[EPT];
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) old[e] = dst.load_old(b, colc, t + TPS * e);
+    for (int e = 0; e < EPT; ++e) old[e] = dst.load_old(b, colc, tq + TPS * e);
     if (col < ncols) {
 #pragma unroll
-      for (int e = 0; e < EPT; ++e) dst.store(b, col, t + TPS * e, v[e], old[e]);
+      for (int e = 0; e < EPT; ++e) dst.store(b, col, tq + TPS * e, v[e], old[e]);
     }
   }
 }
@@ -440,26 +444,28 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   int step;
   if (!pipe_range(nitems, w, hi, step)) return;
   cx<T> nx[EPT];
-  auto fetch = [&](long long item, cx<T>(&v)[EPT]) {
+  auto fetch = [&](long long item, int tq, cx<T>(&v)[EPT]) {
     const int b = (int)(item / ntiles), col = (int)(item % ntiles) * C + c;
     const int colc = col < ncols ? col : ncols - 1;
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) v[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
+    for (int e = 0; e < EPT; ++e) v[e] = a.src[(long long)b * a.sb + (long long)(tq + TPS * e) * a.sp + colc];
   };
-  fetch(w, nx);
+  fetch(w, t, nx);
   const ColTw<T, L, V> ft(tw, twl);
   for (; w < hi; w += step) {
+    int tq = t;  // recomputed per item (see k_col_pipe)
+    asm volatile("" : "+v"(tq));
     cx<T> x[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) x[e] = nx[e];
-    if (w + step < hi) fetch(w + step, nx);
+    if (w + step < hi) fetch(w + step, tq, nx);
     const int b = (int)(w / ntiles), col = (int)(w % ntiles) * C + c;
     const int colc = col < ncols ? col : ncols - 1;
     ft.template exec<false>(x, lds + c, C, t);
     const bool nyqc = colc == nh;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int pos = t + TPS * e;
+      const int pos = tq + TPS * e;
       const bool zero = ((pos > nh) & (pos <= a.n)) | (pos == nh) | nyqc;
       const cx<T> o = x[e] * a.fscale;
       x[e] = zero ? mk(T(0), T(0)) : o * a.iscale;
@@ -468,7 +474,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     ft.template exec<true>(x, lds + c, C, t);
     if (col < ncols) {
 #pragma unroll
-      for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(t + TPS * e) * a.osp + col] = x[e];
+      for (int e = 0; e < EPT; ++e) a.out[(long long)b * a.osb + (long long)(tq + TPS * e) * a.osp + col] = x[e];
     }
   }
 }

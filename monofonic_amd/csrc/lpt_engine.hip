@@ -589,7 +589,7 @@ struct Engine {
 
   // y stage: line along j of T1 [x][j][kz], batch x, output [x][y][kz]; one
   // launch per output (a multi-output launch that re-reads its source tile
-  // from the L2 measured slower here than separate launches: 7.8 vs 7.2 ms
+  // from the L2 measured slower here than separate launches: 6.2 vs 5.8 ms
   // for the twelve 768-line outputs of 3LPT at 512^3)
   void y_inverse(bool hessian, cx<T>* const* Y) {
     const int n = l.n, m = l.m, hp = l.hp;
