@@ -180,6 +180,17 @@ template <typename T>
 __device__ __forceinline__ cx<T> ld(lcx<T>* p, int i) {
   return mk(p[i].x, p[i].y);
 }
+// An LDS table of the first H = L/2 twiddles only: w^(i + L/2) = -w^i
+template <typename T, int H>
+struct HalfTw {
+  lcx<T>* p;
+};
+template <typename T, int H>
+__device__ __forceinline__ cx<T> ld(HalfTw<T, H> h, int i) {
+  const bool up = i >= H;
+  const cx<T> v = ld(h.p, up ? i - H : i);
+  return up ? mk(-v.x, -v.y) : v;
+}
 
 // ---------------------------------------------------------------- plans
 // Radix schedule per length and variant V: EPT must be a multiple of every
@@ -331,8 +342,8 @@ __device__ __forceinline__ void stage_loop(cx<T> (&v)[EPT], cx<T>* lds, int ls, 
 // The same with the twiddle table in LDS: each stage reads its own
 // twiddles as it starts (an LDS read is short enough not to be prefetched a
 // stage ahead), which keeps one stage's twiddles in registers instead of two.
-template <typename T, int L, bool INV, int S, int V, int EPT>
-__device__ __forceinline__ void stage_loop_lt(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t, lcx<T>* tw) {
+template <typename T, int L, bool INV, int S, int V, int EPT, class TP>
+__device__ __forceinline__ void stage_loop_lt(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t, TP tw) {
   if constexpr (S < Plan<L, V>::NR) {
     constexpr int R = Plan<L, V>::R[S];
     cx<T> w[EPT];
@@ -353,36 +364,48 @@ __device__ __forceinline__ void fft_exec(cx<T> (&v)[EPT], cx<T>* lds, int ls, in
   stage_loop<T, L, INV, 0, V>(v, lds, ls, t, global_table(tw), w0);
 }
 
-// fft_exec with the twiddle table copied to LDS at twl (L entries, visible
-// to every thread: the copy is followed by a barrier before the first call)
-template <typename T, int L, bool INV, int V = 0, int EPT>
+// fft_exec with the twiddle table copied to LDS at twl (L entries, or the
+// first L/2 with HALF; visible to every thread: the copy is followed by a
+// barrier before the first call)
+template <typename T, int L, bool INV, int V = 0, bool HALF = false, int EPT>
 __device__ __forceinline__ void fft_exec_lt(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t, const cx<T>* twl) {
   static_assert(EPT == Plan<L, V>::EPT, "register array of the plan");
-  stage_loop_lt<T, L, INV, 0, V>(v, lds, ls, t, lds_table(twl));
+  if constexpr (HALF)
+    stage_loop_lt<T, L, INV, 0, V>(v, lds, ls, t, HalfTw<T, L / 2>{lds_table(twl)});
+  else
+    stage_loop_lt<T, L, INV, 0, V>(v, lds, ls, t, lds_table(twl));
 }
 
-// Whether the twiddle table of a column workgroup of C lines of length L fits
-// in static LDS beside its exchange area (64 KiB); otherwise the table is
-// placed in dynamic LDS, for which the launch passes L complex values.
+// LDS layout of a column workgroup of C lines of length L: the exchange area
+// (L*C values) and the twiddle table.  The table goes to static LDS beside
+// the exchange area when both fit in 64 KiB, else to dynamic LDS (the launch
+// passes `dyn` bytes).  Where a full table would keep a third workgroup off
+// a compute unit (160 KiB of LDS) and half of it would not, only the first
+// L/2 twiddles are kept (HalfTw): e.g. 768-line FP32 tiles of 8 columns,
+// 51 instead of 54 KiB per workgroup, three workgroups per CU instead of two.
 template <typename T, int L, int C>
 struct TwLds {
-  static constexpr bool value = (long long)(C + 1) * L * (long long)sizeof(cx<T>) <= 65536;
-  static constexpr size_t dyn = value ? 0 : (size_t)L * sizeof(cx<T>);
+  static constexpr long long cs = sizeof(cx<T>), cu = 160 * 1024;
+  static constexpr bool half = 3 * ((long long)L * C + L) * cs > cu && 3 * ((long long)L * C + L / 2) * cs <= cu;
+  static constexpr int entries = half ? L / 2 : L;
+  static constexpr bool value = ((long long)L * C + entries) * cs <= 65536;
+  static constexpr size_t dyn = value ? 0 : (size_t)entries * sizeof(cx<T>);
 };
 
-// Copies the twiddle table into LDS at lt and runs the transforms through
-// it.  Construct once per workgroup, before any transform; the constructor
-// ends with a barrier.
-template <typename T, int L, int V>
+// Copies the twiddle table (its first TwLds::entries values) into LDS at lt
+// and runs the transforms through it.  Construct once per workgroup, before
+// any transform; the constructor ends with a barrier.
+template <typename T, int L, int C, int V>
 struct ColTw {
+  static constexpr bool HALF = TwLds<T, L, C>::half;
   cx<T>* s;
   __device__ __forceinline__ ColTw(const cx<T>* tw, cx<T>* lt) : s(lt) {
-    for (int i = threadIdx.x; i < L; i += blockDim.x) s[i] = tw[i];
+    for (int i = threadIdx.x; i < TwLds<T, L, C>::entries; i += blockDim.x) s[i] = tw[i];
     __syncthreads();
   }
   template <bool INV, int EPT>
   __device__ __forceinline__ void exec(cx<T> (&v)[EPT], cx<T>* lds, int ls, int t) const {
-    fft_exec_lt<T, L, INV, V>(v, lds, ls, t, s);
+    fft_exec_lt<T, L, INV, V, HALF>(v, lds, ls, t, s);
   }
 };
 

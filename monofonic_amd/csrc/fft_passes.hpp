@@ -228,7 +228,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
   extern __shared__ __align__(16) unsigned char col_lds_raw[];
   cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   cx<T> v[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, t + TPS * e);
-  const ColTw<T, L, V> ft(tw, twl);  // after the loads: their latency overlaps the table copy
+  const ColTw<T, L, C, V> ft(tw, twl);  // after the loads: their latency overlaps the table copy
   ft.template exec<INV>(v, lds + c, C, t);
   cx<T> old[EPT];
 #pragma unroll
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_pipe(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
   extern __shared__ __align__(16) unsigned char col_lds_raw[];
   cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     for (int e = 0; e < EPT; ++e) v[e] = src.load(b, colc, tq + TPS * e);
   };
   fetch(w, t, nx);
-  const ColTw<T, L, V> ft(tw, twl);
+  const ColTw<T, L, C, V> ft(tw, twl);
   for (; w < hi; w += step) {
     // per-element positions recomputed each item (an empty asm statement
     // hides that they do not change), not held in registers across the loop
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_multi(MultiPow<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
   extern __shared__ __align__(16) unsigned char col_lds_raw[];
   cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   const int col = tile * C + c;
   const bool act = col < ncols;
   const int colc = act ? col : ncols - 1;
-  const ColTw<T, L, V> ft(tw, twl);
+  const ColTw<T, L, C, V> ft(tw, twl);
 #pragma unroll 1
   for (int o = 0; o < a.no; ++o) {
     SrcPadPow<T> s = a.src;
@@ -397,7 +397,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
   extern __shared__ __align__(16) unsigned char col_lds_raw[];
   cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -410,7 +410,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   cx<T> x[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) x[e] = a.src[(long long)b * a.sb + (long long)(t + TPS * e) * a.sp + colc];
-  const ColTw<T, L, V> ft(tw, twl);
+  const ColTw<T, L, C, V> ft(tw, twl);
   ft.template exec<false>(x, lds + c, C, t);
   const bool nyqc = colc == nh;
 #pragma unroll
@@ -435,7 +435,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn_pipe(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? L : 1];
+  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
   extern __shared__ __align__(16) unsigned char col_lds_raw[];
   cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
   const int c = threadIdx.x % C, t = threadIdx.x / C;
@@ -451,7 +451,7 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     for (int e = 0; e < EPT; ++e) v[e] = a.src[(long long)b * a.sb + (long long)(tq + TPS * e) * a.sp + colc];
   };
   fetch(w, t, nx);
-  const ColTw<T, L, V> ft(tw, twl);
+  const ColTw<T, L, C, V> ft(tw, twl);
   for (; w < hi; w += step) {
     int tq = t;  // recomputed per item (see k_col_pipe)
     asm volatile("" : "+v"(tq));
@@ -517,7 +517,7 @@ void col_lds_prep(Ctx& c, K kernel, size_t stat, size_t dyn) {
 
 template <typename T, int L, int C>
 constexpr size_t col_static_lds() {
-  return (size_t)(L * C + (TwLds<T, L, C>::value ? L : 1)) * sizeof(cx<T>);
+  return (size_t)(L * C + (TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1)) * sizeof(cx<T>);
 }
 
 template <typename T, int L, int C, bool INV, bool PIPE, int V, class Src, class Dst>
