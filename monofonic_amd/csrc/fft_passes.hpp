@@ -775,17 +775,12 @@ void launch_multi_LV(Ctx& c, int v, const MultiPow<T>& a, int ncols, int nbatch,
   launch_multi_LC<T, L, CM, INV, V>(c, a, ncols, nbatch, name, bytes);
 }
 
-// at the tile width and plan tuned for (L, T, INV); the multi-output kernel
-// has no pipelined form
+// at the tile width tuned for (L, T, INV); the multi-output kernel has no
+// pipelined form and always runs plan 0 (with the second plan it compiles to
+// 179 instead of 95 VGPRs, two waves per SIMD instead of five)
 template <typename T, int L, bool INV>
 void launch_multi_L(Ctx& c, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
   const int v = c.col_tile(L, (int)sizeof(T), INV, ColCfg<T, L>::C, stride_class(a.src.sp, (int)sizeof(cx<T>)));
-  if constexpr (PlanAlt<L>::value) {
-    if (v & COL_PLAN_ALT) {
-      launch_multi_LV<T, L, INV, 1>(c, v, a, ncols, nbatch, name, bytes);
-      return;
-    }
-  }
   launch_multi_LV<T, L, INV, 0>(c, v, a, ncols, nbatch, name, bytes);
 }
 
