@@ -73,14 +73,15 @@ int mfx_memcpy_h2d(mfx_ctx *c, void *dev, const void *host, size_t bytes);
 /* ------------------------------------------------------------ autotuning
  * No counterpart in the reference (FFTW plans are made with FFTW_ESTIMATE,
  * src/grid_fft.cc:53-54).  Times the column-pass variants of lengths n and
- * 3n/2 in `precision` -- tile width (widest feasible or half), plain or
+ * 3n/2 in `precision` -- tile width (64 KiB tile, half or twice), plain or
  * persistent pipelined kernel, and for lengths 384, 768, 1024 and 1536 the
  * second radix plan -- and keeps the fastest per length and direction in
  * the context.  Tile width and kernel form never change results; the radix
  * plan changes them at the level of rounding. */
 int mfx_autotune(mfx_ctx *c, size_t n, int precision);
 /* set the column-pass variant of length L explicitly: the tile width (the
- * widest feasible tile or half of it), or'ed with MFX_COL_PIPELINED for the
+ * widest tile whose exchange area fits 64 KiB of LDS, half of it, or -- where
+ * one workgroup can hold it -- twice it), or'ed with MFX_COL_PIPELINED for the
  * persistent pipelined kernel and with MFX_COL_PLAN_ALT for the length's
  * second radix plan (384 = 12.8.4, 768 = 12.8.8, 1024 = 16.16.4,
  * 1536 = 12.8.8.2 instead of 12.4.4.2, 12.4.4.4, 8.8.4.4, 12.4.4.4.2) */

@@ -1169,8 +1169,10 @@ int mfx_set_column_tile(mfx_ctx* c, int L, int precision, int inverse, int tile)
     const int cm = column_tile_max(L, precision);
     const int w = tile & COL_WIDTH;
     const int flags = COL_PIPELINED | (column_plan_alt(L) ? COL_PLAN_ALT : 0);
-    MFX_REQUIRE((tile & ~(flags | COL_WIDTH)) == 0 && (w == cm || (cm >= 2 && w == cm / 2)),
-                "tile must be " + std::to_string(cm) + (cm >= 2 ? " or " + std::to_string(cm / 2) : std::string()) +
+    const int cw = column_tile_wide(L, precision, (tile & COL_PLAN_ALT) ? 1 : 0);
+    MFX_REQUIRE((tile & ~(flags | COL_WIDTH)) == 0 && (w == cm || (cm >= 2 && w == cm / 2) || (cw > 0 && w == cw)),
+                "tile must be " + std::to_string(cm) + (cm >= 2 ? ", " + std::to_string(cm / 2) : std::string()) +
+                    (cw > 0 ? " or " + std::to_string(cw) : std::string()) +
                     ", optionally or'ed with MFX_COL_PIPELINED" +
                     (column_plan_alt(L) ? " and MFX_COL_PLAN_ALT" : " (this length has one radix plan)"));
     for (int cls = 0; cls < 2; ++cls) c->c.coltile[std::make_tuple(L, precision, inverse ? 1 : 0, cls)] = tile;
@@ -1187,7 +1189,9 @@ int mfx_autotune_report(mfx_ctx* c, char* buf, size_t len) {
     };
     for (const auto& r : c->c.tune_log) {
       std::snprintf(line, sizeof line, "L=%d %s %s%s:", r.L, r.tsize == 4 ? "f32" : "f64",
-                    r.inv ? "inverse" : "forward", r.cls ? " plane-strided" : "");
+                    r.inv == COL_TURN ? "turn" : r.inv == COL_TRUNC_OP ? "truncating forward"
+                                                : (r.inv ? "inverse" : "forward"),
+                    r.cls ? " plane-strided" : "");
       out += line;
       for (int q = 0; q < r.ncand; ++q) {
         std::snprintf(line, sizeof line, " %s %.4f ms%s", name(r.cand[q]).c_str(), r.ms[q], q + 1 < r.ncand ? "," : "");

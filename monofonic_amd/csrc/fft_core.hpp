@@ -376,20 +376,18 @@ __device__ __forceinline__ void fft_exec_lt(cx<T> (&v)[EPT], cx<T>* lds, int ls,
     stage_loop_lt<T, L, INV, 0, V>(v, lds, ls, t, lds_table(twl));
 }
 
-// LDS layout of a column workgroup of C lines of length L: the exchange area
-// (L*C values) and the twiddle table.  The table goes to static LDS beside
-// the exchange area when both fit in 64 KiB, else to dynamic LDS (the launch
-// passes `dyn` bytes).  Where a full table would keep a third workgroup off
-// a compute unit (160 KiB of LDS) and half of it would not, only the first
-// L/2 twiddles are kept (HalfTw): e.g. 768-line FP32 tiles of 8 columns,
-// 51 instead of 54 KiB per workgroup, three workgroups per CU instead of two.
+// LDS layout of a column workgroup of C lines of length L, all in dynamic LDS:
+// the exchange area (L*C values) followed by the twiddle table.  Where a full
+// table would keep a third workgroup off a compute unit (160 KiB of LDS) and
+// half of it would not, only the first L/2 twiddles are kept (HalfTw): e.g.
+// 768-line FP32 tiles of 8 columns, 51 instead of 54 KiB per workgroup, three
+// workgroups per CU instead of two.
 template <typename T, int L, int C>
 struct TwLds {
   static constexpr long long cs = sizeof(cx<T>), cu = 160 * 1024;
   static constexpr bool half = 3 * ((long long)L * C + L) * cs > cu && 3 * ((long long)L * C + L / 2) * cs <= cu;
   static constexpr int entries = half ? L / 2 : L;
-  static constexpr bool value = ((long long)L * C + entries) * cs <= 65536;
-  static constexpr size_t dyn = value ? 0 : (size_t)entries * sizeof(cx<T>);
+  static constexpr size_t bytes = (size_t)((long long)L * C + entries) * sizeof(cx<T>);
 };
 
 // Copies the twiddle table (its first TwLds::entries values) into LDS at lt

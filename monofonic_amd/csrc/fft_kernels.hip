@@ -132,11 +132,13 @@ void conv2(Ctx& c, const cx<T>* fa, const MultSpec& ma, const cx<T>* fb, const M
 // ------------------------------------------------------------------ autotuning
 // Column-pass variant: the widest tile whose exchange area fits the LDS
 // budget keeps 8-16 adjacent columns per line access (wider HBM segments),
-// half of it doubles the workgroups per CU; the persistent pipelined kernel
+// half of it doubles the workgroups per CU, twice it (where a workgroup can
+// hold it, in up to 160 KiB of LDS) doubles the run length again; the
+// persistent pipelined kernel
 // keeps the next tile's loads in flight while a tile is transformed; a
 // length's second radix plan (where it has one) trades an LDS exchange for
 // more elements per thread.  Which is faster depends on the length and
-// precision, so the (up to) eight combinations are timed on a block of the library's own shape -- L/2+1
+// precision, so the (up to) twelve combinations are timed on a block of the library's own shape -- L/2+1
 // adjacent columns, up to 256 batches -- in both directions, five launches
 // each after two warm-ups, and the fastest is kept.
 template <typename T>
@@ -153,46 +155,63 @@ void autotune_columns(Ctx& c, int L) {
     MFX_HIP(hipMemsetAsync(buf, 0, sizeof(cx<T>) * cnt, c.stream));
     MFX_HIP(hipEventCreate(&e0));
     MFX_HIP(hipEventCreate(&e1));
+    // times every candidate variant of (direction, stride class) with
+    // pass(), keeps the fastest and logs the timings
+    auto tune = [&](int dir, int cls, bool alts, auto&& pass) {
+      TuneRec rec{L, (int)sizeof(T), dir, cls, 0, {}, {}, cm};
+      for (int alt = 0; alt < (alts && col_plan_alt(L) ? 2 : 1); ++alt) {
+        const int widths[3] = {cm, cm >= 2 ? cm / 2 : 0, col_tile_wide<T>(L, alt)};
+        for (int pipe = 0; pipe < 2; ++pipe)
+          for (int w : widths)
+            if (w > 0) rec.cand[rec.ncand++] = w | (pipe ? COL_PIPELINED : 0) | (alt ? COL_PLAN_ALT : 0);
+      }
+      double best = 1e300;
+      for (int q = 0; q < rec.ncand; ++q) {
+        c.coltile[std::make_tuple(L, (int)sizeof(T), dir, cls)] = rec.cand[q];
+        pass();
+        pass();
+        MFX_HIP(hipEventRecord(e0, c.stream));
+        for (int r = 0; r < 5; ++r) pass();
+        MFX_HIP(hipEventRecord(e1, c.stream));
+        MFX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        rec.ms[q] = ms / 5.0;
+        if (rec.ms[q] < best) {
+          best = rec.ms[q];
+          rec.chosen = rec.cand[q];
+        }
+      }
+      c.coltile[std::make_tuple(L, (int)sizeof(T), dir, cls)] = rec.chosen;
+      c.tune_log.push_back(rec);
+    };
     for (int cls = 0; cls < 2; ++cls) {
       // class 0: lines of L elements a row of columns apart ([batch][line][col]);
       // class 1: a plane apart ([line][batch][col]); only where the plane is
       // large enough to be class 1 at all
       const long long sb = cls ? ncols : (long long)L * ncols, sp = cls ? (long long)nb * ncols : ncols;
       if (stride_class(sp, (int)sizeof(cx<T>)) != cls) continue;
-      for (int inv = 0; inv < 2; ++inv) {
-        TuneRec rec{L, (int)sizeof(T), inv, cls, 0, {}, {}, cm};
-        for (int alt = 0; alt < (col_plan_alt(L) ? 2 : 1); ++alt)
-          for (int pipe = 0; pipe < 2; ++pipe)
-            for (int h = 0; h < (cm >= 2 ? 2 : 1); ++h)
-              rec.cand[rec.ncand++] = (h ? cm / 2 : cm) | (pipe ? COL_PIPELINED : 0) | (alt ? COL_PLAN_ALT : 0);
-        const int ncand = rec.ncand;
-        double best = 1e300;
-        for (int q = 0; q < ncand; ++q) {
-          c.coltile[std::make_tuple(L, (int)sizeof(T), inv, cls)] = rec.cand[q];
-          auto pass = [&] {
-            if (inv)
-              launch_col<T, true>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune",
-                                  0);
-            else
-              launch_col<T, false>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune",
-                                   0);
-          };
-          pass();
-          pass();
-          MFX_HIP(hipEventRecord(e0, c.stream));
-          for (int r = 0; r < 5; ++r) pass();
-          MFX_HIP(hipEventRecord(e1, c.stream));
-          MFX_HIP(hipEventSynchronize(e1));
-          float ms = 0;
-          MFX_HIP(hipEventElapsedTime(&ms, e0, e1));
-          rec.ms[q] = ms / 5.0;
-          if (rec.ms[q] < best) {
-            best = rec.ms[q];
-            rec.chosen = rec.cand[q];
-          }
-        }
-        c.coltile[std::make_tuple(L, (int)sizeof(T), inv, cls)] = rec.chosen;
-        c.tune_log.push_back(rec);
+      for (int inv = 0; inv < 2; ++inv)
+        tune(inv, cls, true, [&] {
+          if (inv)
+            launch_col<T, true>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune", 0);
+          else
+            launch_col<T, false>(c, L, SrcPlain<T>{buf, sb, sp}, DstPlain<T>{buf, sb, sp, T(1)}, ncols, nb, "tune", 0);
+        });
+      // the projection (turn) passes of the padded lengths M = 3n/2, in place
+      // as in the LPT engine: two transforms per line and more registers than
+      // a plain pass, so they are tuned on their own
+      if (L % 3 == 0 && (L / 3) % 2 == 0) {
+        const int nn = 2 * L / 3;
+        const TurnArgs<T> ta{buf, sb, sp, nn, T(1), T(1), buf, sb, sp};
+        tune(COL_TURN, cls, true, [&] { launch_turn<T>(c, L, ta, ncols, nb, "tune", 0); });
+        // the truncating forward pass with an output operator (the last x
+        // pass of a convolution or LPT source), in place: each workgroup
+        // writes only the retained positions of the lines it has read; these
+        // passes always run plan 0
+        DstTruncOp<T> dt{buf, sb, sp, nn, OP_ASSIGN, T(0), T(1), 1, T(1)};
+        tune(COL_TRUNC_OP, cls, false,
+             [&] { launch_col<T, false>(c, L, SrcPlain<T>{buf, sb, sp}, dt, ncols, nb, "tune", 0); });
       }
     }
   } catch (...) {
@@ -213,6 +232,10 @@ template void autotune_columns<float>(Ctx&, int);
 template void autotune_columns<double>(Ctx&, int);
 
 int column_plan_alt(int L) { return col_plan_alt(L) ? 1 : 0; }
+
+int column_tile_wide(int L, int tsize, int alt) {
+  return tsize == 4 ? col_tile_wide<float>(L, alt) : col_tile_wide<double>(L, alt);
+}
 
 int column_tile_max(int L, int tsize) {
   return tsize == 4 ? col_tile_max<float>(L) : col_tile_max<double>(L);

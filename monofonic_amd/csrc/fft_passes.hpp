@@ -14,6 +14,11 @@
 namespace mfx {
 
 // ------------------------------------------------------------------ config
+// Tile widths of a column pass of length L (plan variant V): C, the widest
+// tile (at most 16 columns) whose exchange area fits 64 KiB of LDS, and CW =
+// 2C where C < 16 and a tile of 2C columns still fits one workgroup (1024
+// threads, 160 KiB of LDS with its twiddle table; one workgroup per CU), else
+// 0.  Wider tiles read longer runs of each line (2C complex values).
 template <typename T, int L, int V = 0>
 struct ColCfg {
   static constexpr int TPS = PlanInfo<L, V>::TPS;
@@ -21,6 +26,8 @@ struct ColCfg {
   static constexpr int lim_thr = 1024 / TPS;
   static constexpr int lim = (lim_lds < lim_thr ? lim_lds : lim_thr) < 16 ? (lim_lds < lim_thr ? lim_lds : lim_thr) : 16;
   static constexpr int C = lim >= 16 ? 16 : lim >= 8 ? 8 : lim >= 4 ? 4 : lim >= 2 ? 2 : 1;
+  static constexpr int CW =
+      (C < 16 && 2 * C * TPS <= 1024 && TwLds<T, L, 2 * C>::bytes <= (size_t)160 * 1024) ? 2 * C : 0;
 };
 
 
@@ -227,10 +234,9 @@ template <typename T, int L, bool INV, class Src, class Dst, int C, int V = 0>
 __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
-  __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
-  extern __shared__ __align__(16) unsigned char col_lds_raw[];
-  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];  // TwLds<T, L, C>::bytes
+  cx<T>* const lds = reinterpret_cast<cx<T>*>(col_lds_raw);
+  cx<T>* const twl = lds + L * C;
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
@@ -275,10 +281,9 @@ template <typename T, int L, bool INV, class Src, class Dst, int C, int V = 0>
 __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_pipe(Src src, Dst dst, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
-  __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
-  extern __shared__ __align__(16) unsigned char col_lds_raw[];
-  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];  // TwLds<T, L, C>::bytes
+  cx<T>* const lds = reinterpret_cast<cx<T>*>(col_lds_raw);
+  cx<T>* const twl = lds + L * C;
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   long long w, hi;
   int step;
@@ -337,10 +342,9 @@ template <typename T, int L, bool INV, int C, int V = 0>
 __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_multi(MultiPow<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
-  __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
-  extern __shared__ __align__(16) unsigned char col_lds_raw[];
-  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];  // TwLds<T, L, C>::bytes
+  cx<T>* const lds = reinterpret_cast<cx<T>*>(col_lds_raw);
+  cx<T>* const twl = lds + L * C;
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
@@ -396,10 +400,9 @@ template <typename T, int L, int C, int V = 0>
 __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
-  __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
-  extern __shared__ __align__(16) unsigned char col_lds_raw[];
-  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];  // TwLds<T, L, C>::bytes
+  cx<T>* const lds = reinterpret_cast<cx<T>*>(col_lds_raw);
+  cx<T>* const twl = lds + L * C;
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   int tile, b;
   xcd_tile(tile, b);
@@ -434,10 +437,9 @@ template <typename T, int L, int C, int V = 0>
 __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     k_col_turn_pipe(TurnArgs<T> a, const cx<T>* __restrict__ tw, int ncols, int ntiles, long long nitems) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
-  __shared__ cx<T> lds[L * C];
-  __shared__ cx<T> twl_s[TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1];
-  extern __shared__ __align__(16) unsigned char col_lds_raw[];
-  cx<T>* const twl = TwLds<T, L, C>::value ? twl_s : reinterpret_cast<cx<T>*>(col_lds_raw);
+  extern __shared__ __align__(16) unsigned char col_lds_raw[];  // TwLds<T, L, C>::bytes
+  cx<T>* const lds = reinterpret_cast<cx<T>*>(col_lds_raw);
+  cx<T>* const twl = lds + L * C;
   const int c = threadIdx.x % C, t = threadIdx.x / C;
   const int nh = a.n >> 1;
   long long w, hi;
@@ -501,13 +503,12 @@ unsigned pipe_grid(Ctx& c, K kernel, int block, long long nitems, size_t dyn = 0
   return (unsigned)((g + 7) / 8 * 8);
 }
 
-// A column kernel whose twiddle table goes to dynamic LDS (TwLds) may need
-// more than the default 64 KiB per workgroup in total; allow it once per
-// kernel.
+// A column kernel's LDS (TwLds) is dynamic; beyond the default 64 KiB per
+// workgroup the kernel must be allowed more, once per kernel.
 template <class K>
-void col_lds_prep(Ctx& c, K kernel, size_t stat, size_t dyn) {
-  if (dyn == 0 || stat + dyn <= 65536) return;
-  MFX_REQUIRE(stat + dyn <= (size_t)GEN_LDS_MAX, "column tile too large for one compute unit's LDS");
+void col_lds_prep(Ctx& c, K kernel, size_t dyn) {
+  if (dyn <= 65536) return;
+  MFX_REQUIRE(dyn <= (size_t)GEN_LDS_MAX, "column tile too large for one compute unit's LDS");
   int& done = c.lds_attr[(const void*)kernel];
   if (done < (int)dyn) {
     MFX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
@@ -515,26 +516,21 @@ void col_lds_prep(Ctx& c, K kernel, size_t stat, size_t dyn) {
   }
 }
 
-template <typename T, int L, int C>
-constexpr size_t col_static_lds() {
-  return (size_t)(L * C + (TwLds<T, L, C>::value ? TwLds<T, L, C>::entries : 1)) * sizeof(cx<T>);
-}
-
 template <typename T, int L, int C, bool INV, bool PIPE, int V, class Src, class Dst>
 void launch_col_LC(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
   const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
-  constexpr size_t dyn = TwLds<T, L, C>::dyn;
+  constexpr size_t dyn = TwLds<T, L, C>::bytes;
   if constexpr (PIPE) {
     const long long nitems = (long long)ntiles * nbatch;
     auto k = k_col_pipe<T, L, INV, Src, Dst, C, V>;
-    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    col_lds_prep(c, k, dyn);
     hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems, dyn)), block, dyn, c.stream, s, d,
                        c.twiddles<T>(L), ncols, ntiles, nitems);
   } else {
     auto k = k_col<T, L, INV, Src, Dst, C, V>;
-    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    col_lds_prep(c, k, dyn);
     hipLaunchKernelGGL(k, dim3(ntiles, nbatch), block, dyn, c.stream, s, d, c.twiddles<T>(L), ncols);
   }
   check_launch(name);
@@ -546,8 +542,17 @@ void launch_col_LC(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name
 // tuning table (col_tile) picks one.
 template <typename T, int L, bool INV, int V, class Src, class Dst>
 void launch_col_LV(Ctx& c, int v, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
-  constexpr int CM = ColCfg<T, L, V>::C;
+  constexpr int CM = ColCfg<T, L, V>::C, CW = ColCfg<T, L, V>::CW;
   const bool pipe = (v & COL_PIPELINED) != 0;
+  if constexpr (CW > 0) {
+    if ((v & COL_WIDTH) == CW) {
+      if (pipe)
+        launch_col_LC<T, L, CW, INV, true, V>(c, s, d, ncols, nbatch, name, bytes);
+      else
+        launch_col_LC<T, L, CW, INV, false, V>(c, s, d, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
   if constexpr (CM >= 2) {
     if ((v & COL_WIDTH) == CM / 2) {
       if (pipe)
@@ -567,7 +572,8 @@ void launch_col_LV(Ctx& c, int v, Src s, Dst d, int ncols, int nbatch, const cha
 // second radix plan's 24 elements per thread the final x pass (DstTruncOp)
 // compiles to 172 VGPRs (2 waves per SIMD) instead of 88 and measured slower
 // (2.33 vs 2.11 ms for the five 768-line x passes of 3LPT at 512^3), so such
-// passes keep plan 0 whatever the tuner chose.
+// passes keep plan 0; their tile width and kernel form are tuned on their own
+// (COL_TRUNC_OP).
 template <class D>
 struct HeavyDst {
   static constexpr bool value = false;
@@ -579,7 +585,9 @@ struct HeavyDst<DstTruncOp<T>> {
 
 template <typename T, int L, bool INV, class Src, class Dst>
 void launch_col_L(Ctx& c, Src s, Dst d, int ncols, int nbatch, const char* name, double bytes) {
-  const int v = c.col_tile(L, (int)sizeof(T), INV, ColCfg<T, L>::C, stride_class(s.sp, (int)sizeof(cx<T>)));
+  // register-heavy destinations have tuning decisions of their own
+  const int dir = HeavyDst<Dst>::value ? COL_TRUNC_OP : (INV ? 1 : 0);
+  const int v = c.col_tile(L, (int)sizeof(T), dir, ColCfg<T, L>::C, stride_class(s.sp, (int)sizeof(cx<T>)));
   if constexpr (PlanAlt<L>::value && !HeavyDst<Dst>::value) {
     if (v & COL_PLAN_ALT) {
       launch_col_LV<T, L, INV, 1>(c, v, s, d, ncols, nbatch, name, bytes);
@@ -600,6 +608,23 @@ int col_tile_max(int L) {
 #undef MFX_CASE
     default:
       throw Error(1, "unsupported transform length " + std::to_string(L));
+  }
+}
+
+// the wide tile width of length L, plan variant alt (0 if none)
+template <typename T>
+int col_tile_wide(int L, int alt) {
+  switch (L) {
+#define MFX_CASE(LL)                                                         \
+  case LL:                                                                   \
+    if constexpr (PlanAlt<LL>::value) {                                      \
+      if (alt) return ColCfg<T, LL, PlanAlt<LL>::value ? 1 : 0>::CW;         \
+    }                                                                        \
+    return ColCfg<T, LL>::CW;
+    MFX_COL_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      return 0;
   }
 }
 
@@ -681,16 +706,16 @@ void launch_turn_LC(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const c
   const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
-  constexpr size_t dyn = TwLds<T, L, C>::dyn;
+  constexpr size_t dyn = TwLds<T, L, C>::bytes;
   if constexpr (PIPE) {
     const long long nitems = (long long)ntiles * nbatch;
     auto k = k_col_turn_pipe<T, L, C, V>;
-    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    col_lds_prep(c, k, dyn);
     hipLaunchKernelGGL(k, dim3(pipe_grid(c, k, (int)block.x, nitems, dyn)), block, dyn, c.stream, a,
                        c.twiddles<T>(L), ncols, ntiles, nitems);
   } else {
     auto k = k_col_turn<T, L, C, V>;
-    col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+    col_lds_prep(c, k, dyn);
     hipLaunchKernelGGL(k, dim3(ntiles, nbatch), block, dyn, c.stream, a, c.twiddles<T>(L), ncols);
   }
   check_launch(name);
@@ -698,8 +723,17 @@ void launch_turn_LC(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const c
 
 template <typename T, int L, int V>
 void launch_turn_LV(Ctx& c, int v, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
-  constexpr int CM = ColCfg<T, L, V>::C;
+  constexpr int CM = ColCfg<T, L, V>::C, CW = ColCfg<T, L, V>::CW;
   const bool pipe = (v & COL_PIPELINED) != 0;
+  if constexpr (CW > 0) {
+    if ((v & COL_WIDTH) == CW) {
+      if (pipe)
+        launch_turn_LC<T, L, CW, true, V>(c, a, ncols, nbatch, name, bytes);
+      else
+        launch_turn_LC<T, L, CW, false, V>(c, a, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
   if constexpr (CM >= 2) {
     if ((v & COL_WIDTH) == CM / 2) {
       if (pipe)
@@ -715,10 +749,10 @@ void launch_turn_LV(Ctx& c, int v, const TurnArgs<T>& a, int ncols, int nbatch, 
     launch_turn_LC<T, L, CM, false, V>(c, a, ncols, nbatch, name, bytes);
 }
 
-// at the variant tuned for the inverse transforms of (L, T)
+// at the variant tuned for the turn passes of (L, T) (key COL_TURN)
 template <typename T, int L>
 void launch_turn_L(Ctx& c, const TurnArgs<T>& a, int ncols, int nbatch, const char* name, double bytes) {
-  const int v = c.col_tile(L, (int)sizeof(T), true, ColCfg<T, L>::C, stride_class(a.sp, (int)sizeof(cx<T>)));
+  const int v = c.col_tile(L, (int)sizeof(T), COL_TURN, ColCfg<T, L>::C, stride_class(a.sp, (int)sizeof(cx<T>)));
   if constexpr (PlanAlt<L>::value) {
     if (v & COL_PLAN_ALT) {
       launch_turn_LV<T, L, 1>(c, v, a, ncols, nbatch, name, bytes);
@@ -756,16 +790,22 @@ void launch_multi_LC(Ctx& c, const MultiPow<T>& a, int ncols, int nbatch, const 
   const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
-  constexpr size_t dyn = TwLds<T, L, C>::dyn;
+  constexpr size_t dyn = TwLds<T, L, C>::bytes;
   auto k = k_col_multi<T, L, INV, C, V>;
-  col_lds_prep(c, k, col_static_lds<T, L, C>(), dyn);
+  col_lds_prep(c, k, dyn);
   hipLaunchKernelGGL(k, dim3(ntiles, nbatch), block, dyn, c.stream, a, c.twiddles<T>(L), ncols);
   check_launch(name);
 }
 
 template <typename T, int L, bool INV, int V>
 void launch_multi_LV(Ctx& c, int v, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
-  constexpr int CM = ColCfg<T, L, V>::C;
+  constexpr int CM = ColCfg<T, L, V>::C, CW = ColCfg<T, L, V>::CW;
+  if constexpr (CW > 0) {
+    if ((v & COL_WIDTH) == CW) {
+      launch_multi_LC<T, L, CW, INV, V>(c, a, ncols, nbatch, name, bytes);
+      return;
+    }
+  }
   if constexpr (CM >= 2) {
     if ((v & COL_WIDTH) == CM / 2) {
       launch_multi_LC<T, L, CM / 2, INV, V>(c, a, ncols, nbatch, name, bytes);

@@ -38,6 +38,8 @@ struct Error : std::runtime_error {
 constexpr int COL_PIPELINED = 0x100;
 constexpr int COL_PLAN_ALT = 0x200;  // the length's second radix plan
 constexpr int COL_WIDTH = 0xff;       // tile width bits of a column variant
+constexpr int COL_TURN = 2;           // tuning direction key of the turn (projection) passes
+constexpr int COL_TRUNC_OP = 3;       // tuning direction key of the truncating x passes with an output operator
 // Stride class of a column pass: 0 when consecutive line elements are close
 // (the y passes, a row of kz columns apart), 1 when they are 64 KiB or more
 // apart (the x passes, a whole plane apart).  The two access patterns load
@@ -47,8 +49,8 @@ inline int stride_class(long long line_stride, int csize) { return line_stride *
 struct TuneRec {
   int L, tsize, inv, cls;
   int ncand;
-  int cand[8];
-  double ms[8];
+  int cand[12];
+  double ms[12];
   int chosen;
 };
 
@@ -76,8 +78,9 @@ struct Ctx {
   // column-pass variant per (length, element size, direction, stride class)
   std::map<std::tuple<int, int, int, int>, int> coltile;
   std::vector<TuneRec> tune_log;
-  int col_tile(int L, int tsize, bool inv, int cmax, int cls = 0) const {
-    auto it = coltile.find(std::make_tuple(L, tsize, (int)inv, cls));
+  // dir: 0 forward, 1 inverse, COL_TURN the projection (turn) passes
+  int col_tile(int L, int tsize, int dir, int cmax, int cls = 0) const {
+    auto it = coltile.find(std::make_tuple(L, tsize, dir, cls));
     return it == coltile.end() ? cmax : it->second;
   }
   int cus = 0;  // compute units of the device (queried once)
@@ -153,5 +156,6 @@ int column_length_supported(int L);
 // widest tile of length L for element size tsize (4: FP32, 8: FP64)
 int column_tile_max(int L, int tsize);
 int column_plan_alt(int L);  // 1 if length L has a second radix plan
+int column_tile_wide(int L, int tsize, int alt);  // wide tile width of (L, plan), 0 if none
 
 }  // namespace mfx

@@ -26,10 +26,12 @@ def set_all_tiles(ctx, L, prec, flags):
             raise AssertionError(f"no tile accepted for L={L} {prec} flags {flags:#x}")
 
 
-@pytest.mark.parametrize("n,prec,wide,half", [(96, "float", 16, 8), (192, "double", 16, 8), (256, "float", 16, 8),
-                                              (768, "float", 8, 4)])
-def test_tile_width_does_not_change_results(n, prec, wide, half):
-    """tile widths and the persistent pipelined kernel give identical transforms"""
+@pytest.mark.parametrize("n,prec,wide,half,double", [(96, "float", 16, 8, 0), (192, "double", 16, 8, 0),
+                                                     (256, "float", 16, 8, 0), (768, "float", 8, 4, 16),
+                                                     (768, "double", 4, 2, 8)])
+def test_tile_width_does_not_change_results(n, prec, wide, half, double):
+    """tile widths (including the double-width tile in dynamic LDS beyond
+    64 KiB) and the persistent pipelined kernel give identical transforms"""
     import monofonic_amd as M
     ctx = M.Context(0)
     a = random_real_grid(n, 77) if n <= 256 else None
@@ -38,7 +40,8 @@ def test_tile_width_does_not_change_results(n, prec, wide, half):
         a = np.zeros((n, 16, n + 2))
         a[:, :, :n] = rng.standard_normal((n, 16, n))
     out = []
-    for tile in (wide, half, wide | PIPE, half | PIPE):
+    tiles = [wide, half, wide | PIPE, half | PIPE] + ([double, double | PIPE] if double else [])
+    for tile in tiles:
         for inv in (0, 1):
             ctx.set_column_tile(n, prec, inv, tile)
         g = M.Grid_FFT(ctx, a.shape[:2] + (n,), 100.0, prec)
@@ -48,7 +51,7 @@ def test_tile_width_does_not_change_results(n, prec, wide, half):
         g.FourierTransformBackward()
         out.append((k, g.download().copy()))
         del g
-    for q in range(1, 4):
+    for q in range(1, len(tiles)):
         assert np.array_equal(out[0][0], out[q][0]), q
         assert np.array_equal(out[0][1], out[q][1]), q
     ctx.close()
@@ -144,9 +147,12 @@ def test_autotune_records_decisions():
     rep = ctx.autotune_report()
     print(rep, flush=True)
     lines = [ln for ln in rep.splitlines() if ln]
-    # lengths 128 and 192, forward and inverse, row-strided and plane-strided
-    assert len(lines) == 8
-    assert sum("plane-strided" in ln for ln in lines) == 4
+    # lengths 128 and 192, forward and inverse, row-strided and plane-strided,
+    # and the turn and truncating forward passes of 192 (a padded length 3n/2)
+    assert len(lines) == 12
+    assert sum("plane-strided" in ln for ln in lines) == 6
+    assert sum(ln.startswith("L=192 f32 turn") for ln in lines) == 2
+    assert sum(ln.startswith("L=192 f32 truncating forward") for ln in lines) == 2
     for ln in lines:
         assert "-> C=" in ln
         assert ln.count("ms") == 4  # two tile widths, plain and pipelined
@@ -154,9 +160,12 @@ def test_autotune_records_decisions():
         ctx.set_column_tile(128, "float", 0, 3)
     with pytest.raises(M.MfxError):  # 128 has one radix plan
         ctx.set_column_tile(128, "float", 0, 16 | ALT)
-    ctx.autotune(512, "float")  # lengths 512 and 768: eight candidates at 768
+    ctx.autotune(512, "float")  # lengths 512 and 768: twelve candidates at 768
     lines = [ln for ln in ctx.autotune_report().splitlines() if ln.startswith("L=768")]
-    assert len(lines) == 4
-    for ln in lines:
-        assert ln.count("ms") == 8
+    assert len(lines) == 8
+    for ln in lines:  # widths 8, 4, 16; plain, pipelined; two radix plans (one for the truncating passes)
+        assert ln.count("ms") == (6 if "truncating" in ln else 12)
+    with pytest.raises(M.MfxError):  # no double-width tile at 512 (16 columns already)
+        ctx.set_column_tile(512, "float", 0, 32)
+    ctx.set_column_tile(768, "float", 0, 16 | PIPE | ALT)
     ctx.close()
