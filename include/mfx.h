@@ -281,7 +281,7 @@ int mfx_cosmo_query(const mfx_cosmo *cc, int what, double x, double *out);
 int mfx_cosmo_amplitude_table(const mfx_cosmo *cc, size_t n, double boxlen, double *out, size_t ntable);
 
 /* ------------------------------------------------------------ LPT driver
- * ic_generator::run for the sc lattice and generic (field_lagrangian) or
+ * ic_generator::run for the sc, bcc, fcc and rsc lattices and generic (field_lagrangian) or
  * particle output (src/ic_generator.cc:90-971): noise, phi(1..3), A(3),
  * growth scaling and FP64 position/velocity assembly, all on the device. */
 typedef struct {
@@ -309,6 +309,10 @@ typedef struct {
                            of one realisation (mfx_icgen_download_async) overlaps the next
                            realisation's noise, LPT and assembly stages; 0: they share the
                            LPT workspace when it is large enough (least memory) */
+  int lattice;          /* [setup] ParticleLoad (ic_generator.cc:131-139): 0 sc, 1 bcc, 2 fcc,
+                           3 rsc, i.e. 1, 2, 4 or 8 particles per cell
+                           (particle_generator.hh:47-54,499-543,597-638); not with PLT,
+                           field output or assemble_each */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;
@@ -329,13 +333,16 @@ int mfx_icgen_capture(mfx_icgen *ic);
 int mfx_icgen_launch(mfx_icgen *ic);
 /* device pointers of the FP64 particle arrays: what 0..2 positions x,y,z
  * (displacements with field_output), 3..5 velocities x,y,z, 6 the density
- * (field_output only) (n^3 doubles each, order (i,j,k)).  With the fused
+ * (field_output only) (n^3 doubles each, order (i,j,k); for a bcc, fcc or
+ * rsc load arrays 0..5 hold 2, 4 or 8 such blocks, one per sublattice, as
+ * particle_generator.hh stores them).  With the fused
  * LPT engine they may share device memory with its workspace: valid until the
  * next mfx_icgen_run */
 int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
 /* Asynchronous download of the output arrays of the last realisation: bit w
  * of `what` copies array w (0..5 as mfx_icgen_particles, 6 the density of
- * field output, n^3 doubles each) into host buffer dst[w] on the
+ * field output; n^3 doubles each, times the particles per cell for 0..5)
+ * into host buffer dst[w] on the
  * generator's copy stream, after the realisation's last kernel.  Host
  * buffers from mfx_host_alloc (pinned) make the copy asynchronous.  The next
  * mfx_icgen_run waits for these copies before its first kernel that

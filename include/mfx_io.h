@@ -43,7 +43,7 @@ typedef struct {
   int lpt_order, grid_res;
   int double_prec, long_ids, gadget2_compat;
   int do_fixing, do_inversion, do_baryons, do_baryons_vrel;
-  const char *particle_load; /* "sc" */
+  const char *particle_load; /* "sc", "bcc", "fcc" or "rsc" */
   const char *transfer;      /* "eisenstein" */
   const char *param_set;     /* "none" */
   const char *rng;           /* "MUSIC1", "NGENIC", ... */
@@ -55,11 +55,14 @@ typedef struct {
 
 int mfx_out_gadget_create(const char *fname, const mfx_gadget_params *p, mfx_out **out);
 
-/* write_particle_data (output_gadget_hdf5.cc:268-308) for n^3 particles of
- * the sc lattice in (i,j,k) order: species 1 (dm) / 0 (baryon) / 3
- * (neutrino); pos[d] and vel[d] are the three FP64 component arrays;
+/* write_particle_data (output_gadget_hdf5.cc:268-308) for the particles of
+ * an n^3 grid and the particle load of mfx_out_gadget_create (particle_load
+ * sc, bcc, fcc or rsc: N = 1, 2, 4 or 8 n^3 particles, sublattice-major, each
+ * sublattice in (i,j,k) order): species 1 (dm) / 0 (baryon) / 3 (neutrino);
+ * pos[d] and vel[d] are the three FP64 component arrays of N values;
  * Coordinates = pos * lscale and Velocities = vel * vscale (interleaved to
- * N x 3); ParticleIDs = (i n + j) n + k; the mass table entry
+ * N x 3); ParticleIDs = overload * ((i n + j) n + k) + sublattice
+ * (particle_generator.hh:229-250); the mass table entry
  * omega_species * rhoc L^3 / N (rhoc = 27.7519737 h^2 1e10 Msun/Mpc^3) */
 int mfx_out_gadget_write_particles(mfx_out *o, int species, const double *const pos[3], const double *const vel[3],
                                    size_t n, double omega_species, double lscale, double vscale);

@@ -511,17 +511,21 @@ class PLT:
 
 # ------------------------------------------------------------------ LPT driver (src/ic_generator.cc:90-971)
 GENERATORS = {"MUSIC1": 0, "NGENIC": 1, "PANPHASIA_THREEFRY": 2}
+LATTICES = {"sc": 0, "bcc": 1, "fcc": 2, "rsc": 3}  # [setup] ParticleLoad (ic_generator.cc:131-139)
 
 
 class ICGenerator:
-    """Device-resident ic_generator::run for the sc lattice (FP64 particles)."""
+    """Device-resident ic_generator::run (FP64 particles) for the sc, bcc, fcc
+    and rsc particle loads."""
 
     def __init__(self, ctx, GridRes, BoxLength, zstart, LPTorder=3, precision="double", generator="MUSIC1",
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
-                 autotune=True, assemble_each=False, field_output=False, own_particles=False):
+                 autotune=True, assemble_each=False, field_output=False, own_particles=False, ParticleLoad="sc"):
         self.ctx = ctx
         self.n = int(GridRes)
+        self.lattice = str(ParticleLoad)
+        self.overload = 1 << LATTICES[self.lattice]  # particles per cell
         self.boxlen, self.zstart, self.lpt_order = float(BoxLength), float(zstart), int(LPTorder)
         self.cosmo = dict(H0=H0, Omega_m=Omega_m, Omega_b=Omega_b)
         self.generator, self.plt = generator, bool(PLT)
@@ -537,6 +541,7 @@ class ICGenerator:
         p.autotune = int(autotune)
         p.assemble_each = int(assemble_each)
         p.field_output, p.own_particles = int(field_output), int(own_particles)
+        p.lattice = LATTICES[self.lattice]
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 
@@ -552,14 +557,19 @@ class ICGenerator:
         except Exception:
             pass
 
+    def _shape(self, w):
+        n = self.n
+        return (n, n, n) if (w == 6 or self.overload == 1) else (self.overload, n, n, n)
+
     def _pinned(self, w):
-        """pinned host buffer of output array w (n^3 doubles), as a numpy view"""
+        """pinned host buffer of output array w (n^3 doubles per sublattice),
+        as a numpy view"""
         if self._host[w] is None:
-            nb = self.n ** 3 * 8
+            cnt = int(np.prod(self._shape(w)))
             ptr = ctypes.c_void_p()
-            call("mfx_host_alloc", nb, ctypes.byref(ptr))
-            arr = np.ctypeslib.as_array((ctypes.c_double * (self.n ** 3)).from_address(ptr.value))
-            self._host[w] = (ptr.value, arr.reshape(self.n, self.n, self.n))
+            call("mfx_host_alloc", cnt * 8, ctypes.byref(ptr))
+            arr = np.ctypeslib.as_array((ctypes.c_double * cnt).from_address(ptr.value))
+            self._host[w] = (ptr.value, arr.reshape(self._shape(w)))
         return self._host[w][1]
 
     def download_async(self, what=range(6)):
@@ -609,7 +619,7 @@ class ICGenerator:
         h = self.cosmo["H0"] / 100.0
         with GadgetOutput(fname, self.zstart, L, self.cosmo["Omega_m"], self._omega_de(), self.cosmo["Omega_b"], h,
                           self.lpt_order, self.n, double_prec=double_prec, long_ids=long_ids, rng=self.generator,
-                          plt="yes" if self.plt else "no") as out:
+                          plt="yes" if self.plt else "no", particle_load=self.lattice) as out:
             out.write_particles(arrs[0:3], arrs[3:6], self.n, self.cosmo["Omega_m"], L, L / np.sqrt(a))
 
     def _omega_de(self):
@@ -647,9 +657,10 @@ class ICGenerator:
         return self.ctx.d2h(self.particles_dev(6), n ** 3 * 8, np.float64).reshape(n, n, n)
 
     def particles(self, what):
-        """host copy of positions (0..2) or velocities (3..5), shape (n,n,n)"""
-        n = self.n
-        return self.ctx.d2h(self.particles_dev(what), n ** 3 * 8, np.float64).reshape(n, n, n)
+        """host copy of positions (0..2) or velocities (3..5): shape (n,n,n)
+        for the sc load, (sublattice, n, n, n) for bcc, fcc and rsc"""
+        shp = self._shape(what)
+        return self.ctx.d2h(self.particles_dev(what), int(np.prod(shp)) * 8, np.float64).reshape(shp)
 
     def grid_handle(self, which):
         p = ctypes.c_void_p()

@@ -1334,6 +1334,14 @@ int mfx_plt_gradient(const mfx_plt* p, size_t n, double boxlen, const long long*
 }  // extern "C"
 
 // ------------------------------------------------------------------ LPT driver
+// Sublattice offsets of the Bravais particle loads in cells, the first always
+// zero (particle_generator.hh:47-54): sc, bcc, fcc, rsc
+static const double kLatticeShift[4][8][3] = {
+    {{0, 0, 0}},
+    {{0, 0, 0}, {0.5, 0.5, 0.5}},
+    {{0, 0, 0}, {0, 0.5, 0.5}, {0.5, 0, 0.5}, {0.5, 0.5, 0}},
+    {{0, 0, 0}, {0, 0, 0.5}, {0, 0.5, 0}, {0, 0.5, 0.5}, {0.5, 0, 0}, {0.5, 0, 0.5}, {0.5, 0.5, 0}, {0.5, 0.5, 0.5}}};
+
 struct mfx_icgen {
   mfx_ctx* ctx;
   mfx_icgen_params p;
@@ -1371,6 +1379,7 @@ struct mfx_icgen {
   hipEvent_t run_done = nullptr, copy_done = nullptr;
   bool copy_pending = false;
   int in_batches = 0;  // batches whose graph records this generator
+  int overload = 1;    // particles per cell: 1 sc, 2 bcc, 4 fcc, 8 rsc
 };
 
 static void icgen_free(mfx_icgen* ic) {
@@ -1416,9 +1425,17 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
     MFX_REQUIRE(!(p->n % 4 != 0 && p->lpt_order > 1),
                 "LPTorder>1 needs a grid size divisible by 4 (padded z rows of even length 3n/2)");
     MFX_REQUIRE(p->generator >= 0 && p->generator <= 2, "generator must be 0 (MUSIC1), 1 (NGENIC) or 2 (Threefry)");
+    // ParticleLoad sc/bcc/fcc/rsc (ic_generator.cc:131-139)
+    MFX_REQUIRE(p->lattice >= 0 && p->lattice <= 3, "lattice must be 0 (sc), 1 (bcc), 2 (fcc) or 3 (rsc)");
+    MFX_REQUIRE(p->lattice == 0 || !p->use_plt, "PLT corrections are built for the sc lattice only");
+    MFX_REQUIRE(p->lattice == 0 || !p->field_output,
+                "field output writes grid fields; the particle load (lattice) applies to particle output only");
+    MFX_REQUIRE(p->lattice == 0 || !p->assemble_each, "bcc/fcc/rsc loads need the six-component assembly "
+                                                      "(assemble_each = 0)");
     auto* ic = new mfx_icgen();
     ic->ctx = c;
     ic->p = *p;
+    ic->overload = 1 << p->lattice;
     try {
       ic->cosmo.reset(new Cosmology(p->H0, p->Omega_m, p->Omega_b, p->n_s, p->sigma_8, p->Tcmb, p->zero_radiation != 0));
       const double astart = 1.0 / (1.0 + p->zstart);
@@ -1439,7 +1456,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       // complete, so with the fused engine they share its workspace (at 1024^3
       // FP32 this keeps the realisation within one GPU's HBM); they stay valid
       // until the next run.
-      const size_t npart = p->n * p->n * p->n;
+      const size_t ncell = p->n * p->n * p->n, npart = ncell * (size_t)ic->overload;
       if (ic->conv && p->use_fused_lpt && !p->own_particles && lpt_fused_supported((int)p->n)) {
         lpt_reserve(ic->conv, p->lpt_order);
         if (ic->conv->lpt_bytes >= 6 * npart * sizeof(double)) {
@@ -1460,9 +1477,9 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       } else {
         size_t fr = 0, tot = 0;
         MFX_HIP(hipMemGetInfo(&fr, &tot));
-        if (comp_bytes + ((size_t)8 << 30) <= fr) ic->comp6 = c->c.alloc(comp_bytes);
+        if (comp_bytes + ((size_t)8 << 30) <= fr || p->lattice > 0) ic->comp6 = c->c.alloc(comp_bytes);
       }
-      if (p->field_output) ic->delta = (double*)c->c.alloc(sizeof(double) * npart);
+      if (p->field_output) ic->delta = (double*)c->c.alloc(sizeof(double) * ncell);
       MFX_HIP(hipStreamCreate(&ic->copy_stream));
       MFX_HIP(hipEventCreateWithFlags(&ic->run_done, hipEventDisableTiming));
       MFX_HIP(hipEventCreateWithFlags(&ic->copy_done, hipEventDisableTiming));
@@ -1632,6 +1649,23 @@ static void icgen_device_run(mfx_icgen* ic) {
     }
     tmp->space = MFX_RSPACE;
   };
+  // bcc/fcc/rsc (particle_generator.hh:499-543,597-638): sublattice l of
+  // component w holds its field at the cell positions shifted by
+  // kLatticeShift[l] (shift_field by the difference to the previous
+  // sublattice, on the k-space component kept in comp6), positions
+  // (i + s)/n + Psi; array w holds the sublattices one after another
+  const long long ncell = (long long)p.n * p.n * p.n;
+  auto sublattices = [&](int w, auto* comp) {
+    using TT = decltype(comp->x);
+    const double(*s)[3] = kLatticeShift[p.lattice];
+    for (int l = 0; l < ic->overload; ++l) {
+      if (l > 0)
+        lattice_shift<TT>(ic->ctx->c, comp, (int)p.n, s[l][0] - s[l - 1][0], s[l][1] - s[l - 1][1],
+                          s[l][2] - s[l - 1][2]);
+      fft_c2r_particles<TT>(ic->ctx->c, (TT*)tmp->d, (int)p.n, (TT)norm_fac(tmp), ic->part[w] + l * ncell, w % 3, 1.0,
+                            okind(w), (const TT*)comp, s[l][w % 3]);
+    }
+  };
   if (ic->comp6) {
     const long long ws = (long long)p.n * p.n * (p.n / 2 + 1);
     if (tmp->prec == MFX_FLOAT) {
@@ -1640,16 +1674,22 @@ static void icgen_device_run(mfx_icgen* ic) {
                        (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, ic->vfact, 1.0, plt_tab, plt_ng,
                        gr);
       for (int w = 0; w < 6; ++w)
-        fft_c2r_particles<float>(ic->ctx->c, (float*)((cx<float>*)ic->comp6 + w * ws), (int)p.n,
-                                 (float)norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
+        if (ic->overload == 1)
+          fft_c2r_particles<float>(ic->ctx->c, (float*)((cx<float>*)ic->comp6 + w * ws), (int)p.n,
+                                   (float)norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
+        else
+          sublattices(w, (cx<float>*)ic->comp6 + w * ws);
     } else {
       assemble6<double>(ic->ctx->c, (cx<double>*)ic->comp6, ws, (const cx<double>*)phi->d,
                         (const cx<double>*)P(phi2), (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]),
                         (const cx<double>*)P(A3[1]), (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order,
                         ic->vfact, 1.0, plt_tab, plt_ng, gr);
       for (int w = 0; w < 6; ++w)
-        fft_c2r_particles<double>(ic->ctx->c, (double*)((cx<double>*)ic->comp6 + w * ws), (int)p.n, norm_fac(tmp),
-                                  ic->part[w], w % 3, 1.0, okind(w));
+        if (ic->overload == 1)
+          fft_c2r_particles<double>(ic->ctx->c, (double*)((cx<double>*)ic->comp6 + w * ws), (int)p.n,
+                                    norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
+        else
+          sublattices(w, (cx<double>*)ic->comp6 + w * ws);
     }
     density();
     return;
@@ -1760,8 +1800,8 @@ extern "C" int mfx_icgen_download_async(mfx_icgen* ic, unsigned what, void* cons
     for (int w = 0; w < 7; ++w)
       if (what & (1u << w)) {
         MFX_REQUIRE(dst[w], "null destination buffer");
-        MFX_HIP(hipMemcpyAsync(dst[w], w < 6 ? ic->part[w] : ic->delta, bytes, hipMemcpyDeviceToHost,
-                               ic->copy_stream));
+        MFX_HIP(hipMemcpyAsync(dst[w], w < 6 ? ic->part[w] : ic->delta, w < 6 ? bytes * ic->overload : bytes,
+                               hipMemcpyDeviceToHost, ic->copy_stream));
       }
     MFX_HIP(hipEventRecord(ic->copy_done, ic->copy_stream));
     ic->copy_pending = true;

@@ -282,20 +282,21 @@ __global__ void __launch_bounds__(GEN_THREADS)
 template <typename T>
 __global__ void __launch_bounds__(GEN_THREADS)
     k_row_c2r_part_gen(const cx<T>* src, long long spitch, int hin, double* out, T scale, int idim, double lunit,
-                       int kind, GenPlan plan, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+                       int kind, double qoff, GenPlan plan, const cx<T>* __restrict__ twK,
+                       const cx<T>* __restrict__ twM) {
   MFX_GEN_ROW_PROLOGUE(T)
   const int n = 2 * K;
   gen_load_half(gr.A, src + row * spitch, hin, K);
   const cx<T>* z = gr.c2r();
   const int i = (int)(row / n), j = (int)(row % n);
-  const double qrow = (double)(idim == 0 ? i : j) / (double)n * lunit;
+  const double qrow = (double)(idim == 0 ? i : j) / (double)n * lunit + qoff;
   double* o = out + row * n;
   for (int m = tid; m < K; m += GEN_THREADS) {
     double f0 = (double)(z[m].x * scale), f1 = (double)(z[m].y * scale);
     if (kind == 0) {
       if (idim == 2) {
-        f0 = (double)(2 * m) / (double)n * lunit + f0;
-        f1 = (double)(2 * m + 1) / (double)n * lunit + f1;
+        f0 = (double)(2 * m) / (double)n * lunit + qoff + f0;
+        f1 = (double)(2 * m + 1) / (double)n * lunit + qoff + f1;
       } else {
         f0 = qrow + f0;
         f1 = qrow + f1;

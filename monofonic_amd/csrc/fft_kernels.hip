@@ -53,23 +53,27 @@ void fft_c2r_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale) {
 
 // inverse transform of an n^3 k-space grid into one FP64 particle array: the
 // x and y passes in place, the z pass writing the particles (positions add
-// the lattice coordinate); the grid is left partially transformed
+// the lattice coordinate (i + qshift)/n * lunit along idim); the grid is left
+// partially transformed.  With src the x pass reads src (which is left
+// intact) and writes data.
 template <typename T>
-void fft_c2r_particles(Ctx& c, T* data, int n, T scale, double* out, int idim, double lunit, int kind) {
+void fft_c2r_particles(Ctx& c, T* data, int n, T scale, double* out, int idim, double lunit, int kind,
+                       const T* src, double qshift) {
   MFX_REQUIRE(n % 2 == 0, "c2r requires an even last dimension");
   const long long H = n / 2 + 1;
   cx<T>* cd = reinterpret_cast<cx<T>*>(data);
+  const cx<T>* cs = src ? reinterpret_cast<const cx<T>*>(src) : cd;
   const double S = (double)n * n * (n + 2) * sizeof(T);
-  launch_col<T, true>(c, n, SrcPlain<T>{cd, H, (long long)n * H}, DstPlain<T>{cd, H, (long long)n * H, T(1)},
+  launch_col<T, true>(c, n, SrcPlain<T>{cs, H, (long long)n * H}, DstPlain<T>{cd, H, (long long)n * H, T(1)},
                       (int)H, n, "fft_x_inv", 2 * S);
   launch_col<T, true>(c, n, SrcPlain<T>{cd, (long long)n * H, H}, DstPlain<T>{cd, (long long)n * H, H, T(1)},
                       (int)H, n, "fft_y_inv", 2 * S);
-  launch_row_c2r_part<T>(c, n / 2, cd, H, (int)H, out, (long long)n * n, scale, idim, lunit, kind, "fft_z_c2r_part",
-                         S + (double)n * n * n * sizeof(double));
+  launch_row_c2r_part<T>(c, n / 2, cd, H, (int)H, out, (long long)n * n, scale, idim, lunit, kind,
+                         qshift / (double)n * lunit, "fft_z_c2r_part", S + (double)n * n * n * sizeof(double));
 }
 
-template void fft_c2r_particles<float>(Ctx&, float*, int, float, double*, int, double, int);
-template void fft_c2r_particles<double>(Ctx&, double*, int, double, double*, int, double, int);
+template void fft_c2r_particles<float>(Ctx&, float*, int, float, double*, int, double, int, const float*, double);
+template void fft_c2r_particles<double>(Ctx&, double*, int, double, double*, int, double, int, const double*, double);
 template void fft_r2c_3d<float>(Ctx&, float*, int, int, int, float);
 template void fft_r2c_3d<double>(Ctx&, double*, int, int, int, double);
 template void fft_c2r_3d<float>(Ctx&, float*, int, int, int, float);

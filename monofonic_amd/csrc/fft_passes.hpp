@@ -953,13 +953,13 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
 template <typename T, int K>
 __global__ void __launch_bounds__(RowLanes<K>::THREADS)
     k_row_c2r_part(const cx<T>* src, long long spitch, int hin, double* out, long long nrows, T scale, int idim,
-                   double lunit, int kind, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
+                   double lunit, int kind, double qoff, const cx<T>* __restrict__ twK, const cx<T>* __restrict__ twM) {
   MFX_ROW_PROLOGUE(T, K)
   constexpr int E = RowCtx<T, K>::E, n = 2 * K;
   cx<T> v[E];
   rc.c2r(v, src + rowc * spitch, hin);
   const int i = (int)(rowc / n), j = (int)(rowc % n);
-  const double qrow = (double)(idim == 0 ? i : j) / (double)n * lunit;
+  const double qrow = (double)(idim == 0 ? i : j) / (double)n * lunit + qoff;
   const __amdgpu_buffer_rsrc_t r =
       row_rsrc_u<RowLanes<K>::RPW == 1>(out + rowc * n, act ? (unsigned)(n * sizeof(double)) : 0u);
 #pragma unroll
@@ -969,8 +969,8 @@ __global__ void __launch_bounds__(RowLanes<K>::THREADS)
       double f0 = (double)(v[e].x * scale), f1 = (double)(v[e].y * scale);
       if (kind == 0) {
         if (idim == 2) {
-          f0 = (double)(2 * m) / (double)n * lunit + f0;
-          f1 = (double)(2 * m + 1) / (double)n * lunit + f1;
+          f0 = (double)(2 * m) / (double)n * lunit + qoff + f0;
+          f1 = (double)(2 * m + 1) / (double)n * lunit + qoff + f1;
         } else {
           f0 = qrow + f0;
           f1 = qrow + f1;
@@ -1089,23 +1089,23 @@ void launch_row_c2r(Ctx& c, int K, const cx<T>* src, long long spitch, int hin, 
 
 template <typename T, int K>
 void launch_row_c2r_part_K(Ctx& c, const cx<T>* src, long long spitch, int hin, double* out, long long nrows, T scale,
-                           int idim, double lunit, int kind, const char* name, double bytes) {
+                           int idim, double lunit, int kind, double qoff, const char* name, double bytes) {
   const dim3 grid = row_grid<T, K>(nrows);
   LaunchScope ls(c, name, bytes);
   hipLaunchKernelGGL((k_row_c2r_part<T, K>), grid, dim3(RowBlk<T, K>::THREADS), 0, c.stream, src, spitch, hin, out, nrows, scale, idim,
-                     lunit, kind, c.twiddles<T>(K), c.twiddles<T>(2 * K));
+                     lunit, kind, qoff, c.twiddles<T>(K), c.twiddles<T>(2 * K));
   check_launch(name);
 }
 
 template <typename T>
 void launch_row_c2r_part(Ctx& c, int K, const cx<T>* src, long long spitch, int hin, double* out, long long nrows,
-                         T scale, int idim, double lunit, int kind, const char* name, double bytes) {
-#define MFX_CASE(KK)                                                                                      \
-  case KK:                                                                                                \
-    launch_row_c2r_part_K<T, KK>(c, src, spitch, hin, out, nrows, scale, idim, lunit, kind, name, bytes); \
+                         T scale, int idim, double lunit, int kind, double qoff, const char* name, double bytes) {
+#define MFX_CASE(KK)                                                                                            \
+  case KK:                                                                                                      \
+    launch_row_c2r_part_K<T, KK>(c, src, spitch, hin, out, nrows, scale, idim, lunit, kind, qoff, name, bytes); \
     return;
   MFX_ROW_SWITCH(K, MFX_CASE,
-                 MFX_GEN_ROW_LAUNCH(k_row_c2r_part_gen, 2, src, spitch, hin, out, scale, idim, lunit, kind))
+                 MFX_GEN_ROW_LAUNCH(k_row_c2r_part_gen, 2, src, spitch, hin, out, scale, idim, lunit, kind, qoff))
 #undef MFX_CASE
 }
 

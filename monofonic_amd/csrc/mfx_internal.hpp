@@ -137,7 +137,8 @@ void fft_r2c_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale);
 template <typename T>
 void fft_c2r_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale);
 template <typename T>
-void fft_c2r_particles(Ctx& c, T* data, int n, T scale, double* out, int idim, double lunit, int kind);
+void fft_c2r_particles(Ctx& c, T* data, int n, T scale, double* out, int idim, double lunit, int kind,
+                       const T* src = nullptr, double qshift = 0.0);
 
 // Orszag 3/2 product of two n-grid k-space fields with multipliers, written
 // through op into out (k space n-grid).  ws must hold conv_workspace_bytes.

@@ -50,6 +50,10 @@ void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double 
 // out = dplus |k|^2 phi (the field output's 1st-order density in k space)
 template <typename T>
 void density_k(Ctx& c, cx<T>* out, const cx<T>* phi, int n, double boxlen, double dplus);
+// shift_field of a k-space n-grid by s cells (Hermitian part of the kz = 0, n/2
+// planes, then the phase), in place
+template <typename T>
+void lattice_shift(Ctx& c, cx<T>* a, int n, double sx, double sy, double sz);
 template <typename T>
 void add_const(Ctx& c, T* f, int n0, int n1, int n2, long long pitch, double v);
 template <typename T>

@@ -426,6 +426,60 @@ void density_k(Ctx& c, cx<T>* out, const cx<T>* phi, int n, double boxlen, doubl
   check_launch("density_k");
 }
 
+// Grid_FFT::shift_field (grid_fft.hh:832-842) on a field kept in k space,
+// as the reference applies it between the sublattices of a Bravais load
+// (particle_generator.hh:510-522): there the field goes back to real space
+// (c2r) and forward again before each shift, which replaces the kz = 0 and
+// kz = n/2 planes by their Hermitian part, (F(k) + conj F(-k))/2, since c2r
+// reads only the real part of those columns after the x and y transforms;
+// then every mode is multiplied by exp(i s.k dx), k from get_k (+n/2 at
+// Nyquist), s in cells.  Here both happen in one pass: the thread of mode
+// (i, j) of those planes handles its partner (-i, -j) as well.
+template <typename T>
+__global__ void k_lattice_shift(cx<T>* a, int n, double sx, double sy, double sz) {
+  const int nh = n / 2 + 1;
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)n * n * nh) return;
+  const int k = (int)(idx % nh);
+  const long long ij = idx / nh;
+  const int j = (int)(ij % n), i = (int)(ij / n);
+  // angle / pi of exp(i s.k dx), k dx = 2 pi (index - [index > n/2] n) / n
+  auto phase = [&](int ii, int jj, int kk) {
+    const double t = 2.0 * (sx * (double)(ii - (ii > n / 2) * n) + sy * (double)(jj - (jj > n / 2) * n) +
+                            sz * (double)(kk - (kk > n / 2) * n)) / (double)n;
+    double sn, cs;
+    sincospi(t, &sn, &cs);
+    return mk(cs, sn);
+  };
+  auto mul = [](cx<T> v, cx<double> p) {
+    return mk(T((double)v.x * p.x - (double)v.y * p.y), T((double)v.x * p.y + (double)v.y * p.x));
+  };
+  if (k != 0 && k != n / 2) {
+    a[idx] = mul(a[idx], phase(i, j, k));
+    return;
+  }
+  const int ip = (n - i) % n, jp = (n - j) % n;
+  const long long pidx = ((long long)ip * n + jp) * nh + k;
+  if (pidx < idx) return;  // the partner's thread handles the pair
+  const cx<T> u = a[idx], w = a[pidx];
+  // Hermitian part: h(k) = (u + conj w)/2, h(-k) = conj h(k)
+  const cx<double> h = mk(0.5 * ((double)u.x + (double)w.x), 0.5 * ((double)u.y - (double)w.y));
+  const cx<double> p0 = phase(i, j, k);
+  a[idx] = mk(T(h.x * p0.x - h.y * p0.y), T(h.x * p0.y + h.y * p0.x));
+  if (pidx != idx) {
+    const cx<double> p1 = phase(ip, jp, k);
+    a[pidx] = mk(T(h.x * p1.x + h.y * p1.y), T(h.x * p1.y - h.y * p1.x));
+  }
+}
+
+template <typename T>
+void lattice_shift(Ctx& c, cx<T>* a, int n, double sx, double sy, double sz) {
+  const long long cnt = (long long)n * n * (n / 2 + 1);
+  LaunchScope ls(c, "lattice_shift", 2.0 * (double)cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_lattice_shift<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, a, n, sx, sy, sz);
+  check_launch("lattice_shift");
+}
+
 template <typename T>
 void set_particles(Ctx& c, const T* field, double* out, int n, int idim, double lunit, int kind) {
   long long cnt = (long long)n * n * n;
@@ -506,6 +560,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
                              const cx<T>*, const cx<T>*, int, double, int, double, double, const double*, int, Growth); \
   template void set_particles<T>(Ctx&, const T*, double*, int, int, double, int);                                 \
   template void density_k<T>(Ctx&, cx<T>*, const cx<T>*, int, double, double);                                  \
+  template void lattice_shift<T>(Ctx&, cx<T>*, int, double, double, double);                                    \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \
   template int power_spectrum<T>(Ctx&, const cx<T>*, int, double, double*, double*, double*, double*, int);

@@ -102,24 +102,26 @@ class GadgetOutput(_Writer):
 
     def __init__(self, fname, zstart, boxlength, Omega_m, Omega_DE, Omega_b, h, lpt_order, grid_res,
                  double_prec=False, long_ids=False, gadget2_compat=False, rng="MUSIC1", plt="no",
-                 precision="DOUBLE", transfer="eisenstein"):
+                 precision="DOUBLE", transfer="eisenstein", particle_load="sc"):
         super().__init__()
+        self.overload = {"sc": 1, "bcc": 2, "fcc": 4, "rsc": 8}[particle_load]
         p = GadgetParams()
         p.zstart, p.boxlength, p.omega_m, p.omega_de, p.omega_b, p.h = zstart, boxlength, Omega_m, Omega_DE, \
             Omega_b, h
         p.lpt_order, p.grid_res = int(lpt_order), int(grid_res)
         p.double_prec, p.long_ids, p.gadget2_compat = int(double_prec), int(long_ids), int(gadget2_compat)
-        p.particle_load, p.transfer, p.param_set = b"sc", transfer.encode(), b"none"
+        p.particle_load, p.transfer, p.param_set = particle_load.encode(), transfer.encode(), b"none"
         p.rng, p.precision, p.convolutions, p.plt = rng.encode(), precision.encode(), b"ORSZAG", plt.encode()
         self._keep = p
         _call("mfx_out_gadget_create", os.fsencode(fname), ctypes.byref(p), ctypes.byref(self._h))
 
     def write_particles(self, pos, vel, n, omega_species, lscale, vscale, species=1):
-        """pos, vel: three FP64 arrays of n^3 particles each (host)"""
+        """pos, vel: three FP64 arrays of n^3 particles per sublattice each
+        (host), sublattice-major"""
         arrs = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in list(pos) + list(vel)]
         for x in arrs:
-            if x.size != n ** 3:
-                raise ValueError("each component array must hold n^3 values")
+            if x.size != self.overload * n ** 3:
+                raise ValueError("each component array must hold n^3 values per sublattice")
         P = (ctypes.c_void_p * 3)(*[x.ctypes.data for x in arrs[:3]])
         V = (ctypes.c_void_p * 3)(*[x.ctypes.data for x in arrs[3:]])
         _call("mfx_out_gadget_write_particles", self._h, int(species), P, V, int(n), float(omega_species),

@@ -101,6 +101,7 @@ struct mfx_out {
   bool gadget = false;
   bool double_prec = true, long_ids = false, gadget2 = false;
   double boxlength = 0;
+  int overload = 1;  // particles per cell of the particle load
   GadgetHeader hdr;
 };
 
@@ -168,6 +169,13 @@ int mfx_out_gadget_create(const char* fname, const mfx_gadget_params* p, mfx_out
     o->long_ids = p->long_ids != 0;
     o->gadget2 = p->gadget2_compat != 0;
     o->boxlength = p->boxlength;
+    // particles per cell of [setup] ParticleLoad (particle_generator.hh:223)
+    const std::string load = p->particle_load ? p->particle_load : "sc";
+    o->overload = load == "sc" ? 1 : load == "bcc" ? 2 : load == "fcc" ? 4 : load == "rsc" ? 8 : 0;
+    if (o->overload == 0) {
+      delete o;
+      throw IoError("particle load must be sc, bcc, fcc or rsc, not " + load);
+    }
     const double astart = 1.0 / (1.0 + p->zstart);
     GadgetHeader& h = o->hdr;
     h.time = astart;
@@ -212,7 +220,7 @@ int mfx_out_gadget_write_particles(mfx_out* o, int species, const double* const 
     IO_CHECK(o && o->gadget && o->file >= 0, "not a Gadget-HDF5 writer");
     IO_CHECK(species >= 0 && species < 6, "species index must be 0..5");
     for (int d = 0; d < 3; ++d) IO_CHECK(pos && vel && pos[d] && vel[d], "null particle array");
-    const unsigned long long np = (unsigned long long)n * n * n;
+    const unsigned long long ncell = (unsigned long long)n * n * n, np = ncell * (unsigned long long)o->overload;
     GadgetHeader& h = o->hdr;
     h.npart[species] = (unsigned)np;
     h.npartTotal[species] = (unsigned)np;
@@ -256,7 +264,9 @@ int mfx_out_gadget_write_particles(mfx_out* o, int species, const double* const 
     };
     write_vec("Coordinates", pos, lscale);
     write_vec("Velocities", vel, vscale);
-    // ParticleIDs: the 1D cell index (particle_generator.hh:319-338, IDoffset 0)
+    // ParticleIDs: particle p = l ncell + cell of sublattice l gets
+    // overload * cell + l (particle_generator.hh:229-250, IDoffset 0); the
+    // 1D cell index for the sc load
     {
       const hid_t itype = o->long_ids ? H5T_NATIVE_ULLONG : H5T_NATIVE_UINT;
       hsize_t dims[1] = {np};
@@ -268,12 +278,14 @@ int mfx_out_gadget_write_particles(mfx_out* o, int species, const double* const 
       herr_t st = 0;
       for (unsigned long long p0 = 0; p0 < np && st >= 0; p0 += slab) {
         const size_t m = (size_t)std::min<unsigned long long>(slab, np - p0);
+        const unsigned long long ov = (unsigned long long)o->overload;
+        auto id = [&](unsigned long long q) { return ov * (q % ncell) + q / ncell; };
         if (o->long_ids) {
           i64.resize(m);
-          for (size_t q = 0; q < m; ++q) i64[q] = p0 + q;
+          for (size_t q = 0; q < m; ++q) i64[q] = id(p0 + q);
         } else {
           i32.resize(m);
-          for (size_t q = 0; q < m; ++q) i32[q] = (unsigned)(p0 + q);
+          for (size_t q = 0; q < m; ++q) i32[q] = (unsigned)id(p0 + q);
         }
         hsize_t start[1] = {p0}, count[1] = {m};
         H5Sselect_hyperslab(fs, H5S_SELECT_SET, start, nullptr, count, nullptr);

@@ -367,6 +367,44 @@ void orc_assemble_component(const double *phi, const double *phi2, const double 
   orc_fft_c2r(out, n);
 }
 
+/* Grid_FFT::shift_field (grid_fft.hh:832-842) of a real field of side n in
+ * the padded layout, box length L: forward transform, every mode times
+ * exp(i (s_x k_x + s_y k_y + s_z k_z) dx) with k from get_k (kfac = 2 pi / L)
+ * and dx = L / n, backward transform.  orc_lattice_mutation = 1 plants a
+ * wrong sign of the phase (for the tests that must catch it). */
+int orc_lattice_mutation = 0;
+void orc_shift_field(double *f, int n, double boxlen, double sx, double sy, double sz) {
+  const double kfac = 2.0 * PI / boxlen, dx = boxlen / (double)n;
+  const double sg = orc_lattice_mutation == 1 ? -1.0 : 1.0;
+  const size_t nh = nh_of(n);
+  orc_fft_r2c(f, n);
+  for (size_t i = 0; i < (size_t)n; ++i)
+    for (size_t j = 0; j < (size_t)n; ++j)
+      for (size_t k = 0; k < nh; ++k) {
+        const double sh = sg * (sx * orc_get_k(i, n, kfac) * dx + sy * orc_get_k(j, n, kfac) * dx +
+                                sz * orc_get_k(k, n, kfac) * dx);
+        double *z = f + 2 * ((i * n + j) * nh + k);
+        const double c = cos(sh), s = sin(sh), re = z[0], im = z[1];
+        z[0] = re * c - im * s;
+        z[1] = re * s + im * c;
+      }
+  orc_fft_c2r(f, n);
+}
+
+/* set_positions of a Bravais sublattice (particle_generator.hh:525-541) with
+ * the sublattice offset s (cells) along idim: q = (i + s)/n */
+void orc_set_positions_shifted(const double *field, double *pos, int n, int idim, double lunit, double s) {
+  const size_t np = (size_t)n + 2;
+#pragma omp parallel for
+  for (size_t i = 0; i < (size_t)n; ++i)
+    for (size_t j = 0; j < (size_t)n; ++j)
+      for (size_t k = 0; k < (size_t)n; ++k) {
+        size_t ijk[3] = {i, j, k};
+        double q = ((double)ijk[idim] + s) / (double)n;
+        pos[(i * n + j) * n + k] = q * lunit + field[(i * n + j) * np + k];
+      }
+}
+
 /* particle_generator.hh:525-541, sc lattice, 64-bit reals */
 void orc_set_positions(const double *field, double *pos, int n, int idim, double lunit) {
   const size_t np = (size_t)n + 2;

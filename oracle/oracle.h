@@ -96,6 +96,11 @@ void orc_assemble_component(const double *phi, const double *phi2, const double 
                             double vfac, double unit);
 /* pos[p] = q_d*lunit + field (sc lattice, particle order (i,j,k)) */
 void orc_set_positions(const double *field, double *pos, int n, int idim, double lunit);
+/* Grid_FFT::shift_field (grid_fft.hh:832-842) of a real padded field, in place */
+extern int orc_lattice_mutation;
+void orc_shift_field(double *f, int n, double boxlen, double sx, double sy, double sz);
+/* set_positions with a sublattice offset s (cells) along idim */
+void orc_set_positions_shifted(const double *field, double *pos, int n, int idim, double lunit, double s);
 
 /* ---------- reductions / diagnostics (grid_fft.cc:1133-1210) ------------- */
 int orc_power_spectrum(const double *f, int n, double boxlen, double *bin_k, double *bin_P,

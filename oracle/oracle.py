@@ -169,6 +169,57 @@ def set_positions(field, idim, lunit=1.0):
     return pos
 
 
+def set_positions_shifted(field, idim, s, lunit=1.0):
+    """set_positions of a sublattice offset by s cells along idim"""
+    n = field.shape[0]
+    pos = np.zeros(n * n * n)
+    lib().orc_set_positions_shifted(_p(field), _p(pos), ctypes.c_int(n), ctypes.c_int(idim),
+                                    ctypes.c_double(lunit), ctypes.c_double(s))
+    return pos
+
+
+def shift_field(f, boxlen, s):
+    """Grid_FFT::shift_field of a real padded field (in place, returned)"""
+    n = f.shape[0]
+    lib().orc_shift_field(_p(f), ctypes.c_int(n), ctypes.c_double(boxlen), ctypes.c_double(s[0]),
+                          ctypes.c_double(s[1]), ctypes.c_double(s[2]))
+    return f
+
+
+# sublattice offsets of the Bravais loads in cells (particle_generator.hh:47-54)
+LATTICE_SHIFTS = {
+    "sc": [(0, 0, 0)],
+    "bcc": [(0, 0, 0), (0.5, 0.5, 0.5)],
+    "fcc": [(0, 0, 0), (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)],
+    "rsc": [(0, 0, 0), (0, 0, 0.5), (0, 0.5, 0), (0, 0.5, 0.5), (0.5, 0, 0), (0.5, 0, 0.5), (0.5, 0.5, 0),
+            (0.5, 0.5, 0.5)],
+}
+
+
+def lattice_values(field, boxlen, lattice, idim=None, lunit=1.0):
+    """set_positions (idim given) or set_velocities (idim None) of a Bravais
+    load as the reference runs them (particle_generator.hh:499-543,597-638):
+    for sublattice l the field is shifted by the difference to the previous
+    offset and read out; returns (sublattices, n, n, n)"""
+    n = field.shape[0]
+    f = np.array(field, dtype=np.float64, copy=True)
+    sh = LATTICE_SHIFTS[lattice]
+    out = []
+    for l, s in enumerate(sh):
+        if l > 0:
+            shift_field(f, boxlen, [s[d] - sh[l - 1][d] for d in range(3)])
+        if idim is None:
+            out.append(f[:, :, :n].copy())
+        else:
+            out.append(set_positions_shifted(f, idim, s[idim], lunit).reshape(n, n, n))
+    return np.stack(out)
+
+
+def lattice_set_mutation(m):
+    """plant a mistake in shift_field (0 = none, 1 = phase sign)"""
+    ctypes.c_int.in_dll(lib(), "orc_lattice_mutation").value = int(m)
+
+
 def power_spectrum(f, boxlen):
     n = f.shape[0]
     mb = n

@@ -94,6 +94,36 @@ def test_gadget_particles(tmp_path, double_prec, long_ids, gadget2):
             assert list(h.attr("Header", "NumPart_Total_HighWord")[0]) == [0] * 6
 
 
+@pytest.mark.parametrize("load,ov", [("bcc", 2), ("fcc", 4), ("rsc", 8)])
+def test_gadget_particles_of_a_bravais_load(tmp_path, load, ov):
+    """ov n^3 particles, sublattice-major; ParticleIDs = ov * cell + sublattice
+    (particle_generator.hh:229-250); the particle mass divides by the
+    particle count; the load is recorded in ICs_parameters"""
+    from monofonic_amd.io import GadgetOutput
+    n, box, z = 4, 50.0, 24.0
+    rng = np.random.default_rng(ov)
+    pos = [rng.random(ov * n ** 3) for _ in range(3)]
+    vel = [rng.standard_normal(ov * n ** 3) for _ in range(3)]
+    fn = tmp_path / "ics.hdf5"
+    with GadgetOutput(fn, z, box, 0.3, 0.7, 0.045, 0.7, 2, n, double_prec=True, long_ids=True,
+                      particle_load=load) as out:
+        out.write_particles(pos, vel, n, 0.3, 1.0, 1.0)
+        with pytest.raises(ValueError):  # n^3 values per component are not enough
+            out.write_particles([p[: n ** 3] for p in pos], vel, n, 0.3, 1.0, 1.0)
+    with File(fn) as h:
+        c, _ = h.dataset("PartType1/Coordinates")
+        assert c.shape == (ov * n ** 3, 3) and np.array_equal(c, np.stack(pos, 1))
+        ids, _ = h.dataset("PartType1/ParticleIDs")
+        p = np.arange(ov * n ** 3)
+        assert np.array_equal(ids, ov * (p % n ** 3) + p // n ** 3)
+        assert sorted(ids) == list(range(ov * n ** 3))
+        npt, _ = h.attr("Header", "NumPart_Total")
+        assert list(npt) == [0, ov * n ** 3, 0, 0, 0, 0]
+        mt, _ = h.attr("Header", "MassTable")
+        assert mt[1] == 0.3 * (27.7519737 * box ** 3) / (ov * n ** 3)
+        assert h.attr("ICs_parameters", "Particle Load")[0] == load
+
+
 def test_writer_reports_errors(tmp_path):
     from monofonic_amd.io import GenericOutput, MfxIoError
     with pytest.raises(MfxIoError, match="cannot create"):
