@@ -371,10 +371,11 @@ int mfx_batch_destroy(mfx_batch *b);
 int mfx_host_alloc(size_t bytes, void **ptr);
 int mfx_host_free(void *ptr);
 /* grids of the realisation: 0 wnoise, 1 phi, 2 phi2, 3 phi3, 4..6 A3, 7 tmp.
- * After a run the noise holds the k-space field before its normalisation
- * (ic_generator.cc:341-346 normalises in place; here the normalisation is
- * applied as phi(1) reads it) and, for MUSIC1, before its mean subtraction
- * (which only reaches the k = 0 mode, zeroed by phi(1)), the potentials hold the fields before the
+ * After a run the noise grid holds, for NGENIC, the k-space field before its
+ * normalisation (ic_generator.cc:341-346 normalises in place; here the
+ * normalisation is applied as phi(1) reads it) and, for MUSIC1 and Threefry
+ * noise, whose forward transform writes phi(1) from its last pass, an
+ * intermediate (scratch); the potentials hold the fields before the
  * growth factors (ic_generator.cc:550-563 scales them in place; here the
  * factors are applied as the assembly reads them) and tmp is scratch. */
 int mfx_icgen_grid(mfx_icgen *ic, int which, mfx_grid **g);

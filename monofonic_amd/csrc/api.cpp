@@ -1608,11 +1608,21 @@ static void icgen_device_run(mfx_icgen* ic) {
     ngenic_dev(wn, ic->dev_table);
   else
     threefry_dev(wn, (const unsigned long long*)ic->dev_seed);
-  grid_forward(wn, true);
-  // normalisation (341-346) applied as phi(1) (450-457) reads the noise; the
-  // noise grid keeps the unnormalised field
+  // normalisation (341-346) applied as phi(1) (450-457) reads the noise:
+  // for noise filled in real space, by the last pass of its forward
+  // transform (the noise grid is then scratch); NGENIC fills k space
   phi->space = MFX_KSPACE;
-  phi1_impl(phi, wn, ic->amp, noise_norm_spec(volfac, p.fixing, p.inversion));
+  const NoiseNorm nn = noise_norm_spec(volfac, p.fixing, p.inversion);
+  if (wn->space == MFX_RSPACE) {
+    if (wn->prec == MFX_FLOAT)
+      fft_r2c_phi1<float>(ic->ctx->c, (float*)wn->d, (int)p.n, (float)norm_fac(wn), (cx<float>*)phi->d, p.boxlen,
+                          ic->amp, nn);
+    else
+      fft_r2c_phi1<double>(ic->ctx->c, (double*)wn->d, (int)p.n, norm_fac(wn), (cx<double>*)phi->d, p.boxlen,
+                           ic->amp, nn);
+  } else {
+    phi1_impl(phi, wn, ic->amp, nn);
+  }
   // phi(2), phi(3), A(3)
   if (p.lpt_order > 1) {
     if (p.use_fused_lpt)

@@ -21,6 +21,7 @@
 //   P5  x-forward, keep the n retained planes, Nyquist zeroing, sqrt(n^3)/M^3
 //       factor and the output operator (assign/add/subtract/multiply-add)
 #include "fft_passes.hpp"
+#include "ops.hpp"
 
 namespace mfx {
 
@@ -50,6 +51,45 @@ void fft_c2r_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale) {
                       (int)H, n0, "fft_y_inv", 2 * S);
   launch_row_c2r<T>(c, n2 / 2, cd, H, (int)H, data, n2 + 2, (long long)n0 * n1, scale, "fft_z_c2r", 2 * S);
 }
+
+// The last (x) pass of the noise's forward transform writing phi(1) instead
+// of the k-space noise (ic_generator.cc:341-346,450-457): each transformed
+// value is scaled as the plain pass would store it (rounded to T) and turned
+// into phi(1) by the same per-mode function the phi(1) kernel uses, so the
+// result is bit-identical to transform-then-phi(1).  Lines along x (pos = i),
+// batch j, column kz.
+template <typename T>
+struct DstPhi1 {
+  cx<T>* phi;
+  long long sb, sp;
+  int n;
+  T scale;
+  double kfac;
+  const double* amp;
+  NoiseNorm nn;
+  __device__ __forceinline__ cx<T> load_old(int, int, int) const { return mk(T(0), T(0)); }
+  __device__ __forceinline__ void store(int b, int col, int pos, cx<T> v, cx<T>) const {
+    phi[(long long)b * sb + (long long)pos * sp + col] = phi1_value(v * scale, pos, b, col, n, kfac, amp, nn);
+  }
+};
+
+// forward transform of a real-space noise n-grid whose last pass writes
+// phi(1) into phi; the noise grid is left holding its y-stage values
+template <typename T>
+void fft_r2c_phi1(Ctx& c, T* wn, int n, T scale, cx<T>* phi, double boxlen, const double* amp, NoiseNorm nn) {
+  MFX_REQUIRE(n % 2 == 0, "r2c requires an even last dimension");
+  const long long H = n / 2 + 1;
+  cx<T>* cd = reinterpret_cast<cx<T>*>(wn);
+  const double S = (double)n * n * (n + 2) * sizeof(T);
+  launch_row_r2c<T>(c, n / 2, wn, n + 2, cd, H, (int)H, (long long)n * n, T(1), "fft_z_r2c", 2 * S);
+  launch_col<T, false>(c, n, SrcPlain<T>{cd, (long long)n * H, H}, DstPlain<T>{cd, (long long)n * H, H, T(1)},
+                       (int)H, n, "fft_y_fwd", 2 * S);
+  DstPhi1<T> d{phi, H, (long long)n * H, n, scale, 2.0 * M_PI / boxlen, amp, nn};
+  launch_col<T, false>(c, n, SrcPlain<T>{cd, H, (long long)n * H}, d, (int)H, n, "fft_x_fwd_phi1", 2 * S);
+}
+
+template void fft_r2c_phi1<float>(Ctx&, float*, int, float, cx<float>*, double, const double*, NoiseNorm);
+template void fft_r2c_phi1<double>(Ctx&, double*, int, double, cx<double>*, double, const double*, NoiseNorm);
 
 // inverse transform of an n^3 k-space grid into one FP64 particle array: the
 // x and y passes in place, the z pass writing the particles (positions add

@@ -321,14 +321,19 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   }
 }
 
-// Up to three outputs of one source tile (the x and y stages of the Hessian
-// rows): output o transforms the zero-padded source line times the power
-// pw[o] of its wavenumber factor (SrcPadPow) into out[o] (DstPlain strides).
-// The tile is read again from memory for each output instead of being held
-// in registers, which would cost occupancy; a re-read follows the first read
-// of the same workgroup by one transform and is served by the L2 or the
-// MALL, so HBM sees the source once.  o is uniform: the selects below pick
-// kernel-argument fields without indexing into private memory.
+// Several outputs of one launch over the same tiles.  A spec M provides the
+// output count `no` and, for output o, load(o, b, col, pos) and
+// store(o, b, col, pos, v); o is uniform, so a spec picks kernel-argument
+// fields by selects without indexing into private memory.  The kernel reads
+// each output's source values again instead of holding them in registers,
+// which would cost occupancy; a re-read follows the first read of the same
+// workgroup by one transform and is served by the L2 or the MALL, so HBM sees
+// a shared source once.
+//
+// MultiPow: up to three outputs of one source (the x and y stages of the
+// Hessian rows): output o transforms the zero-padded source line times the
+// power pw[o] of its wavenumber factor (SrcPadPow) into out[o] (DstPlain
+// strides).
 template <typename T>
 struct MultiPow {
   SrcPadPow<T> src;
@@ -336,11 +341,21 @@ struct MultiPow {
   cx<T>* out[3];
   int pw[3];
   int no;
+  __device__ __forceinline__ cx<T> load(int o, int b, int col, int pos) const {
+    SrcPadPow<T> s = src;
+    s.pw = o == 0 ? pw[0] : (o == 1 ? pw[1] : pw[2]);
+    return s.load(b, col, pos);
+  }
+  __device__ __forceinline__ void store(int o, int b, int col, int pos, cx<T> v) const {
+    DstPlain<T> d = dst;
+    d.p = o == 0 ? out[0] : (o == 1 ? out[1] : out[2]);
+    d.store(b, col, pos, v, v);
+  }
 };
 
-template <typename T, int L, bool INV, int C, int V = 0>
+template <typename T, int L, bool INV, int C, int V, class M>
 __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
-    k_col_multi(MultiPow<T> a, const cx<T>* __restrict__ tw, int ncols) {
+    k_col_multi(M a, const cx<T>* __restrict__ tw, int ncols) {
   constexpr int EPT = PlanInfo<L, V>::EPT, TPS = PlanInfo<L, V>::TPS;
   extern __shared__ __align__(16) unsigned char col_lds_raw[];  // TwLds<T, L, C>::bytes
   cx<T>* const lds = reinterpret_cast<cx<T>*>(col_lds_raw);
@@ -354,10 +369,6 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
   const ColTw<T, L, C, V> ft(tw, twl);
 #pragma unroll 1
   for (int o = 0; o < a.no; ++o) {
-    SrcPadPow<T> s = a.src;
-    s.pw = o == 0 ? a.pw[0] : (o == 1 ? a.pw[1] : a.pw[2]);
-    DstPlain<T> d = a.dst;
-    d.p = o == 0 ? a.out[0] : (o == 1 ? a.out[1] : a.out[2]);
     // the line position and column pass through an empty asm statement per
     // output, so that the element addresses and wavenumber factors are
     // recomputed for each output instead of being held in registers across
@@ -366,16 +377,32 @@ __global__ void __launch_bounds__((C * PlanInfo<L, V>::TPS))
     asm volatile("" : "+v"(tq), "+v"(cq), "+v"(colq));
     cx<T> v[EPT];
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) v[e] = s.load(b, cq, tq + TPS * e);
+    for (int e = 0; e < EPT; ++e) v[e] = a.load(o, b, cq, tq + TPS * e);
     // the previous output's last exchange ended with a barrier after its
     // last LDS read, so this transform's first exchange may write
     ft.template exec<INV>(v, lds + c, C, t);
     if (act) {
 #pragma unroll
-      for (int e = 0; e < EPT; ++e) d.store(b, colq, tq + TPS * e, v[e], v[e]);
+      for (int e = 0; e < EPT; ++e) a.store(o, b, colq, tq + TPS * e, v[e]);
     }
   }
 }
+
+// one output of a multi-output spec as a plain source / destination (the
+// generic-length path runs one pass per output)
+template <typename T, class M>
+struct MultiSrc {
+  M m;
+  int o;
+  __device__ __forceinline__ cx<T> load(int b, int col, int pos) const { return m.load(o, b, col, pos); }
+};
+template <typename T, class M>
+struct MultiDst {
+  M m;
+  int o;
+  __device__ __forceinline__ cx<T> load_old(int, int, int) const { return mk(T(0), T(0)); }
+  __device__ __forceinline__ void store(int b, int col, int pos, cx<T> v, cx<T>) const { m.store(o, b, col, pos, v); }
+};
 
 // Projection of a padded line onto the modes of the n-grid: forward
 // transform, truncation to the n kept modes with the Nyquist mode and the
@@ -785,20 +812,20 @@ void launch_turn(Ctx& c, int L, const TurnArgs<T>& a, int ncols, int nbatch, con
   }
 }
 
-template <typename T, int L, int C, bool INV, int V>
-void launch_multi_LC(Ctx& c, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+template <typename T, int L, int C, bool INV, int V, class M>
+void launch_multi_LC(Ctx& c, const M& a, int ncols, int nbatch, const char* name, double bytes) {
   const dim3 block(C * PlanInfo<L, V>::TPS);
   const int ntiles = (ncols + C - 1) / C;
   LaunchScope ls(c, name, bytes);
   constexpr size_t dyn = TwLds<T, L, C>::bytes;
-  auto k = k_col_multi<T, L, INV, C, V>;
+  auto k = k_col_multi<T, L, INV, C, V, M>;
   col_lds_prep(c, k, dyn);
   hipLaunchKernelGGL(k, dim3(ntiles, nbatch), block, dyn, c.stream, a, c.twiddles<T>(L), ncols);
   check_launch(name);
 }
 
-template <typename T, int L, bool INV, int V>
-void launch_multi_LV(Ctx& c, int v, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
+template <typename T, int L, bool INV, int V, class M>
+void launch_multi_LV(Ctx& c, int v, const M& a, int ncols, int nbatch, const char* name, double bytes) {
   constexpr int CM = ColCfg<T, L, V>::C, CW = ColCfg<T, L, V>::CW;
   if constexpr (CW > 0) {
     if ((v & COL_WIDTH) == CW) {
@@ -815,37 +842,40 @@ void launch_multi_LV(Ctx& c, int v, const MultiPow<T>& a, int ncols, int nbatch,
   launch_multi_LC<T, L, CM, INV, V>(c, a, ncols, nbatch, name, bytes);
 }
 
-// at the tile width tuned for (L, T, INV); the multi-output kernel has no
-// pipelined form and always runs plan 0 (with the second plan it compiles to
-// 179 instead of 95 VGPRs, two waves per SIMD instead of five)
-template <typename T, int L, bool INV>
-void launch_multi_L(Ctx& c, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bytes) {
-  const int v = c.col_tile(L, (int)sizeof(T), INV, ColCfg<T, L>::C, stride_class(a.src.sp, (int)sizeof(cx<T>)));
+// at the tile width tuned for (L, T, INV) and the stride class of line
+// stride sp; the multi-output kernel has no pipelined form and always runs
+// plan 0 (with the second plan the three-output Hessian pass compiles to 179
+// instead of 95 VGPRs, two waves per SIMD instead of five)
+template <typename T, int L, bool INV, class M>
+void launch_multi_L(Ctx& c, const M& a, long long sp, int ncols, int nbatch, const char* name, double bytes) {
+  const int v = c.col_tile(L, (int)sizeof(T), INV, ColCfg<T, L>::C, stride_class(sp, (int)sizeof(cx<T>)));
   launch_multi_LV<T, L, INV, 0>(c, v, a, ncols, nbatch, name, bytes);
 }
 
-// a.no outputs of one source; bytes: the source read once plus a.no outputs.
-// Lengths without tuned kernels run one generic pass per output.
+// a.no outputs of one launch; sp is the line stride of the source (for the
+// tuning class); bytes: the algorithmic bytes of the launch.  Lengths
+// without tuned kernels run one generic pass per output.
+template <typename T, bool INV, class M>
+void launch_multi(Ctx& c, int L, const M& a, long long sp, int ncols, int nbatch, const char* name, double bytes) {
+  switch (L) {
+#define MFX_CASE(LL)                                                \
+  case LL:                                                          \
+    launch_multi_L<T, LL, INV>(c, a, sp, ncols, nbatch, name, bytes); \
+    return;
+    MFX_COL_LENGTHS(MFX_CASE)
+#undef MFX_CASE
+    default:
+      for (int o = 0; o < a.no; ++o)
+        launch_col_gen<T, INV>(c, L, MultiSrc<T, M>{a, o}, MultiDst<T, M>{a, o}, ncols, nbatch, name, bytes / a.no);
+  }
+}
+
+// MultiPow outputs of one source; bytes: the source read once plus a.no outputs
 template <typename T, bool INV>
 void launch_col_multi(Ctx& c, int L, const MultiPow<T>& a, int ncols, int nbatch, const char* name, double bsrc,
                       double bdst) {
   MFX_REQUIRE(a.no >= 1 && a.no <= 3, "multi-output pass: 1 to 3 outputs");
-  switch (L) {
-#define MFX_CASE(LL)                                                                          \
-  case LL:                                                                                    \
-    launch_multi_L<T, LL, INV>(c, a, ncols, nbatch, name, bsrc + a.no * bdst);                \
-    return;
-    MFX_PAD_LENGTHS(MFX_CASE)
-#undef MFX_CASE
-    default:
-      for (int o = 0; o < a.no; ++o) {
-        SrcPadPow<T> s = a.src;
-        s.pw = a.pw[o];
-        DstPlain<T> d = a.dst;
-        d.p = a.out[o];
-        launch_col_gen<T, INV>(c, L, s, d, ncols, nbatch, name, bsrc + bdst);
-      }
-  }
+  launch_multi<T, INV>(c, L, a, a.src.sp, ncols, nbatch, name, bsrc + a.no * bdst);
 }
 
 // ------------------------------------------------------------------ row kernels (z axis)

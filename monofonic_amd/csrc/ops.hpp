@@ -23,8 +23,43 @@ inline NoiseNorm noise_norm_spec(double volfac, int fixing, int inversion) {
   nn.sgn = inversion ? -1.0 : 1.0;
   return nn;
 }
+
+// the normalisation of one noise mode (1/volfac, amplitude fixing, phase
+// inversion; ic_generator.cc:341-346), rounded to T as the grid stores it
+template <typename T>
+__device__ __forceinline__ cx<T> norm_noise(cx<T> v, const NoiseNorm& nn) {
+  double re = v.x, im = v.y;
+  if (nn.fixing) {
+    double a = hypot(re, im);
+    if (a != 0.0) {
+      re /= a;
+      im /= a;
+    }
+  }
+  return mk(T(nn.sgn * re * nn.ivolfac), T(nn.sgn * im * nn.ivolfac));
+}
+
+// phi(1) of one mode (ic_generator.cc:450-457): -wn A(|k|) / |k|^2 with A
+// tabulated (FP64) on the integer |n|^2 = ix^2 + iy^2 + iz^2, zero at k = 0;
+// with nn.on the noise value w is normalised first (norm_noise)
+template <typename T>
+__device__ __forceinline__ cx<T> phi1_value(cx<T> w, int i, int j, int k, int n, double kfac,
+                                            const double* __restrict__ amp, const NoiseNorm& nn) {
+  const int ix = i - (i > n / 2) * n, iy = j - (j > n / 2) * n, iz = k - (k > n / 2) * n;
+  const double kx = ix * kfac, ky = iy * kfac, kz = iz * kfac;
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  const long long n2 = (long long)ix * ix + (long long)iy * iy + (long long)iz * iz;
+  if (nn.on) w = norm_noise(w, nn);
+  const double f = (n2 == 0) ? 0.0 : amp[n2] / k2;
+  return mk(T(-(double)w.x * f), T(-(double)w.y * f));
+}
 template <typename T>
 void phi1(Ctx& c, cx<T>* phi, const cx<T>* wn, int n, double boxlen, const double* amp, NoiseNorm nn = NoiseNorm{});
+// forward transform of real-space noise whose last pass writes phi(1) (the
+// noise grid is left as scratch); bit-identical to the transform followed by
+// phi1 (fft_kernels.hip)
+template <typename T>
+void fft_r2c_phi1(Ctx& c, T* wn, int n, T scale, cx<T>* phi, double boxlen, const double* amp, NoiseNorm nn);
 template <typename T>
 void inverse_laplacian(Ctx& c, cx<T>* f, int n, double boxlen);
 template <typename T>

@@ -24,19 +24,6 @@ __device__ __forceinline__ double gd_of(int i, int n, double kfac) {
 // ------------------------------------------------------------------ noise normalisation
 // ic_generator.cc:341-346: wn <- (inv ? -1 : 1) * [fixing ? wn/|wn| : wn] / volfac
 template <typename T>
-__device__ __forceinline__ cx<T> norm_noise(cx<T> v, const NoiseNorm& nn) {
-  double re = v.x, im = v.y;
-  if (nn.fixing) {
-    double a = hypot(re, im);
-    if (a != 0.0) {
-      re /= a;
-      im /= a;
-    }
-  }
-  return mk(T(nn.sgn * re * nn.ivolfac), T(nn.sgn * im * nn.ivolfac));
-}
-
-template <typename T>
 __global__ void k_noise_norm(cx<T>* wn, long long cnt, NoiseNorm nn) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= cnt) return;
@@ -57,14 +44,7 @@ __global__ void k_phi1(cx<T>* phi, const cx<T>* wn, int n, double kfac, const do
   int k = (int)(idx % nh);
   I ij = idx / nh;
   int j = (int)(ij % nn_), i = (int)(ij / nn_);
-  int ix = i - (i > n / 2) * n, iy = j - (j > n / 2) * n, iz = k - (k > n / 2) * n;
-  double kx = kd_of(i, n, kfac), ky = kd_of(j, n, kfac), kz = kd_of(k, n, kfac);
-  double k2 = kx * kx + ky * ky + kz * kz;
-  long long n2 = (long long)ix * ix + (long long)iy * iy + (long long)iz * iz;
-  cx<T> w = wn[idx];
-  if (nn.on) w = norm_noise(w, nn);
-  double f = (idx == 0) ? 0.0 : amp[n2] / k2;
-  phi[idx] = mk(T(-(double)w.x * f), T(-(double)w.y * f));
+  phi[idx] = phi1_value(wn[idx], i, j, k, n, kfac, amp, nn);
 }
 
 // grid_fft.hh:440-449
@@ -123,10 +103,14 @@ __global__ void k_fill(T* f, long long cnt, T v) {
 // one component from the growth-scaled potentials s1..s3 and A3 components
 // a3 (each rounded to T as the reference's in-place scaling of a real_t grid
 // stores it), the gradient g and the PLT velocity factor vc
+// The same from scalars: gd, gp, gpp are the gradient components along
+// idim, idim+1 and idim+2 (cyclic), a1 and a2 the A3 components idim+1 and
+// idim+2 (callers whose idim is not a compile-time constant pick them with
+// selects, keeping everything in registers)
 template <typename T, bool PLT>
-__device__ __forceinline__ cx<T> assemble_value(int kind, int idim, int order, cx<T> s1, cx<T> s2, cx<T> s3,
-                                                const cx<T> (&a3)[3], const double (&g)[3], double vc, double vfac,
-                                                double unit_over_L) {
+__device__ __forceinline__ cx<T> assemble_core(int kind, int order, cx<T> s1, cx<T> s2, cx<T> s3, cx<T> a1, cx<T> a2,
+                                               double gd, double gp, double gpp, double vc, double vfac,
+                                               double unit_over_L) {
   double pr, pi;
   if (kind == 0) {
     pr = s1.x;
@@ -151,12 +135,10 @@ __device__ __forceinline__ cx<T> assemble_value(int kind, int idim, int order, c
       pi += 3 * vfac * s3.y;
     }
   }
-  const int idp = (idim + 1) % 3, idpp = (idim + 2) % 3;
-  double tr = -g[idim] * pi, ti = g[idim] * pr;
+  double tr = -gd * pi, ti = gd * pr;
   if (order > 2) {
-    const cx<T> a2 = a3[idpp], a1 = a3[idp];
-    double cr = (-g[idp] * (double)a2.y) - (-g[idpp] * (double)a1.y);
-    double ci = (g[idp] * (double)a2.x) - (g[idpp] * (double)a1.x);
+    double cr = (-gp * (double)a2.y) - (-gpp * (double)a1.y);
+    double ci = (gp * (double)a2.x) - (gpp * (double)a1.x);
     if (kind == 1) {
       cr *= 3 * vfac;
       ci *= 3 * vfac;
@@ -171,6 +153,15 @@ __device__ __forceinline__ cx<T> assemble_value(int kind, int idim, int order, c
   tr *= unit_over_L;
   ti *= unit_over_L;
   return mk(T(tr), T(ti));
+}
+
+template <typename T, bool PLT>
+__device__ __forceinline__ cx<T> assemble_value(int kind, int idim, int order, cx<T> s1, cx<T> s2, cx<T> s3,
+                                                const cx<T> (&a3)[3], const double (&g)[3], double vc, double vfac,
+                                                double unit_over_L) {
+  const int idp = (idim + 1) % 3, idpp = (idim + 2) % 3;
+  return assemble_core<T, PLT>(kind, order, s1, s2, s3, a3[idp], a3[idpp], g[idim], g[idp], g[idpp], vc, vfac,
+                               unit_over_L);
 }
 
 template <typename T>
