@@ -313,6 +313,12 @@ typedef struct {
                            3 rsc, i.e. 1, 2, 4 or 8 particles per cell
                            (particle_generator.hh:47-54,499-543,597-638); not with PLT,
                            field output or assemble_each */
+  int baryons;          /* [setup] DoBaryons (ic_generator.cc:153-168,608-671): a second species
+                           on the secondary lattice (particle_generator.hh:56-63), arrays 7..12;
+                           with the Eisenstein-Hu transfer function its difference modes
+                           delta_bc, theta_bc are zero (transfer_eisenstein.cc:240), so both
+                           species follow the total-matter fields; particle output only, not
+                           with PLT or assemble_each */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;
@@ -335,13 +341,15 @@ int mfx_icgen_launch(mfx_icgen *ic);
  * (displacements with field_output), 3..5 velocities x,y,z, 6 the density
  * (field_output only) (n^3 doubles each, order (i,j,k); for a bcc, fcc or
  * rsc load arrays 0..5 hold 2, 4 or 8 such blocks, one per sublattice, as
- * particle_generator.hh stores them).  With the fused
+ * particle_generator.hh stores them); with baryons, 7..12 the baryons'
+ * positions and velocities on the secondary lattice.  With the fused
  * LPT engine they may share device memory with its workspace: valid until the
  * next mfx_icgen_run */
 int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
 /* Asynchronous download of the output arrays of the last realisation: bit w
- * of `what` copies array w (0..5 as mfx_icgen_particles, 6 the density of
- * field output; n^3 doubles each, times the particles per cell for 0..5)
+ * of `what` copies array w (0..12 as mfx_icgen_particles: 6 the density of
+ * field output, 7..12 the baryons'; n^3 doubles each, times the particles
+ * per cell except for 6; dst has an entry per bit up to the highest set)
  * into host buffer dst[w] on the
  * generator's copy stream, after the realisation's last kernel.  Host
  * buffers from mfx_host_alloc (pinned) make the copy asynchronous.  The next
@@ -349,7 +357,7 @@ int mfx_icgen_particles(mfx_icgen *ic, int what, double **dev_ptr);
  * overwrites the arrays (the assembly, with own_particles; its first kernel
  * otherwise); mfx_icgen_launch waits before the replay.  No counterpart in
  * the reference, whose output plugins read host memory (output_plugin.hh). */
-int mfx_icgen_download_async(mfx_icgen *ic, unsigned what, void *const dst[7]);
+int mfx_icgen_download_async(mfx_icgen *ic, unsigned what, void *const dst[]);
 /* blocks until the copies of mfx_icgen_download_async are complete */
 int mfx_icgen_download_wait(mfx_icgen *ic);
 /* Batched realisations (no counterpart in the reference, which generates

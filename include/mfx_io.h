@@ -63,7 +63,11 @@ int mfx_out_gadget_create(const char *fname, const mfx_gadget_params *p, mfx_out
  * Coordinates = pos * lscale and Velocities = vel * vscale (interleaved to
  * N x 3); ParticleIDs = overload * ((i n + j) n + k) + sublattice
  * (particle_generator.hh:229-250); the mass table entry
- * omega_species * rhoc L^3 / N (rhoc = 27.7519737 h^2 1e10 Msun/Mpc^3) */
+ * omega_species * rhoc L^3 / N (rhoc = 27.7519737 h^2 1e10 Msun/Mpc^3).
+ * With do_baryons (DoBaryons) the masses go to a Masses dataset instead
+ * (the mass table entry is 0), every particle (1/N) (omega_species rhoc L^3)
+ * (the Eisenstein-Hu transfer function has no baryon-CDM difference mode),
+ * and species 0 (baryons) numbers its particles from N on (IDoffset 1). */
 int mfx_out_gadget_write_particles(mfx_out *o, int species, const double *const pos[3], const double *const vel[3],
                                    size_t n, double omega_species, double lscale, double vscale);
 

@@ -57,7 +57,7 @@ class IcgenParams(ctypes.Structure):
                 ("zero_radiation", ctypes.c_int), ("use_fused_lpt", ctypes.c_int), ("use_plt", ctypes.c_int),
                 ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int),
                 ("assemble_each", ctypes.c_int), ("field_output", ctypes.c_int), ("own_particles", ctypes.c_int),
-                ("lattice", ctypes.c_int)]
+                ("lattice", ctypes.c_int), ("baryons", ctypes.c_int)]
 
 
 def load():

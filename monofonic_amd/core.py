@@ -521,7 +521,8 @@ class ICGenerator:
     def __init__(self, ctx, GridRes, BoxLength, zstart, LPTorder=3, precision="double", generator="MUSIC1",
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
-                 autotune=True, assemble_each=False, field_output=False, own_particles=False, ParticleLoad="sc"):
+                 autotune=True, assemble_each=False, field_output=False, own_particles=False, ParticleLoad="sc",
+                 DoBaryons=False):
         self.ctx = ctx
         self.n = int(GridRes)
         self.lattice = str(ParticleLoad)
@@ -530,7 +531,8 @@ class ICGenerator:
         self.cosmo = dict(H0=H0, Omega_m=Omega_m, Omega_b=Omega_b)
         self.generator, self.plt = generator, bool(PLT)
         self.field_output = bool(field_output)
-        self._host = [None] * 7
+        self.baryons = bool(DoBaryons)
+        self._host = [None] * 13
         p = _lib.IcgenParams()
         p.n, p.boxlen, p.zstart, p.lpt_order = self.n, float(BoxLength), float(zstart), int(LPTorder)
         p.precision, p.generator, p.seed = _prec(precision), GENERATORS[generator], int(seed)
@@ -542,6 +544,7 @@ class ICGenerator:
         p.assemble_each = int(assemble_each)
         p.field_output, p.own_particles = int(field_output), int(own_particles)
         p.lattice = LATTICES[self.lattice]
+        p.baryons = int(self.baryons)
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 
@@ -553,7 +556,7 @@ class ICGenerator:
             for b in self._host:
                 if b is not None:
                     call("mfx_host_free", ctypes.c_void_p(b[0]))
-            self._host = [None] * 7
+            self._host = [None] * 13
         except Exception:
             pass
 
@@ -574,10 +577,11 @@ class ICGenerator:
 
     def download_async(self, what=range(6)):
         """start copying output arrays `what` (0..5, 6 = density of the field
-        output) of the last realisation into pinned host buffers; the next
-        run overlaps these copies until it overwrites the arrays"""
+        output, 7..12 the baryons') of the last realisation into pinned host
+        buffers; the next run overlaps these copies until it overwrites the
+        arrays"""
         mask = 0
-        dst = (ctypes.c_void_p * 7)()
+        dst = (ctypes.c_void_p * 13)()
         for w in what:
             self._pinned(w)
             dst[w] = self._host[w][0]
@@ -617,10 +621,15 @@ class ICGenerator:
         a = 1.0 / (1.0 + self.zstart)
         L = self.boxlen
         h = self.cosmo["H0"] / 100.0
-        with GadgetOutput(fname, self.zstart, L, self.cosmo["Omega_m"], self._omega_de(), self.cosmo["Omega_b"], h,
+        Om, Ob = self.cosmo["Omega_m"], self.cosmo["Omega_b"]
+        with GadgetOutput(fname, self.zstart, L, Om, self._omega_de(), Ob, h,
                           self.lpt_order, self.n, double_prec=double_prec, long_ids=long_ids, rng=self.generator,
-                          plt="yes" if self.plt else "no", particle_load=self.lattice) as out:
-            out.write_particles(arrs[0:3], arrs[3:6], self.n, self.cosmo["Omega_m"], L, L / np.sqrt(a))
+                          plt="yes" if self.plt else "no", particle_load=self.lattice,
+                          do_baryons=self.baryons) as out:
+            # ic_generator.cc:158-168: with baryons the dark matter carries Omega_m - Omega_b
+            out.write_particles(arrs[0:3], arrs[3:6], self.n, Om - Ob if self.baryons else Om, L, L / np.sqrt(a))
+            if self.baryons:
+                out.write_particles(arrs[7:10], arrs[10:13], self.n, Ob, L, L / np.sqrt(a), species=0)
 
     def _omega_de(self):
         c = CosmologyCalculator(H0=self.cosmo["H0"], Omega_m=self.cosmo["Omega_m"], Omega_b=self.cosmo["Omega_b"])
@@ -656,9 +665,12 @@ class ICGenerator:
         n = self.n
         return self.ctx.d2h(self.particles_dev(6), n ** 3 * 8, np.float64).reshape(n, n, n)
 
-    def particles(self, what):
-        """host copy of positions (0..2) or velocities (3..5): shape (n,n,n)
+    def particles(self, what, species="dm"):
+        """host copy of positions (0..2) or velocities (3..5) of the dark
+        matter or (species "baryon", DoBaryons) the baryons: shape (n,n,n)
         for the sc load, (sublattice, n, n, n) for bcc, fcc and rsc"""
+        if species == "baryon":
+            what = what + 7
         shp = self._shape(what)
         return self.ctx.d2h(self.particles_dev(what), int(np.prod(shp)) * 8, np.float64).reshape(shp)
 

@@ -1342,13 +1342,17 @@ static const double kLatticeShift[4][8][3] = {
     {{0, 0, 0}, {0, 0.5, 0.5}, {0.5, 0, 0.5}, {0.5, 0.5, 0}},
     {{0, 0, 0}, {0, 0, 0.5}, {0, 0.5, 0}, {0, 0.5, 0.5}, {0.5, 0, 0}, {0.5, 0, 0.5}, {0.5, 0.5, 0}, {0.5, 0.5, 0.5}}};
 
+// offset of the secondary lattice on which baryons are placed, in cells
+// (particle_generator.hh:56-63): sc (CsCl), bcc, fcc (NaCl), rsc
+static const double kSecondLatticeShift[4][3] = {{0.5, 0.5, 0.5}, {0.5, 0.5, 0.0}, {0.5, 0.5, 0.5}, {0.25, 0.25, 0.25}};
+
 struct mfx_icgen {
   mfx_ctx* ctx;
   mfx_icgen_params p;
   std::unique_ptr<Cosmology> cosmo;
   mfx_grid* g[8] = {};  // wnoise, phi, phi2, phi3, A3x, A3y, A3z, tmp
   mfx_conv* conv = nullptr;
-  double* part[6] = {};
+  double* part[12] = {};  // dark matter 0..5, baryons 6..11 (DoBaryons)
   double* amp = nullptr;
   mfx_plt* plt = nullptr;
   bool part_in_ws = false;  // particle arrays live in the LPT workspace
@@ -1390,9 +1394,10 @@ static void icgen_free(mfx_icgen* ic) {
       gg = nullptr;
     }
   if (ic->conv) mfx_conv_destroy(ic->conv);
-  for (auto*& p : ic->part) {
-    if (p && !ic->part_in_ws) ic->ctx->c.release(p);
-    p = nullptr;
+  for (int w = 0; w < 12; ++w) {
+    // the baryon arrays (6..11) always have their own allocation
+    if (ic->part[w] && (w >= 6 || !ic->part_in_ws)) ic->ctx->c.release(ic->part[w]);
+    ic->part[w] = nullptr;
   }
   if (ic->comp6 && !ic->comp6_in_ws) ic->ctx->c.release(ic->comp6);
   ic->comp6 = nullptr;
@@ -1432,6 +1437,13 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
                 "field output writes grid fields; the particle load (lattice) applies to particle output only");
     MFX_REQUIRE(p->lattice == 0 || !p->assemble_each, "bcc/fcc/rsc loads need the six-component assembly "
                                                       "(assemble_each = 0)");
+    // DoBaryons (ic_generator.cc:153-168,608-671): a second species on the
+    // secondary lattice; with the Eisenstein-Hu transfer function the baryon-
+    // CDM difference modes are zero (transfer_eisenstein.cc:240), so both
+    // species move with the total-matter fields and have uniform masses
+    MFX_REQUIRE(p->baryons == 0 || p->baryons == 1, "baryons must be 0 or 1");
+    MFX_REQUIRE(!p->baryons || (!p->use_plt && !p->field_output && !p->assemble_each),
+                "baryons: particle output with the six-component assembly, without PLT");
     auto* ic = new mfx_icgen();
     ic->ctx = c;
     ic->p = *p;
@@ -1465,7 +1477,9 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
         }
       }
       if (!ic->part_in_ws)
-        for (auto*& pp : ic->part) pp = (double*)c->c.alloc(sizeof(double) * npart);
+        for (int w = 0; w < 6; ++w) ic->part[w] = (double*)c->c.alloc(sizeof(double) * npart);
+      if (p->baryons)
+        for (int w = 6; w < 12; ++w) ic->part[w] = (double*)c->c.alloc(sizeof(double) * npart);
       // six component grids: after the particles in the LPT workspace, else
       // their own allocation when it leaves 8 GiB free
       const size_t comp_bytes = 6 * (size_t)p->n * p->n * (p->n / 2 + 1) * 2 * (size_t)p->precision;
@@ -1477,7 +1491,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       } else {
         size_t fr = 0, tot = 0;
         MFX_HIP(hipMemGetInfo(&fr, &tot));
-        if (comp_bytes + ((size_t)8 << 30) <= fr || p->lattice > 0) ic->comp6 = c->c.alloc(comp_bytes);
+        if (comp_bytes + ((size_t)8 << 30) <= fr || p->lattice > 0 || p->baryons) ic->comp6 = c->c.alloc(comp_bytes);
       }
       if (p->field_output) ic->delta = (double*)c->c.alloc(sizeof(double) * ncell);
       MFX_HIP(hipStreamCreate(&ic->copy_stream));
@@ -1664,16 +1678,24 @@ static void icgen_device_run(mfx_icgen* ic) {
   // kLatticeShift[l] (shift_field by the difference to the previous
   // sublattice, on the k-space component kept in comp6), positions
   // (i + s)/n + Psi; array w holds the sublattices one after another
+  // For the baryons (species 1) the component is first shifted to the
+  // secondary lattice (the shift_field of particle_generator.hh:512-516);
+  // their arrays are part[6 + w].
   const long long ncell = (long long)p.n * p.n * p.n;
-  auto sublattices = [&](int w, auto* comp) {
+  auto sublattices = [&](int w, auto* comp, int species) {
     using TT = decltype(comp->x);
     const double(*s)[3] = kLatticeShift[p.lattice];
+    const double* s2 = kSecondLatticeShift[p.lattice];
+    const double o[3] = {species ? s2[0] : 0.0, species ? s2[1] : 0.0, species ? s2[2] : 0.0};
     for (int l = 0; l < ic->overload; ++l) {
       if (l > 0)
         lattice_shift<TT>(ic->ctx->c, comp, (int)p.n, s[l][0] - s[l - 1][0], s[l][1] - s[l - 1][1],
                           s[l][2] - s[l - 1][2]);
-      fft_c2r_particles<TT>(ic->ctx->c, (TT*)tmp->d, (int)p.n, (TT)norm_fac(tmp), ic->part[w] + l * ncell, w % 3, 1.0,
-                            okind(w), (const TT*)comp, s[l][w % 3]);
+      else if (species)
+        lattice_shift<TT>(ic->ctx->c, comp, (int)p.n, o[0], o[1], o[2]);
+      fft_c2r_particles<TT>(ic->ctx->c, (TT*)tmp->d, (int)p.n, (TT)norm_fac(tmp),
+                            ic->part[6 * species + w] + l * ncell, w % 3, 1.0, okind(w), (const TT*)comp,
+                            s[l][w % 3] + o[w % 3]);
     }
   };
   if (ic->comp6) {
@@ -1683,23 +1705,37 @@ static void icgen_device_run(mfx_icgen* ic) {
                        (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]), (const cx<float>*)P(A3[1]),
                        (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order, ic->vfact, 1.0, plt_tab, plt_ng,
                        gr);
-      for (int w = 0; w < 6; ++w)
-        if (ic->overload == 1)
-          fft_c2r_particles<float>(ic->ctx->c, (float*)((cx<float>*)ic->comp6 + w * ws), (int)p.n,
-                                   (float)norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
-        else
-          sublattices(w, (cx<float>*)ic->comp6 + w * ws);
+      for (int sp = 0; sp <= p.baryons; ++sp) {
+        if (sp > 0)  // the baryons' components afresh, as the reference assembles them per species
+          assemble6<float>(ic->ctx->c, (cx<float>*)ic->comp6, ws, (const cx<float>*)phi->d,
+                           (const cx<float>*)P(phi2), (const cx<float>*)P(phi3), (const cx<float>*)P(A3[0]),
+                           (const cx<float>*)P(A3[1]), (const cx<float>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order,
+                           ic->vfact, 1.0, plt_tab, plt_ng, gr);
+        for (int w = 0; w < 6; ++w)
+          if (ic->overload == 1 && sp == 0 && !p.baryons)
+            fft_c2r_particles<float>(ic->ctx->c, (float*)((cx<float>*)ic->comp6 + w * ws), (int)p.n,
+                                     (float)norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
+          else
+            sublattices(w, (cx<float>*)ic->comp6 + w * ws, sp);
+      }
     } else {
       assemble6<double>(ic->ctx->c, (cx<double>*)ic->comp6, ws, (const cx<double>*)phi->d,
                         (const cx<double>*)P(phi2), (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]),
                         (const cx<double>*)P(A3[1]), (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen, p.lpt_order,
                         ic->vfact, 1.0, plt_tab, plt_ng, gr);
-      for (int w = 0; w < 6; ++w)
-        if (ic->overload == 1)
-          fft_c2r_particles<double>(ic->ctx->c, (double*)((cx<double>*)ic->comp6 + w * ws), (int)p.n,
-                                    norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
-        else
-          sublattices(w, (cx<double>*)ic->comp6 + w * ws);
+      for (int sp = 0; sp <= p.baryons; ++sp) {
+        if (sp > 0)
+          assemble6<double>(ic->ctx->c, (cx<double>*)ic->comp6, ws, (const cx<double>*)phi->d,
+                            (const cx<double>*)P(phi2), (const cx<double>*)P(phi3), (const cx<double>*)P(A3[0]),
+                            (const cx<double>*)P(A3[1]), (const cx<double>*)P(A3[2]), (int)p.n, p.boxlen,
+                            p.lpt_order, ic->vfact, 1.0, plt_tab, plt_ng, gr);
+        for (int w = 0; w < 6; ++w)
+          if (ic->overload == 1 && sp == 0 && !p.baryons)
+            fft_c2r_particles<double>(ic->ctx->c, (double*)((cx<double>*)ic->comp6 + w * ws), (int)p.n,
+                                      norm_fac(tmp), ic->part[w], w % 3, 1.0, okind(w));
+          else
+            sublattices(w, (cx<double>*)ic->comp6 + w * ws, sp);
+      }
     }
     density();
     return;
@@ -1794,24 +1830,27 @@ extern "C" int mfx_icgen_launch(mfx_icgen* ic) {
 
 extern "C" int mfx_icgen_particles(mfx_icgen* ic, int what, double** dev_ptr) {
   return guard([&] {
-    MFX_REQUIRE(ic && dev_ptr && what >= 0 && what <= 6, "invalid argument");
-    MFX_REQUIRE(what < 6 || ic->delta, "the density is written with field_output only");
-    *dev_ptr = what < 6 ? ic->part[what] : ic->delta;
+    MFX_REQUIRE(ic && dev_ptr && what >= 0 && what <= 12, "invalid argument");
+    MFX_REQUIRE(what != 6 || ic->delta, "the density is written with field_output only");
+    MFX_REQUIRE(what < 7 || ic->p.baryons, "arrays 7..12 are the baryons' (DoBaryons only)");
+    *dev_ptr = what < 6 ? ic->part[what] : what == 6 ? ic->delta : ic->part[what - 1];
   });
 }
 
-extern "C" int mfx_icgen_download_async(mfx_icgen* ic, unsigned what, void* const dst[7]) {
+extern "C" int mfx_icgen_download_async(mfx_icgen* ic, unsigned what, void* const dst[]) {
   return guard([&] {
     MFX_REQUIRE(ic && dst, "null argument");
-    MFX_REQUIRE((what >> 7) == 0, "what: bits 0..6 only");
+    MFX_REQUIRE((what >> 13) == 0, "what: bits 0..12 only");
     MFX_REQUIRE(!(what & 64u) || ic->delta, "the density is written with field_output only");
+    MFX_REQUIRE((what >> 7) == 0 || ic->p.baryons, "arrays 7..12 are the baryons' (DoBaryons only)");
     const size_t bytes = sizeof(double) * ic->p.n * ic->p.n * ic->p.n;
     MFX_HIP(hipStreamWaitEvent(ic->copy_stream, ic->run_done, 0));
-    for (int w = 0; w < 7; ++w)
+    for (int w = 0; w < 13; ++w)
       if (what & (1u << w)) {
         MFX_REQUIRE(dst[w], "null destination buffer");
-        MFX_HIP(hipMemcpyAsync(dst[w], w < 6 ? ic->part[w] : ic->delta, w < 6 ? bytes * ic->overload : bytes,
-                               hipMemcpyDeviceToHost, ic->copy_stream));
+        const double* src = w < 6 ? ic->part[w] : w == 6 ? ic->delta : ic->part[w - 1];
+        MFX_HIP(hipMemcpyAsync(dst[w], src, w == 6 ? bytes : bytes * ic->overload, hipMemcpyDeviceToHost,
+                               ic->copy_stream));
       }
     MFX_HIP(hipEventRecord(ic->copy_done, ic->copy_stream));
     ic->copy_pending = true;

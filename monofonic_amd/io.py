@@ -102,13 +102,14 @@ class GadgetOutput(_Writer):
 
     def __init__(self, fname, zstart, boxlength, Omega_m, Omega_DE, Omega_b, h, lpt_order, grid_res,
                  double_prec=False, long_ids=False, gadget2_compat=False, rng="MUSIC1", plt="no",
-                 precision="DOUBLE", transfer="eisenstein", particle_load="sc"):
+                 precision="DOUBLE", transfer="eisenstein", particle_load="sc", do_baryons=False):
         super().__init__()
         self.overload = {"sc": 1, "bcc": 2, "fcc": 4, "rsc": 8}[particle_load]
         p = GadgetParams()
         p.zstart, p.boxlength, p.omega_m, p.omega_de, p.omega_b, p.h = zstart, boxlength, Omega_m, Omega_DE, \
             Omega_b, h
         p.lpt_order, p.grid_res = int(lpt_order), int(grid_res)
+        p.do_baryons = int(do_baryons)
         p.double_prec, p.long_ids, p.gadget2_compat = int(double_prec), int(long_ids), int(gadget2_compat)
         p.particle_load, p.transfer, p.param_set = particle_load.encode(), transfer.encode(), b"none"
         p.rng, p.precision, p.convolutions, p.plt = rng.encode(), precision.encode(), b"ORSZAG", plt.encode()

@@ -102,6 +102,7 @@ struct mfx_out {
   bool double_prec = true, long_ids = false, gadget2 = false;
   double boxlength = 0;
   int overload = 1;  // particles per cell of the particle load
+  bool masses = false;  // individual particle masses (DoBaryons)
   GadgetHeader hdr;
 };
 
@@ -176,6 +177,9 @@ int mfx_out_gadget_create(const char* fname, const mfx_gadget_params* p, mfx_out
       delete o;
       throw IoError("particle load must be sc, bcc, fcc or rsc, not " + load);
     }
+    // with baryons every species is written with individual masses
+    // (ic_generator.cc:637, particle_generator.hh:380-420, output_gadget_hdf5.cc:269-300)
+    o->masses = p->do_baryons != 0;
     const double astart = 1.0 / (1.0 + p->zstart);
     GadgetHeader& h = o->hdr;
     h.time = astart;
@@ -229,7 +233,10 @@ int mfx_out_gadget_write_particles(mfx_out* o, int species, const double* const 
     h.npartTotal64[species] = np;
     const double rhoc = 27.7519737;  // h^2 1e10 Msun / Mpc^3
     const double munit = rhoc * std::pow(o->boxlength, 3);
-    h.mass[species] = omega_species * munit / (double)np;
+    h.mass[species] = o->masses ? 0.0 : omega_species * munit / (double)np;
+    // baryons (species 0) number their particles after the dark matter's:
+    // IDoffset 1 (ic_generator.cc:629, particle_generator.hh:231)
+    const unsigned long long idoff = (o->masses && species == 0) ? np : 0;
 
     const std::string grp = "PartType" + std::to_string(species);
     hid_t g = make_group(o->file, grp.c_str());
@@ -279,7 +286,7 @@ int mfx_out_gadget_write_particles(mfx_out* o, int species, const double* const 
       for (unsigned long long p0 = 0; p0 < np && st >= 0; p0 += slab) {
         const size_t m = (size_t)std::min<unsigned long long>(slab, np - p0);
         const unsigned long long ov = (unsigned long long)o->overload;
-        auto id = [&](unsigned long long q) { return ov * (q % ncell) + q / ncell; };
+        auto id = [&](unsigned long long q) { return idoff + ov * (q % ncell) + q / ncell; };
         if (o->long_ids) {
           i64.resize(m);
           for (size_t q = 0; q < m; ++q) i64[q] = id(p0 + q);
@@ -296,6 +303,31 @@ int mfx_out_gadget_write_particles(mfx_out* o, int species, const double* const 
       H5Dclose(ds);
       H5Sclose(fs);
       IO_CHECK(st >= 0, "cannot write dataset ParticleIDs");
+    }
+    // Masses: pmeanmass * rho with pmeanmass = 1/N and rho = (1 + C delta_bc)
+    // Omega munit, delta_bc = 0 for the Eisenstein-Hu transfer function
+    // (ic_generator.cc:650-667, particle_generator.hh:383-411)
+    if (o->masses) {
+      const double pm = (1.0 / (double)np) * (omega_species * munit);
+      hsize_t dims[1] = {np};
+      hid_t fs = H5Screate_simple(1, dims, nullptr);
+      hid_t ds = H5Dcreate2(g, "Masses", rtype, fs, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+      IO_CHECK(ds >= 0, "cannot create dataset Masses");
+      herr_t st = 0;
+      for (unsigned long long p0 = 0; p0 < np && st >= 0; p0 += slab) {
+        const size_t m = (size_t)std::min<unsigned long long>(slab, np - p0);
+        std::vector<double> b64;
+        std::vector<float> b32;
+        if (o->double_prec) b64.assign(m, pm); else b32.assign(m, (float)pm);
+        hsize_t start[1] = {p0}, count[1] = {m};
+        H5Sselect_hyperslab(fs, H5S_SELECT_SET, start, nullptr, count, nullptr);
+        hid_t ms = H5Screate_simple(1, count, nullptr);
+        st = H5Dwrite(ds, rtype, ms, fs, H5P_DEFAULT, o->double_prec ? (const void*)b64.data() : (const void*)b32.data());
+        H5Sclose(ms);
+      }
+      H5Dclose(ds);
+      H5Sclose(fs);
+      IO_CHECK(st >= 0, "cannot write dataset Masses");
     }
     H5Gclose(g);
   });

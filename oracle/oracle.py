@@ -196,22 +196,31 @@ LATTICE_SHIFTS = {
 }
 
 
-def lattice_values(field, boxlen, lattice, idim=None, lunit=1.0):
+# offset of the secondary (baryon) lattice in cells (particle_generator.hh:56-63)
+SECOND_LATTICE_SHIFT = {"sc": (0.5, 0.5, 0.5), "bcc": (0.5, 0.5, 0.0), "fcc": (0.5, 0.5, 0.5),
+                        "rsc": (0.25, 0.25, 0.25)}
+
+
+def lattice_values(field, boxlen, lattice, idim=None, lunit=1.0, secondary=False):
     """set_positions (idim given) or set_velocities (idim None) of a Bravais
     load as the reference runs them (particle_generator.hh:499-543,597-638):
-    for sublattice l the field is shifted by the difference to the previous
-    offset and read out; returns (sublattices, n, n, n)"""
+    on the secondary lattice (lattice_index 1) the field is first shifted by
+    its offset; for sublattice l it is shifted by the difference to the
+    previous offset and read out; returns (sublattices, n, n, n)"""
     n = field.shape[0]
     f = np.array(field, dtype=np.float64, copy=True)
     sh = LATTICE_SHIFTS[lattice]
+    o = SECOND_LATTICE_SHIFT[lattice] if secondary else (0.0, 0.0, 0.0)
     out = []
     for l, s in enumerate(sh):
+        if l == 0 and secondary:
+            shift_field(f, boxlen, o)
         if l > 0:
             shift_field(f, boxlen, [s[d] - sh[l - 1][d] for d in range(3)])
         if idim is None:
             out.append(f[:, :, :n].copy())
         else:
-            out.append(set_positions_shifted(f, idim, s[idim], lunit).reshape(n, n, n))
+            out.append(set_positions_shifted(f, idim, s[idim] + o[idim], lunit).reshape(n, n, n))
     return np.stack(out)
 
 

@@ -82,12 +82,14 @@ def oracle_noise(orc, n, seed, generator="MUSIC1"):
     return wn
 
 
-def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, generator="MUSIC1", lattice="sc"):
+def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, generator="MUSIC1", lattice="sc",
+                    secondary=False):
     """ic_generator::run restated on the CPU: white noise (MUSIC1, NGENIC or
     the Threefry stream), phi(1..3), A(3), growth scaling and FP64
     positions/velocities of the sc lattice, or, for lattice bcc/fcc/rsc,
-    arrays (sublattice, n, n, n) as set_positions/set_velocities fill them;
-    with plt_tabs the assembly uses the PLT gradient (ENABLE_PLT builds)"""
+    arrays (sublattice, n, n, n) as set_positions/set_velocities fill them
+    (on the secondary, baryon lattice with secondary); with plt_tabs the
+    assembly uses the PLT gradient (ENABLE_PLT builds)"""
     a = 1.0 / (1.0 + zstart)
     D = cos.Dplus(a)
     vfac = cos.vfact(a)
@@ -114,8 +116,8 @@ def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, gen
                 f = orc.assemble_component(phi, phi2, phi3, A3, boxlen, order, kind, idim, vfac, 1.0)
             else:
                 f = orc.assemble_component_plt(phi, phi2, phi3, A3, plt_tabs, boxlen, order, kind, idim, vfac, 1.0)
-            if lattice != "sc":
-                out.append(orc.lattice_values(f, boxlen, lattice, idim if kind == 0 else None, 1.0))
+            if lattice != "sc" or secondary:
+                out.append(orc.lattice_values(f, boxlen, lattice, idim if kind == 0 else None, 1.0, secondary))
             elif kind == 0:
                 out.append(orc.set_positions(f, idim, 1.0).reshape(n, n, n))
             else:

@@ -124,6 +124,37 @@ def test_gadget_particles_of_a_bravais_load(tmp_path, load, ov):
         assert h.attr("ICs_parameters", "Particle Load")[0] == load
 
 
+def test_gadget_particles_with_baryons(tmp_path):
+    """DoBaryons: both species with a Masses dataset ((1/N) Omega rhoc L^3,
+    the mass table 0), the baryons (PartType0) numbered after the dark
+    matter (IDoffset 1)"""
+    from monofonic_amd.io import GadgetOutput
+    n, box, z, ov = 4, 50.0, 24.0, 2
+    rng = np.random.default_rng(3)
+    arr = lambda: [rng.random(ov * n ** 3) for _ in range(3)]
+    fn = tmp_path / "ics.hdf5"
+    Om, Ob = 0.3, 0.045
+    with GadgetOutput(fn, z, box, Om, 0.7, Ob, 0.7, 3, n, double_prec=True, long_ids=True, particle_load="bcc",
+                      do_baryons=True) as out:
+        out.write_particles(arr(), arr(), n, Om - Ob, 1.0, 1.0, species=1)
+        out.write_particles(arr(), arr(), n, Ob, 1.0, 1.0, species=0)
+    munit = 27.7519737 * box ** 3
+    N = ov * n ** 3
+    with File(fn) as h:
+        mt, _ = h.attr("Header", "MassTable")
+        assert list(mt) == [0.0] * 6
+        npt, _ = h.attr("Header", "NumPart_Total")
+        assert list(npt) == [N, N, 0, 0, 0, 0]
+        for sp, om, off in ((1, Om - Ob, 0), (0, Ob, N)):
+            m, dt = h.dataset(f"PartType{sp}/Masses")
+            assert dt == np.float64 and m.shape == (N,)
+            assert np.all(m == (1.0 / N) * (om * munit))
+            ids, _ = h.dataset(f"PartType{sp}/ParticleIDs")
+            p = np.arange(N)
+            assert np.array_equal(ids, off + ov * (p % n ** 3) + p // n ** 3)
+        assert h.attr("ICs_parameters", "Baryons")[0] == 1
+
+
 def test_writer_reports_errors(tmp_path):
     from monofonic_amd.io import GenericOutput, MfxIoError
     with pytest.raises(MfxIoError, match="cannot create"):
