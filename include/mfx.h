@@ -319,6 +319,12 @@ typedef struct {
                            delta_bc, theta_bc are zero (transfer_eisenstein.cc:240), so both
                            species follow the total-matter fields; particle output only, not
                            with PLT or assemble_each */
+  int eulerian;         /* with field_output: [output] generic_out_eulerian (output_generic.cc:
+                           43,63-68), the semiclassical (PPT) Eulerian fields of one species
+                           (ic_generator.cc:674-790) instead of the Lagrangian ones: array 6
+                           the density |psi|^2 - 1 (DM_delta), arrays 3..5 the velocities
+                           (DM_vx..DM_vz); arrays 0..2 are not written.  Not with PLT; grid
+                           sizes 16..1024 of the form 2^k or 3*2^k */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;
@@ -337,6 +343,10 @@ int mfx_icgen_run(mfx_icgen *ic);
 int mfx_icgen_set_seed(mfx_icgen *ic, long seed);
 int mfx_icgen_capture(mfx_icgen *ic);
 int mfx_icgen_launch(mfx_icgen *ic);
+/* hbar of the last realisation's semiclassical fields (ic_generator.cc:718-725:
+ * max |phi(neighbour) - phi| / pi / D+ * 1.01, computed on the device); waits
+ * for the realisation.  Eulerian field output only. */
+int mfx_icgen_hbar(mfx_icgen *ic, double *hbar);
 /* device pointers of the FP64 particle arrays: what 0..2 positions x,y,z
  * (displacements with field_output), 3..5 velocities x,y,z, 6 the density
  * (field_output only) (n^3 doubles each, order (i,j,k); for a bcc, fcc or

@@ -32,7 +32,7 @@ EXPORTS = [
     "mfx_rng_ranlxd1_raw", "mfx_rng_threefry_uniforms", "mfx_cosmo_create", "mfx_cosmo_destroy",
     "mfx_cosmo_query", "mfx_cosmo_amplitude_table", "mfx_icgen_create", "mfx_icgen_destroy", "mfx_icgen_run",
     "mfx_icgen_set_seed", "mfx_icgen_capture", "mfx_icgen_launch",
-    "mfx_icgen_particles", "mfx_icgen_grid", "mfx_icgen_scalars", "mfx_icgen_download_async",
+    "mfx_icgen_particles", "mfx_icgen_grid", "mfx_icgen_scalars", "mfx_icgen_hbar", "mfx_icgen_download_async",
     "mfx_icgen_download_wait", "mfx_host_alloc", "mfx_host_free", "mfx_batch_create", "mfx_batch_launch",
     "mfx_batch_synchronize", "mfx_batch_destroy",
 ]
@@ -57,7 +57,8 @@ class IcgenParams(ctypes.Structure):
                 ("zero_radiation", ctypes.c_int), ("use_fused_lpt", ctypes.c_int), ("use_plt", ctypes.c_int),
                 ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int),
                 ("assemble_each", ctypes.c_int), ("field_output", ctypes.c_int), ("own_particles", ctypes.c_int),
-                ("lattice", ctypes.c_int), ("baryons", ctypes.c_int)]
+                ("lattice", ctypes.c_int), ("baryons", ctypes.c_int),
+                ("eulerian", ctypes.c_int)]
 
 
 def load():
@@ -135,6 +136,7 @@ def load():
         "mfx_batch_create": [ctypes.POINTER(vp), i32, i32, pp], "mfx_batch_launch": [vp],
         "mfx_batch_synchronize": [vp], "mfx_batch_destroy": [vp],
         "mfx_icgen_scalars": [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)],
+        "mfx_icgen_hbar": [vp, ctypes.POINTER(dbl)],
     }
     for name, args in sig.items():
         f = getattr(L, name)

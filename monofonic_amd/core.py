@@ -522,7 +522,7 @@ class ICGenerator:
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
                  autotune=True, assemble_each=False, field_output=False, own_particles=False, ParticleLoad="sc",
-                 DoBaryons=False):
+                 DoBaryons=False, generic_out_eulerian=False):
         self.ctx = ctx
         self.n = int(GridRes)
         self.lattice = str(ParticleLoad)
@@ -532,6 +532,8 @@ class ICGenerator:
         self.generator, self.plt = generator, bool(PLT)
         self.field_output = bool(field_output)
         self.baryons = bool(DoBaryons)
+        # [output] generic_out_eulerian: semiclassical density and velocity fields
+        self.eulerian = bool(generic_out_eulerian)
         self._host = [None] * 13
         p = _lib.IcgenParams()
         p.n, p.boxlen, p.zstart, p.lpt_order = self.n, float(BoxLength), float(zstart), int(LPTorder)
@@ -545,6 +547,7 @@ class ICGenerator:
         p.field_output, p.own_particles = int(field_output), int(own_particles)
         p.lattice = LATTICES[self.lattice]
         p.baryons = int(self.baryons)
+        p.eulerian = int(self.eulerian)
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 
@@ -596,8 +599,9 @@ class ICGenerator:
 
     def write_generic(self, fname, fields=None):
         """generic HDF5 output (src/plugins/output_generic.cc): DM_dx..DM_vz
-        and, with field_output, DM_delta, from the downloaded arrays (field
-        output only: displacements without the lattice, lunit = vunit = 1)"""
+        and DM_delta from the downloaded arrays (field output only:
+        displacements without the lattice, lunit = vunit = 1); with
+        generic_out_eulerian the semiclassical DM_delta and DM_vx..DM_vz"""
         from .io import GenericOutput
         if not self.field_output:
             raise ValueError("generic output needs field_output=True")
@@ -607,7 +611,7 @@ class ICGenerator:
         Om, Ob = self.cosmo["Omega_m"], self.cosmo["Omega_b"]
         with GenericOutput(fname, self.boxlen, astart, Ob, Om, self._omega_de()) as out:
             for w, name in enumerate(names):
-                if arrs[w] is not None:
+                if arrs[w] is not None and not (self.eulerian and w < 3):
                     out.write_field(name, np.ascontiguousarray(arrs[w]))
 
     def write_gadget(self, fname, double_prec=True, long_ids=False, arrays=None):
@@ -649,6 +653,12 @@ class ICGenerator:
     def launch(self):
         """replay the captured realisation for the staged seed (asynchronous)"""
         call("mfx_icgen_launch", self._h)
+
+    def hbar(self):
+        """hbar of the last realisation's semiclassical fields (Eulerian output)"""
+        h = ctypes.c_double()
+        call("mfx_icgen_hbar", self._h, ctypes.byref(h))
+        return h.value
 
     def scalars(self):
         d, v = ctypes.c_double(), ctypes.c_double()

@@ -1384,6 +1384,10 @@ struct mfx_icgen {
   bool copy_pending = false;
   int in_batches = 0;  // batches whose graph records this generator
   int overload = 1;    // particles per cell: 1 sc, 2 bcc, 4 fcc, 8 rsc
+  // Eulerian (semiclassical) field output: psi and two work buffers (n^3
+  // complex each) and the device scalars (max |dphi| bits, hbar)
+  void* ppt_buf = nullptr;
+  double* ppt_scal = nullptr;
 };
 
 static void icgen_free(mfx_icgen* ic) {
@@ -1406,6 +1410,8 @@ static void icgen_free(mfx_icgen* ic) {
   if (ic->gexec) (void)hipGraphExecDestroy(ic->gexec);
   if (ic->graph) (void)hipGraphDestroy(ic->graph);
   if (ic->delta) ic->ctx->c.release(ic->delta);
+  if (ic->ppt_buf) ic->ctx->c.release(ic->ppt_buf);
+  if (ic->ppt_scal) ic->ctx->c.release(ic->ppt_scal);
   if (ic->copy_stream) {
     (void)hipStreamSynchronize(ic->copy_stream);
     (void)hipStreamDestroy(ic->copy_stream);
@@ -1444,6 +1450,13 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
     MFX_REQUIRE(p->baryons == 0 || p->baryons == 1, "baryons must be 0 or 1");
     MFX_REQUIRE(!p->baryons || (!p->use_plt && !p->field_output && !p->assemble_each),
                 "baryons: particle output with the six-component assembly, without PLT");
+    // generic_out_eulerian (output_generic.cc:43,63-68): the semiclassical
+    // fields of ic_generator.cc:674-790 replace the Lagrangian assembly
+    MFX_REQUIRE(p->eulerian == 0 || p->eulerian == 1, "eulerian must be 0 or 1");
+    MFX_REQUIRE(!p->eulerian || (p->field_output && !p->use_plt),
+                "Eulerian fields are a field output (field_output = 1) without the PLT assembly");
+    MFX_REQUIRE(!p->eulerian || ppt_supported((int)p->n),
+                "Eulerian fields: no column transform kernels for this grid size (16..1024, 2^k or 3*2^k)");
     auto* ic = new mfx_icgen();
     ic->ctx = c;
     ic->p = *p;
@@ -1483,8 +1496,8 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       // six component grids: after the particles in the LPT workspace, else
       // their own allocation when it leaves 8 GiB free
       const size_t comp_bytes = 6 * (size_t)p->n * p->n * (p->n / 2 + 1) * 2 * (size_t)p->precision;
-      if (p->assemble_each) {
-        // one component at a time in tmp
+      if (p->assemble_each || p->eulerian) {
+        // one component at a time in tmp; the Eulerian fields assemble nothing
       } else if (ic->part_in_ws && ic->conv->lpt_bytes >= 6 * npart * sizeof(double) + comp_bytes) {
         ic->comp6 = (char*)ic->conv->lpt_ws + 6 * npart * sizeof(double);
         ic->comp6_in_ws = true;
@@ -1494,6 +1507,10 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
         if (comp_bytes + ((size_t)8 << 30) <= fr || p->lattice > 0 || p->baryons) ic->comp6 = c->c.alloc(comp_bytes);
       }
       if (p->field_output) ic->delta = (double*)c->c.alloc(sizeof(double) * ncell);
+      if (p->eulerian) {
+        ic->ppt_buf = c->c.alloc(3 * ncell * 2 * (size_t)p->precision);
+        ic->ppt_scal = (double*)c->c.alloc(ppt_scalar_bytes());
+      }
       MFX_HIP(hipStreamCreate(&ic->copy_stream));
       MFX_HIP(hipEventCreateWithFlags(&ic->run_done, hipEventDisableTiming));
       MFX_HIP(hipEventCreateWithFlags(&ic->copy_done, hipEventDisableTiming));
@@ -1606,6 +1623,35 @@ static void icgen_wait_copies(mfx_icgen* ic) {
   }
 }
 
+// Semiclassical (PPT) Eulerian fields (ic_generator.cc:674-790): phi and
+// phi2 to real space with the growth factors g1, g2 (550-554) in the final
+// scale, then psi, the drift, delta into the density array and the
+// velocities into arrays 3..5 (vunit = D vfact / L, velocity_unit() = 1 in
+// output_generic.cc:76).  The potential grids are left in real space.
+static void icgen_eulerian(mfx_icgen* ic, double g1, double g2) {
+  const mfx_icgen_params& p = ic->p;
+  mfx_grid *phi = ic->g[1], *phi2 = p.lpt_order > 1 ? ic->g[2] : nullptr;
+  const int n = (int)p.n;
+  const double vunit = ic->dplus * ic->vfact / p.boxlen;
+  double* const vel[3] = {ic->part[3], ic->part[4], ic->part[5]};
+  const size_t ncell = p.n * p.n * p.n;
+  if (phi->prec == MFX_FLOAT) {
+    fft_c2r_3d<float>(ic->ctx->c, (float*)phi->d, n, n, n, (float)(norm_fac(phi) * g1));
+    if (phi2) fft_c2r_3d<float>(ic->ctx->c, (float*)phi2->d, n, n, n, (float)(norm_fac(phi2) * g2));
+    cx<float>* w = (cx<float>*)ic->ppt_buf;
+    ppt_fields<float>(ic->ctx->c, (const float*)phi->d, phi2 ? (const float*)phi2->d : nullptr, n, p.boxlen,
+                      ic->dplus, vunit, w, w + ncell, w + 2 * ncell, ic->delta, vel, ic->ppt_scal);
+  } else {
+    fft_c2r_3d<double>(ic->ctx->c, (double*)phi->d, n, n, n, norm_fac(phi) * g1);
+    if (phi2) fft_c2r_3d<double>(ic->ctx->c, (double*)phi2->d, n, n, n, norm_fac(phi2) * g2);
+    cx<double>* w = (cx<double>*)ic->ppt_buf;
+    ppt_fields<double>(ic->ctx->c, (const double*)phi->d, phi2 ? (const double*)phi2->d : nullptr, n, p.boxlen,
+                       ic->dplus, vunit, w, w + ncell, w + 2 * ncell, ic->delta, vel, ic->ppt_scal);
+  }
+  phi->space = MFX_RSPACE;
+  if (phi2) phi2->space = MFX_RSPACE;
+}
+
 static void icgen_device_run(mfx_icgen* ic) {
   const mfx_icgen_params& p = ic->p;
   if (ic->part_in_ws) icgen_wait_copies(ic);
@@ -1637,12 +1683,13 @@ static void icgen_device_run(mfx_icgen* ic) {
   } else {
     phi1_impl(phi, wn, ic->amp, nn);
   }
-  // phi(2), phi(3), A(3)
+  // phi(2), phi(3), A(3); the Eulerian fields read phi and phi2 only
   if (p.lpt_order > 1) {
+    const int order = p.eulerian ? 2 : p.lpt_order;
     if (p.use_fused_lpt)
-      lpt_fused(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
+      lpt_fused(ic->conv, phi, phi2, phi3, A3, order);
     else
-      lpt_sequence(ic->conv, phi, phi2, phi3, A3, p.lpt_order);
+      lpt_sequence(ic->conv, phi, phi2, phi3, A3, order);
   }
   // growth factors (550-563), applied in the assembly as the potentials are
   // read; the potential grids keep the unscaled fields
@@ -1658,6 +1705,10 @@ static void icgen_device_run(mfx_icgen* ic) {
   const double* plt_tab = ic->plt ? ic->plt->tab : nullptr;
   const int plt_ng = ic->plt ? ic->plt->ng : 0;
   icgen_wait_copies(ic);
+  if (p.eulerian) {
+    icgen_eulerian(ic, g1, g2);
+    return;
+  }
   // output kind per array: positions add the lattice coordinate, the field
   // output's displacements and the velocities do not
   auto okind = [&](int w) { return (w < 3 && !p.field_output) ? 0 : 1; };
@@ -1833,6 +1884,7 @@ extern "C" int mfx_icgen_particles(mfx_icgen* ic, int what, double** dev_ptr) {
     MFX_REQUIRE(ic && dev_ptr && what >= 0 && what <= 12, "invalid argument");
     MFX_REQUIRE(what != 6 || ic->delta, "the density is written with field_output only");
     MFX_REQUIRE(what < 7 || ic->p.baryons, "arrays 7..12 are the baryons' (DoBaryons only)");
+    MFX_REQUIRE(what > 2 || !ic->p.eulerian, "Eulerian field output has no displacement arrays (0..2)");
     *dev_ptr = what < 6 ? ic->part[what] : what == 6 ? ic->delta : ic->part[what - 1];
   });
 }
@@ -1843,6 +1895,7 @@ extern "C" int mfx_icgen_download_async(mfx_icgen* ic, unsigned what, void* cons
     MFX_REQUIRE((what >> 13) == 0, "what: bits 0..12 only");
     MFX_REQUIRE(!(what & 64u) || ic->delta, "the density is written with field_output only");
     MFX_REQUIRE((what >> 7) == 0 || ic->p.baryons, "arrays 7..12 are the baryons' (DoBaryons only)");
+    MFX_REQUIRE(!(what & 7u) || !ic->p.eulerian, "Eulerian field output has no displacement arrays (0..2)");
     const size_t bytes = sizeof(double) * ic->p.n * ic->p.n * ic->p.n;
     MFX_HIP(hipStreamWaitEvent(ic->copy_stream, ic->run_done, 0));
     for (int w = 0; w < 13; ++w)
@@ -1883,6 +1936,15 @@ extern "C" int mfx_icgen_grid(mfx_icgen* ic, int which, mfx_grid** g) {
     MFX_REQUIRE(ic && g && which >= 0 && which < 8, "invalid argument");
     MFX_REQUIRE(ic->g[which] != nullptr, "grid not allocated at this LPT order");
     *g = ic->g[which];
+  });
+}
+
+extern "C" int mfx_icgen_hbar(mfx_icgen* ic, double* hbar) {
+  return guard([&] {
+    MFX_REQUIRE(ic && hbar, "null argument");
+    MFX_REQUIRE(ic->ppt_scal, "hbar is computed with the Eulerian field output only");
+    MFX_HIP(hipStreamSynchronize(ic->ctx->c.stream));
+    MFX_HIP(hipMemcpy(hbar, ic->ppt_scal + 1, sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 

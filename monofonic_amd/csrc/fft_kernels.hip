@@ -112,6 +112,23 @@ void fft_c2r_particles(Ctx& c, T* data, int n, T scale, double* out, int idim, d
                          qshift / (double)n * lunit, "fft_z_c2r_part", S + (double)n * n * n * sizeof(double));
 }
 
+// one column pass over plain strided lines (element (b, col, pos) at
+// b*sb + pos*sp + col), from src to dst (which may be the same), with the
+// context's tuned kernel for length L
+template <typename T, bool INV>
+void col_pass_plain(Ctx& c, int L, const cx<T>* src, cx<T>* dst, long long sb, long long sp, int ncols, int nbatch,
+                    T scale, const char* name, double bytes) {
+  launch_col<T, INV>(c, L, SrcPlain<T>{src, sb, sp}, DstPlain<T>{dst, sb, sp, scale}, ncols, nbatch, name, bytes);
+}
+
+template void col_pass_plain<float, false>(Ctx&, int, const cx<float>*, cx<float>*, long long, long long, int, int,
+                                           float, const char*, double);
+template void col_pass_plain<float, true>(Ctx&, int, const cx<float>*, cx<float>*, long long, long long, int, int,
+                                          float, const char*, double);
+template void col_pass_plain<double, false>(Ctx&, int, const cx<double>*, cx<double>*, long long, long long, int,
+                                            int, double, const char*, double);
+template void col_pass_plain<double, true>(Ctx&, int, const cx<double>*, cx<double>*, long long, long long, int,
+                                           int, double, const char*, double);
 template void fft_c2r_particles<float>(Ctx&, float*, int, float, double*, int, double, int, const float*, double);
 template void fft_c2r_particles<double>(Ctx&, double*, int, double, double*, int, double, int, const double*, double);
 template void fft_r2c_3d<float>(Ctx&, float*, int, int, int, float);

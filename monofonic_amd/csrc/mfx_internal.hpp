@@ -139,6 +139,23 @@ void fft_c2r_3d(Ctx& c, T* data, int n0, int n1, int n2, T scale);
 template <typename T>
 void fft_c2r_particles(Ctx& c, T* data, int n, T scale, double* out, int idim, double lunit, int kind,
                        const T* src = nullptr, double qshift = 0.0);
+// one column pass over plain strided lines (element (b, col, pos) at
+// b*sb + pos*sp + col) from src to dst, the tuned kernel for length L
+template <typename T, bool INV>
+void col_pass_plain(Ctx& c, int L, const cx<T>* src, cx<T>* dst, long long sb, long long sp, int ncols, int nbatch,
+                    T scale, const char* name, double bytes);
+
+// Semiclassical (PPT) Eulerian fields of one species (ic_generator.cc:674-790)
+// from the growth-scaled real-space potentials phi, phi2 (padded rows; phi2
+// null for LPTorder 1): delta = |psi|^2 - 1 and vel[d] (n^3 doubles each,
+// order (i,j,k)); psi, wa, wb are n^3 complex work buffers, scal
+// ppt_scalar_bytes() of device memory whose second double holds hbar after
+// the call.
+template <typename T>
+void ppt_fields(Ctx& c, const T* phi, const T* phi2, int n, double boxlen, double D, double vunit, cx<T>* psi,
+                cx<T>* wa, cx<T>* wb, double* delta, double* const vel[3], double* scal);
+size_t ppt_scalar_bytes();
+bool ppt_supported(int n);
 
 // Orszag 3/2 product of two n-grid k-space fields with multipliers, written
 // through op into out (k space n-grid).  ws must hold conv_workspace_bytes.

@@ -158,3 +158,20 @@ void orc_fft_c2r(double *data, int n) {
 #pragma omp parallel for
   for (size_t i = 0; i < ntot; ++i) data[i] *= norm;
 }
+
+/* Grid_FFT<ccomplex_t> forward (sign -1) or backward (+1) transform of an
+ * n^3 complex grid [i][j][k] in place, unitary (grid_fft.cc:209-270 with the
+ * complex plans of :120-140) */
+void orc_fft_c2c(double *z, int n, int sign) {
+  cz *c = (cz *)z;
+  const size_t nn = (size_t)n;
+  lines_c2c(c, n, nn * nn, nn, 1, 1, sign);  /* along k */
+  lines_c2c(c, n, nn, nn * nn, nn, nn, sign);  /* along j */
+  lines_c2c(c, n, 1, 0, nn * nn, nn * nn, sign); /* along i */
+  const double norm = 1.0 / sqrt((double)n * (double)n * (double)n);
+  for (size_t q = 0; q < nn * nn * nn; ++q) {
+    c[q].re *= norm;
+    c[q].im *= norm;
+  }
+}
+

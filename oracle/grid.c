@@ -456,3 +456,95 @@ int orc_power_spectrum(const double *f, int n, double boxlen, double *bin_k, dou
   }
   return nbins;
 }
+
+/* The semiclassical (PPT) Eulerian fields of ic_generator.cc:674-790 for one
+ * species without baryon modes (rho = sqrt(1 + C delta_bc) = 1): phi, phi2
+ * are the growth-scaled real-space potentials (padded layout; phi2 NULL for
+ * LPTorder 1); D = Dplus0, vunit = D vfac / L * velocity_unit.  Outputs:
+ * delta = |psi|^2 - 1 and v[d] (n^3 each, order (i,j,k)), and hbar.
+ * orc_ppt_mutation = 1 plants a wrong sign of the drift phase. */
+int orc_ppt_mutation = 0;
+void orc_fft_c2c(double *z, int n, int sign);
+void orc_ppt_fields(const double *phi, const double *phi2, int n, double boxlen, double D, double vunit,
+                    double *delta, double *v, double *hbar_out) {
+  const size_t nn = (size_t)n, np = nn + 2, N = nn * nn * nn;
+  const double kfac = 2.0 * PI / boxlen;
+  /* hbar from the largest neighbour difference of phi (:699-722): i stops one
+   * plane short, j and k wrap */
+  double maxdphi = -1.0;
+  for (size_t i = 0; i + 1 < nn; ++i)
+    for (size_t j = 0; j < nn; ++j)
+      for (size_t k = 0; k < nn; ++k) {
+        const size_t ir = i + 1, jr = (j + 1) % nn, kr = (k + 1) % nn;
+        const double c = phi[(i * nn + j) * np + k];
+        const double dx = fabs(phi[(ir * nn + j) * np + k] - c), dy = fabs(phi[(i * nn + jr) * np + k] - c),
+                     dz = fabs(phi[(i * nn + j) * np + kr] - c);
+        maxdphi = fmax(maxdphi, fmax(dx, fmax(dy, dz)));
+      }
+  const double hbar = maxdphi / PI / D * 1.01;
+  *hbar_out = hbar;
+  double *psi = (double *)malloc(sizeof(double) * 2 * N);
+  double *gp = (double *)malloc(sizeof(double) * 2 * N);
+  /* psi = exp(i (phi + phi2) / (hbar D)) (:724-734) */
+  for (size_t i = 0; i < nn; ++i)
+    for (size_t j = 0; j < nn; ++j)
+      for (size_t k = 0; k < nn; ++k) {
+        const size_t r = (i * nn + j) * np + k, q = (i * nn + j) * nn + k;
+        const double ph = ((phi[r] + (phi2 ? phi2[r] : 0.0)) / hbar) / D;
+        psi[2 * q] = cos(ph);
+        psi[2 * q + 1] = sin(ph);
+      }
+  /* one drift step: psi_k exp(-i hbar k^2 D / 2) (:739-744) */
+  const double sg = orc_ppt_mutation == 1 ? -1.0 : 1.0;
+  orc_fft_c2c(psi, n, -1);
+  for (size_t i = 0; i < nn; ++i)
+    for (size_t j = 0; j < nn; ++j)
+      for (size_t k = 0; k < nn; ++k) {
+        const double kx = orc_get_k(i, nn, kfac), ky = orc_get_k(j, nn, kfac), kz = orc_get_k(k, nn, kfac);
+        const double a = -sg * 0.5 * hbar * (kx * kx + ky * ky + kz * kz) * D;
+        const size_t q = (i * nn + j) * nn + k;
+        const double re = psi[2 * q], im = psi[2 * q + 1], c = cos(a), s = sin(a);
+        psi[2 * q] = re * c - im * s;
+        psi[2 * q + 1] = re * s + im * c;
+      }
+  orc_fft_c2c(psi, n, +1);
+  /* the second-order phase after the drift (:746-750) */
+  if (phi2)
+    for (size_t i = 0; i < nn; ++i)
+      for (size_t j = 0; j < nn; ++j)
+        for (size_t k = 0; k < nn; ++k) {
+          const size_t q = (i * nn + j) * nn + k;
+          const double ph = (phi2[(i * nn + j) * np + k] / hbar) / D;
+          const double re = psi[2 * q], im = psi[2 * q + 1], c = cos(ph), s = sin(ph);
+          psi[2 * q] = re * c - im * s;
+          psi[2 * q + 1] = re * s + im * c;
+        }
+  /* density |psi|^2 - 1 (:755-758) */
+  for (size_t q = 0; q < N; ++q) delta[q] = psi[2 * q] * psi[2 * q] + psi[2 * q + 1] * psi[2 * q + 1] - 1.0;
+  /* velocities: v_d = vunit Re[(conj(psi) grad psi - psi conj(grad psi)) / (2i/hbar)] / (1 + delta)
+   * with grad psi = IFFT(i k_d FFT(psi)), k from get_k (:764-783) */
+  for (int d = 0; d < 3; ++d) {
+    memcpy(gp, psi, sizeof(double) * 2 * N);
+    orc_fft_c2c(gp, n, -1);
+    for (size_t i = 0; i < nn; ++i)
+      for (size_t j = 0; j < nn; ++j)
+        for (size_t k = 0; k < nn; ++k) {
+          const size_t idx[3] = {i, j, k};
+          const double kd = orc_get_k(idx[d], nn, kfac);
+          const size_t q = (i * nn + j) * nn + k;
+          const double re = gp[2 * q], im = gp[2 * q + 1];
+          gp[2 * q] = -kd * im;
+          gp[2 * q + 1] = kd * re;
+        }
+    orc_fft_c2c(gp, n, +1);
+    for (size_t q = 0; q < N; ++q) {
+      /* conj(psi) g - psi conj(g) = 2 i Im(conj(psi) g); divided by 2i/hbar */
+      const double pr = psi[2 * q], pi = psi[2 * q + 1], gr = gp[2 * q], gi = gp[2 * q + 1];
+      const double imag = pr * gi - pi * gr;
+      v[(size_t)d * N + q] = vunit * (hbar * imag) / (1.0 + delta[q]);
+    }
+  }
+  free(psi);
+  free(gp);
+}
+

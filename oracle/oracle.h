@@ -99,6 +99,12 @@ void orc_set_positions(const double *field, double *pos, int n, int idim, double
 /* Grid_FFT::shift_field (grid_fft.hh:832-842) of a real padded field, in place */
 extern int orc_lattice_mutation;
 void orc_shift_field(double *f, int n, double boxlen, double sx, double sy, double sz);
+/* Grid_FFT<ccomplex_t> transform of an n^3 complex grid, unitary, in place */
+void orc_fft_c2c(double *z, int n, int sign);
+/* semiclassical (PPT) Eulerian density and velocities (ic_generator.cc:674-790) */
+extern int orc_ppt_mutation;
+void orc_ppt_fields(const double *phi, const double *phi2, int n, double boxlen, double D, double vunit,
+                    double *delta, double *v, double *hbar_out);
 /* set_positions with a sublattice offset s (cells) along idim */
 void orc_set_positions_shifted(const double *field, double *pos, int n, int idim, double lunit, double s);
 

@@ -79,6 +79,14 @@ def fft_c2r(a):
     return a
 
 
+def fft_c2c(z, sign):
+    """unitary transform of a complex128 (n, n, n) grid in place: sign -1
+    forward, +1 backward (Grid_FFT<ccomplex_t>)"""
+    assert z.dtype == np.complex128
+    lib().orc_fft_c2c(_p(z), ctypes.c_int(z.shape[0]), ctypes.c_int(sign))
+    return z
+
+
 def dft1d(x, sign):
     x = np.ascontiguousarray(x, dtype=np.complex128)
     out = np.empty_like(x)
@@ -227,6 +235,25 @@ def lattice_values(field, boxlen, lattice, idim=None, lunit=1.0, secondary=False
 def lattice_set_mutation(m):
     """plant a mistake in shift_field (0 = none, 1 = phase sign)"""
     ctypes.c_int.in_dll(lib(), "orc_lattice_mutation").value = int(m)
+
+
+def ppt_fields(phi, phi2, boxlen, D, vunit):
+    """semiclassical Eulerian fields (ic_generator.cc:674-790) from the
+    growth-scaled real-space potentials phi, phi2 (padded (n, n, n+2) float64,
+    phi2 None for LPTorder 1): (delta, v (3, n, n, n), hbar)"""
+    n = phi.shape[0]
+    delta = np.zeros((n, n, n))
+    v = np.zeros((3, n, n, n))
+    hb = ctypes.c_double()
+    p2 = _p(phi2) if phi2 is not None else None
+    lib().orc_ppt_fields(_p(phi), p2, ctypes.c_int(n), ctypes.c_double(boxlen), ctypes.c_double(D),
+                         ctypes.c_double(vunit), _p(delta), _p(v), ctypes.byref(hb))
+    return delta, v, hb.value
+
+
+def ppt_set_mutation(m):
+    """plant a mistake in ppt_fields (0 = none, 1 = drift phase sign)"""
+    ctypes.c_int.in_dll(lib(), "orc_ppt_mutation").value = int(m)
 
 
 def power_spectrum(f, boxlen):
