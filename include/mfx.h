@@ -256,6 +256,25 @@ int mfx_rng_ngenic_fill(mfx_grid *g, long seed);
  * uniform_rand_threefry4x64.c:254-493): first Box-Muller deviate of each
  * cell (l = log2 n, j = (i,j,k)), real space */
 int mfx_rng_threefry_fill(mfx_grid *g, unsigned long long seed);
+/* PANPHASIA with a Panphasia-HO descriptor ([Panph6,L..,(x,y,z),S..,(KK..,)CH..,name]):
+ * RNG_panphasia::Fill_Grid (src/plugins/random_panphasia.cc:131-180) -- the octree
+ * Legendre-block field (external/panphasia_ho/high_order_panphasia_routines.c:73-150,
+ * 888-1240), its spherical-Bessel k-space synthesis on N0 = 2 * size * 2^rel points
+ * (pan_mpi_routines.c:23-460) and Grid_FFT::FourierInterpolateCopyTo into the grid
+ * (grid_fft.cc:274-333); leaves the grid in k space.  Panph1 (Fortran Panphasia)
+ * descriptors and an odd size * 2^rel are errors; a check digit other than -999 must
+ * match (compute_check_digit_, uniform_rand_threefry4x64.c:495-531). */
+int mfx_rng_panphasia_fill(mfx_grid *g, const char *descriptor);
+/* descriptor fields as parsed (parse_and_validate_descriptor_,
+ * high_order_panphasia_routines.c:1242-1342): out[0..6] order, level, x, y, z,
+ * size, KK limit; out[7] the computed check digit; for n > 0 out[8..10] the
+ * relative level, coefficient cube size * 2^rel and Fourier grid N0 used for an
+ * n-grid (random_panphasia.cc:143-150) */
+int mfx_panphasia_info(const char *descriptor, int n, long long out[11]);
+/* test hook: the order-6 basis tables as this library builds them from closed forms
+ * (pan_matrices_order6.h: constraint_matrices, irank_p, list_by_parity); any
+ * pointer may be null.  C is 84 x 84, degrees 3 x 84, row_index 84. */
+int mfx_panphasia_tables(double *C, int *degrees, int *row_index);
 /* raw streams (test hooks): MT19937 words, ranlxd1 uniforms, Threefry cell
  * uniforms (cells = 4 x ncells of (l, j1, j2, j3)); outputs are host arrays */
 int mfx_rng_mt19937_raw(mfx_ctx *c, const unsigned long *seeds, int nseeds, int count, unsigned *out);
@@ -289,7 +308,7 @@ typedef struct {
   double boxlen, zstart;
   int lpt_order;        /* 1, 2 or 3 */
   int precision;        /* bulk FFT/convolution precision: MFX_FLOAT or MFX_DOUBLE */
-  int generator;        /* 0 MUSIC1, 1 NGENIC, 2 Threefry */
+  int generator;        /* 0 MUSIC1, 1 NGENIC, 2 Threefry, 3 PANPHASIA (descriptor) */
   long seed;
   int fixing, inversion;
   double H0, Omega_m, Omega_b, n_s, sigma_8, Tcmb;
@@ -325,6 +344,9 @@ typedef struct {
                            the density |psi|^2 - 1 (DM_delta), arrays 3..5 the velocities
                            (DM_vx..DM_vz); arrays 0..2 are not written.  Not with PLT; grid
                            sizes 16..1024 of the form 2^k or 3*2^k */
+  const char *descriptor; /* generator 3 (PANPHASIA): the Panphasia-HO descriptor; the
+                             field is computed once at creation (it depends on the
+                             descriptor only; seeds do not apply) */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;

@@ -28,7 +28,8 @@ EXPORTS = [
     "mfx_conv_hessians3", "mfx_conv_sum_of_hessians", "mfx_conv_difference_of_hessians",
     "mfx_conv_multiply_field", "mfx_lpt_sources", "mfx_conv_set_lpt_path", "mfx_conv_lpt_path", "mfx_plt_create", "mfx_plt_destroy", "mfx_plt_tables",
     "mfx_plt_dmatrix", "mfx_plt_gradient", "mfx_assemble_kspace_plt", "mfx_assemble_kspace", "mfx_set_particles",
-    "mfx_rng_music1_fill", "mfx_rng_music1_fill_noalloc", "mfx_rng_ngenic_fill", "mfx_rng_threefry_fill", "mfx_rng_mt19937_raw",
+    "mfx_rng_music1_fill", "mfx_rng_music1_fill_noalloc", "mfx_rng_ngenic_fill", "mfx_rng_threefry_fill", "mfx_rng_panphasia_fill",
+    "mfx_panphasia_info", "mfx_panphasia_tables", "mfx_rng_mt19937_raw",
     "mfx_rng_ranlxd1_raw", "mfx_rng_threefry_uniforms", "mfx_cosmo_create", "mfx_cosmo_destroy",
     "mfx_cosmo_query", "mfx_cosmo_amplitude_table", "mfx_icgen_create", "mfx_icgen_destroy", "mfx_icgen_run",
     "mfx_icgen_set_seed", "mfx_icgen_capture", "mfx_icgen_launch",
@@ -58,7 +59,7 @@ class IcgenParams(ctypes.Structure):
                 ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int),
                 ("assemble_each", ctypes.c_int), ("field_output", ctypes.c_int), ("own_particles", ctypes.c_int),
                 ("lattice", ctypes.c_int), ("baryons", ctypes.c_int),
-                ("eulerian", ctypes.c_int)]
+                ("eulerian", ctypes.c_int), ("descriptor", ctypes.c_char_p)]
 
 
 def load():
@@ -121,6 +122,9 @@ def load():
         "mfx_rng_music1_fill_noalloc": [vp, ctypes.c_long, i32],
         "mfx_rng_ngenic_fill": [vp, ctypes.c_long],
         "mfx_rng_threefry_fill": [vp, ctypes.c_ulonglong],
+        "mfx_rng_panphasia_fill": [vp, ctypes.c_char_p],
+        "mfx_panphasia_info": [ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_longlong)],
+        "mfx_panphasia_tables": [vp, vp, vp],
         "mfx_rng_mt19937_raw": [vp, vp, i32, i32, vp],
         "mfx_rng_ranlxd1_raw": [vp, vp, i32, i32, vp],
         "mfx_rng_threefry_uniforms": [vp, vp, i32, ctypes.c_ulonglong, i32, vp],

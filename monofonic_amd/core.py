@@ -353,8 +353,42 @@ class RNG_threefry:
         call("mfx_rng_threefry_fill", g.handle, self.seed)
 
 
-def select_RNG_plugin(name, seed):
+class RNG_panphasia:
+    """PANPHASIA plugin with a Panphasia-HO descriptor (src/plugins/random_panphasia.cc)"""
+
+    def __init__(self, descriptor):
+        self.descriptor = str(descriptor)
+
+    def isMultiscale(self):
+        return True
+
+    def Fill_Grid(self, g):
+        call("mfx_rng_panphasia_fill", g.handle, self.descriptor.encode())
+
+
+def panphasia_info(descriptor, n=0):
+    """parsed descriptor fields, its check digit and, for an n-grid, the
+    relative level, coefficient cube and Fourier grid sizes"""
+    out = (ctypes.c_longlong * 11)()
+    call("mfx_panphasia_info", str(descriptor).encode(), int(n), out)
+    keys = ["order", "level", "x", "y", "z", "size", "kk", "check", "rel_level", "np", "N0"]
+    return dict(zip(keys, list(out)))
+
+
+def panphasia_tables():
+    """the library's order-6 basis tables: (C 84x84, degrees 3x84, row_index 84)"""
+    C = np.zeros((84, 84))
+    deg = np.zeros((3, 84), dtype=np.int32)
+    ri = np.zeros(84, dtype=np.int32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    call("mfx_panphasia_tables", p(C), p(deg), p(ri))
+    return C, deg, ri
+
+
+def select_RNG_plugin(name, seed, descriptor=None):
     """random_plugin.cc:136-191"""
+    if name == "PANPHASIA":
+        return RNG_panphasia(descriptor)
     table = {"MUSIC1": RNG_music, "NGENIC": RNG_ngenic, "PANPHASIA_THREEFRY": RNG_threefry}
     if name not in table:
         raise RuntimeError(f"Unknown random number generator plugin '{name}'")
@@ -510,7 +544,7 @@ class PLT:
 
 
 # ------------------------------------------------------------------ LPT driver (src/ic_generator.cc:90-971)
-GENERATORS = {"MUSIC1": 0, "NGENIC": 1, "PANPHASIA_THREEFRY": 2}
+GENERATORS = {"MUSIC1": 0, "NGENIC": 1, "PANPHASIA_THREEFRY": 2, "PANPHASIA": 3}
 LATTICES = {"sc": 0, "bcc": 1, "fcc": 2, "rsc": 3}  # [setup] ParticleLoad (ic_generator.cc:131-139)
 
 
@@ -522,7 +556,7 @@ class ICGenerator:
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
                  autotune=True, assemble_each=False, field_output=False, own_particles=False, ParticleLoad="sc",
-                 DoBaryons=False, generic_out_eulerian=False):
+                 DoBaryons=False, generic_out_eulerian=False, descriptor=None):
         self.ctx = ctx
         self.n = int(GridRes)
         self.lattice = str(ParticleLoad)
@@ -548,6 +582,9 @@ class ICGenerator:
         p.lattice = LATTICES[self.lattice]
         p.baryons = int(self.baryons)
         p.eulerian = int(self.eulerian)
+        # [random] descriptor of the PANPHASIA plugin (kept alive with the generator)
+        self._descriptor = None if descriptor is None else str(descriptor).encode()
+        p.descriptor = self._descriptor
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 

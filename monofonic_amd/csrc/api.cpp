@@ -1028,6 +1028,48 @@ int mfx_rng_threefry_fill(mfx_grid* g, unsigned long long seed) {
   });
 }
 
+int mfx_rng_panphasia_fill(mfx_grid* g, const char* descriptor) {
+  return guard([&] {
+    require_grid(g);
+    require_cubic(g);
+    PanDesc d;
+    pan_parse(descriptor, d);
+    const int n = (int)g->n[0];
+    if (g->prec == MFX_FLOAT)
+      panphasia_noise<float>(g->ctx->c, d, n, (cx<float>*)g->d);
+    else
+      panphasia_noise<double>(g->ctx->c, d, n, (cx<double>*)g->d);
+    g->space = MFX_KSPACE;
+  });
+}
+
+int mfx_panphasia_info(const char* descriptor, int n, long long out[11]) {
+  return guard([&] {
+    MFX_REQUIRE(out != nullptr, "null output array");
+    PanDesc d;
+    pan_parse(descriptor, d);
+    const long long f[8] = {(long long)d.order, (long long)d.level, (long long)d.x, (long long)d.y,
+                            (long long)d.z, (long long)d.size, (long long)d.kk, pan_check_digit(d)};
+    for (int q = 0; q < 8; ++q) out[q] = f[q];
+    out[8] = out[9] = out[10] = 0;
+    if (n > 0) {
+      const PanSizes z = pan_sizes(d, n);
+      out[8] = z.rel;
+      out[9] = z.np;
+      out[10] = z.N0;
+    }
+  });
+}
+
+int mfx_panphasia_tables(double* C, int* degrees, int* row_index) {
+  return guard([&] {
+    const PanTables& t = pan_tables();
+    if (C) std::memcpy(C, &t.C[0][0], sizeof(double) * 84 * 84);
+    if (degrees) std::memcpy(degrees, &t.p[0][0], sizeof(int) * 3 * 84);
+    if (row_index) std::memcpy(row_index, t.row_index, sizeof(int) * 84);
+  });
+}
+
 int mfx_rng_mt19937_raw(mfx_ctx* c, const unsigned long* seeds, int nseeds, int count, unsigned* out) {
   return guard([&] {
     MFX_REQUIRE(c && seeds && out && nseeds > 0 && count > 0, "invalid argument");
@@ -1388,6 +1430,10 @@ struct mfx_icgen {
   // complex each) and the device scalars (max |dphi| bits, hbar)
   void* ppt_buf = nullptr;
   double* ppt_scal = nullptr;
+  // Panphasia-HO: the descriptor's k-space noise of this grid, computed at
+  // creation (the field depends on the descriptor only)
+  void* pan_noise = nullptr;
+  std::string descriptor;
 };
 
 static void icgen_free(mfx_icgen* ic) {
@@ -1412,6 +1458,7 @@ static void icgen_free(mfx_icgen* ic) {
   if (ic->delta) ic->ctx->c.release(ic->delta);
   if (ic->ppt_buf) ic->ctx->c.release(ic->ppt_buf);
   if (ic->ppt_scal) ic->ctx->c.release(ic->ppt_scal);
+  if (ic->pan_noise) ic->ctx->c.release(ic->pan_noise);
   if (ic->copy_stream) {
     (void)hipStreamSynchronize(ic->copy_stream);
     (void)hipStreamDestroy(ic->copy_stream);
@@ -1435,7 +1482,11 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
     MFX_REQUIRE(!(p->n % 2 != 0 && p->lpt_order > 1), "Orszag convolver for LPTorder>1 requires even grid size!");
     MFX_REQUIRE(!(p->n % 4 != 0 && p->lpt_order > 1),
                 "LPTorder>1 needs a grid size divisible by 4 (padded z rows of even length 3n/2)");
-    MFX_REQUIRE(p->generator >= 0 && p->generator <= 2, "generator must be 0 (MUSIC1), 1 (NGENIC) or 2 (Threefry)");
+    MFX_REQUIRE(p->generator >= 0 && p->generator <= 3,
+                "generator must be 0 (MUSIC1), 1 (NGENIC), 2 (Threefry) or 3 (PANPHASIA)");
+    // random_panphasia.cc:79-84
+    MFX_REQUIRE(p->generator != 3 || !p->fixing, "PANPHASIA: incompatible parameter (DoFixing)");
+    MFX_REQUIRE(p->generator != 3 || p->descriptor != nullptr, "PANPHASIA: no descriptor");
     // ParticleLoad sc/bcc/fcc/rsc (ic_generator.cc:131-139)
     MFX_REQUIRE(p->lattice >= 0 && p->lattice <= 3, "lattice must be 0 (sc), 1 (bcc), 2 (fcc) or 3 (rsc)");
     MFX_REQUIRE(p->lattice == 0 || !p->use_plt, "PLT corrections are built for the sc lattice only");
@@ -1524,6 +1575,18 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       const size_t table_bytes = p->generator == 1 ? sizeof(unsigned) * p->n * p->n : 0;
       if (p->generator == 1) ic->dev_table = (unsigned*)c->c.alloc(table_bytes);
       if (p->generator == 0) ic->rng_scratch = c->c.alloc(music1_scratch_bytes((int)p->n, 32));
+      if (p->generator == 3) {
+        PanDesc d;
+        pan_parse(p->descriptor, d);
+        ic->descriptor = p->descriptor;
+        ic->p.descriptor = ic->descriptor.c_str();
+        const size_t nk = p->n * p->n * (p->n / 2 + 1);
+        ic->pan_noise = c->c.alloc(nk * 2 * (size_t)p->precision);
+        if (p->precision == MFX_FLOAT)
+          panphasia_noise<float>(c->c, d, (int)p->n, (cx<float>*)ic->pan_noise);
+        else
+          panphasia_noise<double>(c->c, d, (int)p->n, (cx<double>*)ic->pan_noise);
+      }
       for (int q = 0; q < 2; ++q) {
         MFX_HIP(hipHostMalloc(&ic->host_stage[q], 16 + table_bytes, hipHostMallocDefault));
         MFX_HIP(hipEventCreateWithFlags(&ic->staged[q], hipEventDisableTiming));
@@ -1661,13 +1724,19 @@ static void icgen_device_run(mfx_icgen* ic) {
   const double g1 = -D, g2 = -3.0 / 7.0 * D * D, g3 = 1.0 / 3.0 * D * D * D, g3c = 1.0 / 7.0 * D * D * D;
   const double volfac = std::pow(p.boxlen / (double)p.n / 2.0 / M_PI, 1.5);
   // white noise (ic_generator.cc:249-257)
-  if (p.generator == 0)
+  if (p.generator == 0) {
     // the mean of the noise only reaches its k = 0 mode, which phi(1) zeroes
     music1_dev(wn, (const long*)ic->dev_seed, 32, ic->rng_scratch, 2);
-  else if (p.generator == 1)
+  } else if (p.generator == 1) {
     ngenic_dev(wn, ic->dev_table);
-  else
+  } else if (p.generator == 2) {
     threefry_dev(wn, (const unsigned long long*)ic->dev_seed);
+  } else {
+    // Panphasia: the descriptor's field, k space (random_panphasia.cc:131-180)
+    MFX_HIP(hipMemcpyAsync(wn->d, ic->pan_noise, p.n * p.n * (p.n / 2 + 1) * 2 * (size_t)p.precision,
+                           hipMemcpyDeviceToDevice, ic->ctx->c.stream));
+    wn->space = MFX_KSPACE;
+  }
   // normalisation (341-346) applied as phi(1) (450-457) reads the noise:
   // for noise filled in real space, by the last pass of its forward
   // transform (the noise grid is then scratch); NGENIC fills k space

@@ -157,6 +157,32 @@ void ppt_fields(Ctx& c, const T* phi, const T* phi2, int n, double boxlen, doubl
 size_t ppt_scalar_bytes();
 bool ppt_supported(int n);
 
+// Panphasia-HO (external/panphasia_ho): a parsed descriptor, the field sizes
+// for an n-grid, the order-6 basis tables, and the k-space white noise of an
+// n-grid (n^2 (n/2+1) complex, Grid_FFT k-space layout)
+struct PanDesc {
+  unsigned long long order = 0, level = 0, x = 0, y = 0, z = 0, size = 0, kk = 0;
+  long long check = -999;
+  std::string name;
+};
+struct PanSizes {
+  int rel = 0, lmax = 0;
+  long long np = 0, N0 = 0;  // coefficient cube size * 2^rel; Fourier grid 2 np
+};
+struct PanTables {
+  int p[3][84];                    // Legendre degrees of basis function l per axis
+  int row_index[84], row_class[84];  // constraint row r: parent function, parity class
+  int class_begin[9];
+  double C[84][84];                // constraint rows on a parity block of the children
+};
+void pan_parse(const char* s, PanDesc& d);
+long long pan_check_digit(const PanDesc& d);
+const PanTables& pan_tables();
+double pan_sph_bessel(int l, double x);
+PanSizes pan_sizes(const PanDesc& d, int n);
+template <typename T>
+void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn);
+
 // Orszag 3/2 product of two n-grid k-space fields with multipliers, written
 // through op into out (k space n-grid).  ws must hold conv_workspace_bytes.
 template <typename T>

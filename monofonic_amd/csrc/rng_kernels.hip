@@ -21,6 +21,7 @@
 // The acceptance tests compare rounded FP64 expressions; contraction into FMA
 // would change their rounding, so it is disabled in this file.
 #include "ops.hpp"
+#include "threefry.hpp"
 
 #include <cmath>
 #include <mutex>
@@ -334,90 +335,7 @@ void mt19937_uniforms(Ctx& c, unsigned* out, const unsigned long* seeds, int nse
   check_launch("mt19937_raw");
 }
 
-// ------------------------------------------------------------------ Threefry-4x64-20
-__device__ __forceinline__ unsigned long long rotl64(unsigned long long x, int r) {
-  return (x << r) | (x >> (64 - r));
-}
-
-__device__ void threefry4x64_20(const unsigned long long* ctr, unsigned long long* X) {
-  // key = 0 (uniform_rand_threefry4x64.c:48): ks = {0,0,0,0, parity}
-  const unsigned long long ks[5] = {0ull, 0ull, 0ull, 0ull, 0x1BD11BDAA9FC1A22ull};
-  const int R[8][2] = {{14, 16}, {52, 57}, {23, 40}, {5, 37}, {25, 33}, {46, 12}, {58, 22}, {32, 32}};
-#pragma unroll
-  for (int i = 0; i < 4; ++i) X[i] = ctr[i] + ks[i];
-#pragma unroll
-  for (int r = 0; r < 20; ++r) {
-    if ((r & 1) == 0) {
-      X[0] += X[1];
-      X[1] = rotl64(X[1], R[r % 8][0]);
-      X[1] ^= X[0];
-      X[2] += X[3];
-      X[3] = rotl64(X[3], R[r % 8][1]);
-      X[3] ^= X[2];
-    } else {
-      X[0] += X[3];
-      X[3] = rotl64(X[3], R[r % 8][0]);
-      X[3] ^= X[0];
-      X[2] += X[1];
-      X[1] = rotl64(X[1], R[r % 8][1]);
-      X[1] ^= X[2];
-    }
-    if ((r & 3) == 3) {
-      const unsigned long long s = (unsigned long long)((r + 1) / 4);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) X[i] += ks[(s + i) % 5];
-      X[3] += s;
-    }
-  }
-}
-
-// first `count` uniforms of a cell (uniform_rand_threefry4x64.c:336-489),
-// p_order = 6 and Nbasis = 84 (pan_matrices_order6.h:4-5)
-__device__ void pan_uniforms(unsigned long long l, unsigned long long j1, unsigned long long j2,
-                             unsigned long long j3, unsigned long long seed, double* u, int count) {
-  const unsigned long long p_order = 6, Nbasis = 84;
-  const double g_scale = 1.0 / 4294967296.0, g_shift = 0.5;
-  const unsigned long long j0 = (p_order << 60) + ((l << 56) >> 4) + ((seed << 32) >> 12);
-  const double br = ((double)4096) * g_scale;
-  for (int blk = 0; blk * 8 < count; ++blk) {
-    unsigned long long ctr[4] = {j0 + (unsigned long long)blk, j1, j2, j3}, X[4];
-    threefry4x64_20(ctr, X);
-    for (int w = 0; w < 8 && blk * 8 + w < count; ++w) {
-      unsigned int oi = (w & 1) ? (unsigned int)((X[w >> 1] << 32) >> 32) : (unsigned int)(X[w >> 1] >> 32);
-      u[blk * 8 + w] = ((double)oi + g_shift) * g_scale;
-    }
-  }
-  for (int i = 0; i < count; i += 2) {
-    if (u[i] < br) {
-      double rep = br;
-      unsigned long long iind = (unsigned long long)i / 8 + 1 * (Nbasis + (unsigned long long)i);
-      int jind = i % 8;
-      unsigned long long ctr[4] = {j0 + iind, j1, j2, j3}, X[4];
-      threefry4x64_20(ctr, X);
-      unsigned int oi[8];
-#pragma unroll
-      for (int w = 0; w < 4; ++w) {
-        oi[2 * w] = (unsigned int)(X[w] >> 32);
-        oi[2 * w + 1] = (unsigned int)((X[w] << 32) >> 32);
-      }
-      double nv = ((double)oi[jind] + g_shift) * g_scale;
-      if (nv >= br) {
-        rep *= nv;
-      } else {
-        int counter = 0;
-        while (nv < br && counter < 7) {
-          rep *= br;
-          counter++;
-          jind = (jind + 1) % 8;
-          nv = ((double)oi[jind] + g_shift) * g_scale;
-        }
-        rep *= nv;
-      }
-      u[i] = rep;
-    }
-  }
-}
-
+// ------------------------------------------------------------------ Threefry-4x64-20 (threefry.hpp)
 template <typename T>
 __global__ void k_threefry_fill(T* data, int n, unsigned long long l, const unsigned long long* seedp) {
   const unsigned long long seed = *seedp;

@@ -211,4 +211,16 @@ void orc_assemble_component_plt(const double *phi, const double *phi2, const dou
 #ifdef __cplusplus
 }
 #endif
+
+/* Panphasia-HO (panphasia.c) */
+extern int orc_pan_mutation;
+void orc_pan_tables(double *C, int *deg, int *rowidx);
+void orc_pan_solve(const double *parent, const double *children_in, double *out, int constrained);
+void orc_pan_cell_gauss(size_t level, size_t j1, size_t j2, size_t j3, double *g);
+void orc_pan_root(double *root);
+void orc_pan_coefficients(uint64_t ox, uint64_t oy, uint64_t oz, int base_level, int rel, size_t np, double *coef);
+double orc_sph_bessel(int l, double x);
+void orc_panphasia_noise(uint64_t ox, uint64_t oy, uint64_t oz, int base_level, uint64_t size, uint64_t kk, int n,
+                         double *out);
+
 #endif

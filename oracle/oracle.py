@@ -256,6 +256,65 @@ def ppt_set_mutation(m):
     ctypes.c_int.in_dll(lib(), "orc_ppt_mutation").value = int(m)
 
 
+# ---------------------------------------------------------------- Panphasia-HO
+def pan_tables():
+    """(C 84x84, degrees 3x84, row_index 84) of the order-6 basis"""
+    C = np.zeros((84, 84))
+    deg = np.zeros((3, 84), dtype=np.int32)
+    ri = np.zeros(84, dtype=np.int32)
+    lib().orc_pan_tables(_p(C), _p(deg), _p(ri))
+    return C, deg, ri
+
+
+def pan_solve(parent, gauss, constrained=True):
+    """solve_panphasia_cell_: 8 x 84 child coefficients (child 4ix+2iy+iz)"""
+    parent = np.ascontiguousarray(parent, dtype=np.float64)
+    g = np.ascontiguousarray(gauss, dtype=np.float64).reshape(672)
+    out = np.zeros(672)
+    lib().orc_pan_solve(_p(parent), _p(g), _p(out), ctypes.c_int(1 if constrained else 0))
+    return out.reshape(8, 84)
+
+
+def pan_cell_gauss(level, j1, j2, j3):
+    g = np.zeros(672)
+    lib().orc_pan_cell_gauss(ctypes.c_size_t(level), ctypes.c_size_t(j1), ctypes.c_size_t(j2), ctypes.c_size_t(j3),
+                             _p(g))
+    return g
+
+
+def pan_root():
+    r = np.zeros(84)
+    lib().orc_pan_root(_p(r))
+    return r
+
+
+def pan_coefficients(origin, base_level, rel, np_):
+    """PANPHASIA_compute_coefficients_ of the cube: (np, np, np, 84)"""
+    coef = np.zeros(np_ * np_ * (np_ + 2) * 84)
+    lib().orc_pan_coefficients(ctypes.c_uint64(origin[0]), ctypes.c_uint64(origin[1]), ctypes.c_uint64(origin[2]),
+                               ctypes.c_int(base_level), ctypes.c_int(rel), ctypes.c_size_t(np_), _p(coef))
+    return coef.reshape(np_, np_, np_ + 2, 84)[:, :, :np_, :]
+
+
+def sph_bessel(l, x):
+    lib().orc_sph_bessel.restype = ctypes.c_double
+    return lib().orc_sph_bessel(ctypes.c_int(l), ctypes.c_double(x))
+
+
+def panphasia_noise(origin, base_level, size, kk, n):
+    """k-space Panphasia-HO noise of an n-grid, complex (n, n, n//2+1)"""
+    out = np.zeros(n * n * (n // 2 + 1) * 2)
+    lib().orc_panphasia_noise(ctypes.c_uint64(origin[0]), ctypes.c_uint64(origin[1]), ctypes.c_uint64(origin[2]),
+                              ctypes.c_int(base_level), ctypes.c_uint64(size), ctypes.c_uint64(kk), ctypes.c_int(n),
+                              _p(out))
+    return out.view(np.complex128).reshape(n, n, n // 2 + 1)
+
+
+def pan_set_mutation(m):
+    """plant a mistake in the cell solve (0 = none, 1 = parity transform sign)"""
+    ctypes.c_int.in_dll(lib(), "orc_pan_mutation").value = int(m)
+
+
 def power_spectrum(f, boxlen):
     n = f.shape[0]
     mb = n

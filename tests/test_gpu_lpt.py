@@ -71,10 +71,17 @@ def test_lpt_sequence_vs_oracle(ctx, orc, n, order):
             assert rel_max(A3[d].kdata(), orc.kview(rA[d])) < 1e-10
 
 
-def oracle_noise(orc, n, seed, generator="MUSIC1"):
+def oracle_noise(orc, n, seed, generator="MUSIC1", descriptor=None):
     """the white-noise grid after Fill_Grid and wnoise.FourierTransformForward
-    (ic_generator.cc:255-257): NGENIC fills k space directly, so its forward
-    transform is a no-op"""
+    (ic_generator.cc:255-257): NGENIC and PANPHASIA fill k space directly, so
+    the forward transform is a no-op"""
+    if generator == "PANPHASIA":
+        import re
+        m = re.match(r"\[Panph6,L(\d+),\((\d+),(\d+),(\d+)\),S(\d+),(?:KK(\d+),)?CH", descriptor)
+        lev, x, y, z, size, kk = m.groups()
+        wn = np.zeros((n, n, n + 2))
+        orc.kview(wn)[:] = orc.panphasia_noise((int(x), int(y), int(z)), int(lev), int(size), int(kk or 0), n)
+        return wn
     if generator == "NGENIC":
         return orc.ngenic_fill(n, orc.ngenic_seedtable(n, seed))
     wn = orc.music1_fill(n, seed) if generator == "MUSIC1" else orc.threefry_fill(n, seed)
@@ -83,7 +90,7 @@ def oracle_noise(orc, n, seed, generator="MUSIC1"):
 
 
 def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, generator="MUSIC1", lattice="sc",
-                    secondary=False):
+                    secondary=False, descriptor=None):
     """ic_generator::run restated on the CPU: white noise (MUSIC1, NGENIC or
     the Threefry stream), phi(1..3), A(3), growth scaling and FP64
     positions/velocities of the sc lattice, or, for lattice bcc/fcc/rsc,
@@ -94,7 +101,7 @@ def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, gen
     D = cos.Dplus(a)
     vfac = cos.vfact(a)
     volfac = (boxlen / n / 2.0 / np.pi) ** 1.5
-    wn = oracle_noise(orc, n, seed, generator)
+    wn = oracle_noise(orc, n, seed, generator, descriptor)
     orc.noise_normalise(wn, volfac)
     phi = np.zeros_like(wn)
     orc.phi1(phi, wn, boxlen, cos.amplitude_table(n, boxlen))
