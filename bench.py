@@ -31,6 +31,15 @@ Also reported on the same line:
 --config batched reports the 48 uneven 2LPT realisations (16 each at 96^3,
 128^3, 192^3) run sequentially, as per-generator graphs on streams, and as
 one captured graph holding all 48.
+
+--config eulerian times the semiclassical (PPT) Eulerian fields of a 2LPT
+512^3 FP32 realisation (generic_out_eulerian: density and velocities),
+replayed as a graph, with the PPT passes' kernel times and the FP64 oracle's
+PPT step on a 64^3 sample as the CPU baseline.
+
+--config panphasia times RNG_panphasia::Fill_Grid of a Panphasia-HO
+descriptor at 512^3 (a 256^3 coefficient cube, 2.4e7 octree cells), with the
+oracle's field at 64^3 as the CPU baseline.
 """
 import argparse
 import json
@@ -218,6 +227,93 @@ def accuracy_vs_oracle(M, ctx, n, fields):
     return out
 
 
+def run_eulerian(M, n=512, steps=3, warmup=1):
+    """2LPT FP32 realisations with the semiclassical Eulerian fields"""
+    import numpy as np
+    ctx = M.Context(0)
+    ic = M.ICGenerator(ctx, n, 1000.0 * n / 512, 24.0, LPTorder=2, precision="float", generator="MUSIC1",
+                       seed=12345, field_output=True, generic_out_eulerian=True)
+    ic.capture()
+    for _ in range(warmup):
+        ic.launch()
+    ctx.synchronize()
+    ctx.record(0)
+    for s in range(steps):
+        ic.launch()
+        ctx.synchronize()
+        log(f"[bench] eulerian step {s + 1}/{steps} done")
+    ctx.record(1)
+    ctx.synchronize()
+    ms = ctx.elapsed_ms(0, 1) / steps
+    ctx.profile_reset()
+    ctx.profile(True)
+    ic.run()
+    ctx.synchronize()
+    stats = ctx.profile_stats()
+    ctx.profile(False)
+    ppt = {k: v for k, v in stats.items() if k.startswith("ppt")}
+    ppt_ms = sum(v["ms"] for v in ppt.values())
+    roof, _ = roofline(ppt)
+    delta = ic.density()
+    # CPU baseline: the oracle's PPT step (transforms, drift, density,
+    # velocities) on a 64^3 sample of growth-scaled potentials
+    from oracle import oracle as O
+    m = 64
+    rng = np.random.default_rng(1)
+    phi = np.zeros((m, m, m + 2))
+    phi[:, :, :m] = rng.standard_normal((m, m, m))
+    t0 = time.perf_counter()
+    O.ppt_fields(phi, phi.copy(), 100.0, 0.04, 1.0)
+    dt = time.perf_counter() - t0
+    return {"metric": "semiclassical Eulerian fields: realisations/s", "value": round(1e3 / ms, 3),
+            "unit": "realisations/s", "ms_per_step": round(ms, 3), "steps": steps, "warmup": warmup,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"2LPT {n}^3 FP32 MUSIC1, generic_out_eulerian (PPT density + velocities)",
+                       "n": n, "hbar": ic.hbar(), "delta_std": float(np.std(delta))},
+            "ppt_ms": round(ppt_ms, 3), "roofline": roof,
+            "cpu_baseline": {"value": round(m ** 3 / dt, 1), "unit": "cells/s (PPT step only)", "cores":
+                             int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+                             "sample": f"oracle PPT fields at {m}^3, {dt:.2f} s"},
+            "gpu_ppt_cells_per_s": round(n ** 3 / (ppt_ms * 1e-3), 1),
+            "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                            "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                        for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])}}
+
+
+PAN_DESCRIPTOR = "[Panph6,L20,(424060,82570,148256),S1,KK0,CH-999,Auriga_100_vol2]"
+
+
+def run_panphasia(M, n=512, steps=3, warmup=1):
+    """RNG_panphasia::Fill_Grid of a Panphasia-HO descriptor"""
+    ctx = M.Context(0)
+    g = M.Grid_FFT(ctx, n, 1000.0, "float")
+    rng = M.RNG_panphasia(PAN_DESCRIPTOR)
+    for _ in range(warmup):
+        rng.Fill_Grid(g)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        rng.Fill_Grid(g)
+        ctx.synchronize()
+        log(f"[bench] panphasia fill {s + 1}/{steps} done")
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    info = M.panphasia_info(PAN_DESCRIPTOR, n)
+    from oracle import oracle as O
+    m = 64
+    mi = M.panphasia_info(PAN_DESCRIPTOR, m)
+    t1 = time.perf_counter()
+    O.panphasia_noise((mi["x"], mi["y"], mi["z"]), mi["level"], mi["size"], mi["kk"], m)
+    dt = time.perf_counter() - t1
+    return {"metric": "Panphasia-HO white noise: grid points/s", "value": round(n ** 3 / (ms * 1e-3), 1),
+            "unit": "points/s", "ms_per_step": round(ms, 3), "steps": steps, "warmup": warmup,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Fill_Grid {n}^3, descriptor {PAN_DESCRIPTOR}", "n": n,
+                       "coefficient_cube": info["np"], "fourier_grid": info["N0"], "relative_level": info["rel_level"]},
+            "cpu_baseline": {"value": round(m ** 3 / dt, 1), "unit": "points/s", "kind": "port",
+                             "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                             "sample": f"oracle Panphasia-HO field at {m}^3 ({mi['np']}^3 cube), {dt:.2f} s"}}
+
+
 BATCH_CLASSES = [(96, 150.0), (128, 200.0), (192, 300.0)]  # (grid, box Mpc/h), 16 realisations each
 
 
@@ -350,8 +446,9 @@ def main():
                     help="white-noise plugin (the 1024^3 configuration uses PANPHASIA_THREEFRY)")
     ap.add_argument("--plt", action="store_true", help="PLT-corrected assembly (the 1024^3 configuration)")
     ap.add_argument("--box", type=float, default=None, help="box length in Mpc/h (default 1000 * n / 512)")
-    ap.add_argument("--config", default="c3", choices=["c3", "batched"],
-                    help="c3: the 512^3 3LPT headline line; batched: the 48 uneven 2LPT realisations")
+    ap.add_argument("--config", default="c3", choices=["c3", "batched", "eulerian", "panphasia"],
+                    help="c3: the 512^3 3LPT headline line; batched: the 48 uneven 2LPT realisations; "
+                         "eulerian: semiclassical fields at 512^3; panphasia: Panphasia-HO noise at 512^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -361,6 +458,12 @@ def main():
     import monofonic_amd as M
     if args.config == "batched":
         print(json.dumps(run_batched(M)))
+        return
+    if args.config == "eulerian":
+        print(json.dumps(run_eulerian(M, steps=args.steps, warmup=args.warmup)))
+        return
+    if args.config == "panphasia":
+        print(json.dumps(run_panphasia(M, steps=args.steps, warmup=args.warmup)))
         return
     ctx = M.Context(0)
     n = args.n
