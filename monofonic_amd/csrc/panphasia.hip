@@ -361,79 +361,92 @@ struct PanLevelArgs {
   int mode;  // 0 inner level, 1 last level (children into the coefficient fields), 2 root cell
 };
 
-// One parent cell per workgroup: its eight children's 84 coefficients
+// Three parent cells per workgroup of 256 lanes (the per-cell work comes in
+// multiples of 84: 84 Threefry blocks, 84 constraint rows, 672 child
+// coefficients): each cell's eight children's 84 coefficients
 // (compute_all_properties_of_a_panphasia_cell_, high_order_panphasia_routines.c
 // :291-330; solve_panphasia_cell_ :73-150).  The root's values come from the
 // stream at level 0, cell (2, 2, 2), without the parent constraint, child 0
 // (uniform_rand_threefry4x64.c:172-216,313-330).
-__global__ void __launch_bounds__(128) k_pan_cells(PanLevelArgs a) {
-  __shared__ double g[672];
-  __shared__ double diff[84];
+constexpr int kPanCellsPerGroup = 3;
+
+__global__ void __launch_bounds__(256) k_pan_cells(PanLevelArgs a) {
+  constexpr int CPG = kPanCellsPerGroup;
+  __shared__ double g[CPG][672];
+  __shared__ double diff[CPG][84];
   const int t = threadIdx.x;
-  const long long cell = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+  const long long grp = (long long)blockIdx.y * gridDim.x + blockIdx.x;
   const long long ncell = (long long)a.cnt[0] * a.cnt[1] * a.cnt[2];
-  if (cell >= ncell) return;  // uniform per workgroup
-  const int cz = (int)(cell % a.cnt[2]), cy = (int)((cell / a.cnt[2]) % a.cnt[1]),
-            cx = (int)(cell / ((long long)a.cnt[2] * a.cnt[1]));
   const bool root = a.mode == 2;
-  unsigned long long j1 = 2, j2 = 2, j3 = 2;
-  if (!root) {
-    j1 = a.coord[0][cx];
-    j2 = a.coord[1][cy];
-    j3 = a.coord[2][cz];
-  }
-  const unsigned long long j0 = pan_j0(root ? 0ull : a.level, 0ull);
-  // 84 Threefry blocks of 8 uniforms, tail replacement, Box-Muller pairs
-  if (t < 84) {
-    unsigned long long ctr[4] = {j0 + (unsigned long long)t, j1, j2, j3}, X[4];
+  const int* row_index = a.tabi;
+  const int* row_class = a.tabi + 84;
+  const int* class_begin = a.tabi + 168;
+  const int* pbits = a.tabi + 177;
+  // lane t < 252 owns block / row t % 84 of cell slot t / 84
+  const int slot = t / 84, tl = t % 84;
+  const long long cell = grp * CPG + slot;
+  const bool mine = t < 84 * CPG && cell < ncell;
+  // 84 Threefry blocks of 8 uniforms per cell, tail replacement, Box-Muller pairs
+  if (mine) {
+    unsigned long long j1 = 2, j2 = 2, j3 = 2;
+    if (!root) {
+      const int cz = (int)(cell % a.cnt[2]), cy = (int)((cell / a.cnt[2]) % a.cnt[1]),
+                cx = (int)(cell / ((long long)a.cnt[2] * a.cnt[1]));
+      j1 = a.coord[0][cx];
+      j2 = a.coord[1][cy];
+      j3 = a.coord[2][cz];
+    }
+    const unsigned long long j0 = pan_j0(root ? 0ull : a.level, 0ull);
+    unsigned long long ctr[4] = {j0 + (unsigned long long)tl, j1, j2, j3}, X[4];
     threefry4x64_20(ctr, X);
     double u[8];
 #pragma unroll
     for (int w = 0; w < 8; ++w) u[w] = pan_unif(X, w);
     for (int w = 0; w < 8; w += 2)
-      if (u[w] < kPanBranch) u[w] = pan_tail(j0, j1, j2, j3, 8 * t + w);
+      if (u[w] < kPanBranch) u[w] = pan_tail(j0, j1, j2, j3, 8 * tl + w);
     const double pi = 4.0 * atan(1.0);
     const double two_pi = 2.0 * pi;
 #pragma unroll
     for (int w = 0; w < 8; w += 2) {
       const double radius = sqrt(-2.0 * log(u[w]));
       const double angle = two_pi * u[w + 1];
-      g[8 * t + w] = radius * cos(angle);
-      g[8 * t + w + 1] = radius * sin(angle);
+      g[slot][8 * tl + w] = radius * cos(angle);
+      g[slot][8 * tl + w + 1] = radius * sin(angle);
     }
   }
   __syncthreads();
-  const int* row_index = a.tabi;
-  const int* row_class = a.tabi + 84;
-  const int* class_begin = a.tabi + 168;
-  const int* pbits = a.tabi + 177;
   if (!root) {
     // projection of each parity block on its constraint rows
-    if (t < 84) {
+    if (mine) {
       const double* par = a.parent + cell * 84;
-      const int cl = row_class[t];
+      const int cl = row_class[tl];
       double e = 0.0;
-      for (int i = 0; i < 84; ++i) e += a.CT[i * 84 + t] * g[84 * cl + i];
-      diff[t] = par[row_index[t]] - e;
+      for (int i = 0; i < 84; ++i) e += a.CT[i * 84 + tl] * g[slot][84 * cl + i];
+      diff[slot][tl] = par[row_index[tl]] - e;
     }
     __syncthreads();
     // each block moved onto its parent constraints, rows in order
-    for (int q = t; q < 672; q += 128) {
-      const int cl = q / 84, i = q % 84;
-      double v = g[q];
-      for (int r = class_begin[cl]; r < class_begin[cl + 1]; ++r) v = v + a.C[r * 84 + i] * diff[r];
-      g[q] = v;
+    for (int q = t; q < 672 * CPG; q += 256) {
+      const int s2 = q / 672, ql = q % 672;
+      if (grp * CPG + s2 >= ncell) continue;
+      const int cl = ql / 84, i = ql % 84;
+      double v = g[s2][ql];
+      for (int r = class_begin[cl]; r < class_begin[cl + 1]; ++r) v = v + a.C[r * 84 + i] * diff[s2][r];
+      g[s2][ql] = v;
     }
     __syncthreads();
   }
   // parity blocks -> children: child c, coefficient l = sqrt(1/8) sum_j
   // (-1)^popcount(c & (j ^ parity(l))) block_j[l]
   const double nrm = sqrt(0.125);
-  for (int q = t; q < (root ? 84 : 672); q += 128) {
-    const int c = q / 84, l = q % 84, pb = pbits[l];
+  for (int q = t; q < (root ? 84 : 672 * CPG); q += 256) {
+    const int s2 = q / 672, ql = q % 672;
+    const long long cl2 = grp * CPG + s2;
+    if (cl2 >= ncell) continue;
+    const int c = ql / 84, l = ql % 84, pb = pbits[l];
     double e = 0.0;
     for (int j = 0; j < 8; ++j) {
-      const double v = g[84 * j + l];
+      const double v = g[s2][84 * j + l];
       e += (__popc((unsigned)(c & (j ^ pb))) & 1) ? -v : v;
     }
     const double val = nrm * e;
@@ -441,6 +454,8 @@ __global__ void __launch_bounds__(128) k_pan_cells(PanLevelArgs a) {
       a.out[l] = val;
       continue;
     }
+    const int cz = (int)(cl2 % a.cnt[2]), cy = (int)((cl2 / a.cnt[2]) % a.cnt[1]),
+              cx = (int)(cl2 / ((long long)a.cnt[2] * a.cnt[1]));
     const int ix = (c >> 2) & 1, iy = (c >> 1) & 1, iz = c & 1;
     const int chx = a.child[0][2 * cx + ix], chy = a.child[1][2 * cy + iy], chz = a.child[2][2 * cz + iz];
     if (chx < 0 || chy < 0 || chz < 0) continue;
@@ -457,59 +472,84 @@ __global__ void __launch_bounds__(128) k_pan_cells(PanLevelArgs a) {
 // The n-grid's k-space noise (FourierInterpolateCopyTo of the N0 field,
 // grid_fft.cc:296-333: modes with every index below n/2 in magnitude; the
 // Nyquist planes stay zero) from the 84 transformed coefficient fields F_l
-// (np^3 r2c: the N0 grid's transform is F_l at k mod np):
+// (np^3 r2c: the N0 = 2 np grid's transform is F_l at k mod np):
 //   f(k) = sum_l B_px(kx) B_py(ky) B_pz(kz) F_l(k),  W(k) = sum_l |B B B|^2
 // times sqrt(fdim^3) exp(-i fdim pi (kx+ky+kz) / sqrt(W) / N0) / N0^1.5
 // (pan_mpi_routines.c:319-402; the angle is divided by sqrt(W) there), the
 // modes 0 < k^2 <= KK limit scaled to unit modulus (:406-436).
+// One thread per coefficient mode (a, b) and output plane k < n/2 forms the
+// up to four N0 modes (a or a+np, b or b+np, k) that alias to it, so each
+// F_l value is read once for all of them; the per-mode sums over l run in
+// the reference's order.  The grid is zeroed beforehand (Nyquist planes).
+__device__ __forceinline__ int pan_out_index(int x, int n, int N0) {
+  return x < n / 2 ? x : (x > N0 - n / 2 ? x - N0 + n : -1);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_pan_combine(cx<T>* wn, int n, int N0, int np, const cx<double>* __restrict__ F, long long fcs,
                   const cx<double>* __restrict__ sph, const int* __restrict__ prank, unsigned long long kk, int fdim) {
-  const int h = n / 2 + 1;
-  const long long nm = (long long)n * n * h;
+  const int hk = n / 2;
+  const long long nt = (long long)np * np * hk;
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nm) return;
-  const int k = (int)(q % h), j = (int)((q / h) % n), i = (int)(q / ((long long)h * n));
-  cx<T> res = mk(T(0), T(0));
-  if (i != n / 2 && j != n / 2 && k != n / 2) {
-    const int ix = i < n / 2 ? i : i + N0 - n, iy = j < n / 2 ? j : j + N0 - n, iz = k;
-    const int kx = ix > N0 / 2 ? ix - N0 : ix, ky = iy > N0 / 2 ? iy - N0 : iy, kz = iz > N0 / 2 ? iz - N0 : iz;
-    if (kx != N0 / 2 && ky != N0 / 2 && kz != N0 / 2) {
-      const int a = ix % np, b = iy % np, c = iz % np;
-      const int hp = np / 2 + 1;
-      const bool conjg = c > np / 2;
-      const long long fo = conjg ? ((long long)((np - a) % np) * np + (np - b) % np) * hp + (np - c)
-                                 : ((long long)a * np + b) * hp + c;
-      double ar = 0, ai = 0, W = 0;
-      for (int l = 0; l < 84; ++l) {
-        const cx<double> bx = sph[prank[l] * N0 + ix], by = sph[prank[84 + l] * N0 + iy],
-                         bz = sph[prank[168 + l] * N0 + iz];
-        const double wxr = bx.x * by.x - bx.y * by.y, wxi = bx.x * by.y + bx.y * by.x;
-        const double wr = wxr * bz.x - wxi * bz.y, wi = wxr * bz.y + wxi * bz.x;
-        cx<double> f = F[(long long)l * fcs + fo];
-        if (conjg) f.y = -f.y;
-        ar = ar + (wr * f.x - wi * f.y);
-        ai = ai + (wr * f.y + wi * f.x);
-        const double aw = hypot(wr, wi);
-        W = W + aw * aw;
-      }
-      const double pi = 4.0 * atan(1.0);
-      const double th = ((double)fdim * -1.0) * pi * (double)(kx + ky + kz) / sqrt(W) / (double)N0;
-      const double sc = sqrt((double)(fdim * fdim * fdim)), den = pow((double)N0, 1.5);
-      const double pr = sc * cos(th) / den, pim = sc * sin(th) / den;
-      double vr = ar * pr - ai * pim, vi = ar * pim + ai * pr;
-      const unsigned long long k2 =
-          (unsigned long long)((long long)kx * kx + (long long)ky * ky + (long long)kz * kz);
-      if (kk > 0 && k2 <= kk && k2 != 0) {
-        const double m = hypot(vr, vi);
-        vr /= m;
-        vi /= m;
-      }
-      res = mk(T(vr), T(vi));
+  if (q >= nt) return;
+  const int k = (int)(q % hk);
+  const long long ab = q / hk;
+  const int b = (int)(ab % np), a = (int)(ab / np);
+  const int ix[2] = {a, a + np}, iy[2] = {b, b + np};
+  int oi[2], oj[2];
+  for (int u = 0; u < 2; ++u) {
+    oi[u] = pan_out_index(ix[u], n, N0);
+    oj[u] = pan_out_index(iy[u], n, N0);
+  }
+  bool act[4];
+  bool any = false;
+  for (int u = 0; u < 4; ++u) {
+    act[u] = oi[u >> 1] >= 0 && oj[u & 1] >= 0;
+    any = any || act[u];
+  }
+  if (!any) return;
+  const int iz = k, c = iz % np, hp = np / 2 + 1;
+  const bool conjg = c > np / 2;
+  const long long fo = conjg ? ((long long)((np - a) % np) * np + (np - b) % np) * hp + (np - c)
+                             : ((long long)a * np + b) * hp + c;
+  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0}, W[4] = {0, 0, 0, 0};
+  for (int l = 0; l < 84; ++l) {
+    cx<double> f = F[(long long)l * fcs + fo];
+    if (conjg) f.y = -f.y;
+    const cx<double> bz = sph[prank[168 + l] * N0 + iz];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!act[u]) continue;
+      const cx<double> bx = sph[prank[l] * N0 + ix[u >> 1]], by = sph[prank[84 + l] * N0 + iy[u & 1]];
+      const double wxr = bx.x * by.x - bx.y * by.y, wxi = bx.x * by.y + bx.y * by.x;
+      const double wr = wxr * bz.x - wxi * bz.y, wi = wxr * bz.y + wxi * bz.x;
+      ar[u] = ar[u] + (wr * f.x - wi * f.y);
+      ai[u] = ai[u] + (wr * f.y + wi * f.x);
+      const double aw = hypot(wr, wi);
+      W[u] = W[u] + aw * aw;
     }
   }
-  wn[q] = res;
+  const double pi = 4.0 * atan(1.0);
+  const double sc = sqrt((double)(fdim * fdim * fdim)), den = pow((double)N0, 1.5);
+  const int kz = iz;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (!act[u]) continue;
+    const int ixu = ix[u >> 1], iyu = iy[u & 1];
+    const int kx = ixu > N0 / 2 ? ixu - N0 : ixu, ky = iyu > N0 / 2 ? iyu - N0 : iyu;
+    const double th = ((double)fdim * -1.0) * pi * (double)(kx + ky + kz) / sqrt(W[u]) / (double)N0;
+    const double pr = sc * cos(th) / den, pim = sc * sin(th) / den;
+    double vr = ar[u] * pr - ai[u] * pim, vi = ar[u] * pim + ai[u] * pr;
+    const unsigned long long k2 =
+        (unsigned long long)((long long)kx * kx + (long long)ky * ky + (long long)kz * kz);
+    if (kk > 0 && k2 <= kk && k2 != 0) {
+      const double m = hypot(vr, vi);
+      vr /= m;
+      vi /= m;
+    }
+    wn[((long long)oi[u >> 1] * n + oj[u & 1]) * (hk + 1) + k] = mk(T(vr), T(vi));
+  }
 }
 
 }  // namespace
@@ -607,7 +647,7 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
     a.mode = 2;
     a.cnt[0] = a.cnt[1] = a.cnt[2] = 1;
     a.out = root;
-    hipLaunchKernelGGL(k_pan_cells, dim3(1), dim3(128), 0, c.stream, a);
+    hipLaunchKernelGGL(k_pan_cells, dim3(1), dim3(256), 0, c.stream, a);
     check_launch("panphasia_root");
     if (lmax == 0) {
       // the cube is the root cell itself
@@ -626,18 +666,21 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
       }
       a.out = lev[L & 1];
       const long long ncell = (long long)a.cnt[0] * a.cnt[1] * a.cnt[2];
-      const unsigned gx = (unsigned)std::min<long long>(ncell, 65535);
-      const unsigned gy = (unsigned)((ncell + gx - 1) / gx);
-      hipLaunchKernelGGL(k_pan_cells, dim3(gx, gy), dim3(128), 0, c.stream, a);
+      const long long ngrp = (ncell + kPanCellsPerGroup - 1) / kPanCellsPerGroup;
+      const unsigned gx = (unsigned)std::min<long long>(ngrp, 65535);
+      const unsigned gy = (unsigned)((ngrp + gx - 1) / gx);
+      hipLaunchKernelGGL(k_pan_cells, dim3(gx, gy), dim3(256), 0, c.stream, a);
       check_launch("panphasia_cells");
       parent = lev[L & 1];
     }
     // r2c of the 84 coefficient fields (FFTW's unnormalised transform)
     for (int l = 0; l < 84; ++l) fft_r2c_3d<double>(c, coef + (size_t)l * fstride, np, np, np, 1.0);
     const long long nm = (long long)n * n * (n / 2 + 1);
+    fill<T>(c, (T*)wn, 2 * nm, T(0));
     {
       LaunchScope lc(c, "panphasia_combine", 0);
-      hipLaunchKernelGGL(k_pan_combine<T>, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c.stream, wn, n, N0, np,
+      const long long nt = (long long)np * np * (n / 2);
+      hipLaunchKernelGGL(k_pan_combine<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c.stream, wn, n, N0, np,
                          (const cx<double>*)coef, fstride / 2, dsph, dprank, (unsigned long long)d.kk, fdim);
       check_launch("panphasia_combine");
     }
