@@ -346,8 +346,11 @@ PanAxis pan_axis(unsigned long long origin, int rel, int lmax, long long np) {
 
 struct PanLevelArgs {
   const double* parent;  // level L coefficients, 84 per cell (cell-major); root for L = 0
-  const double* C;       // constraint rows [84][84]
-  const double* CT;      // their transpose
+  // the constraint rows' nonzeros (689 of 84 x 84): per row r, entries
+  // rp[r]..rp[r+1] of (ci, cv) in ascending column; per class-block column
+  // q = 84 * class + i, entries cp[q]..cp[q+1] of (rr, rv) in ascending row
+  const int *rp, *ci, *cp, *rr;
+  const double *cv, *rv;
   const int* tabi;       // row_index[84], row_class[84], class_begin[9], parity bits[84]
   const unsigned long long* coord[3];
   const int* child[3];
@@ -374,19 +377,36 @@ __global__ void __launch_bounds__(256) k_pan_cells(PanLevelArgs a) {
   constexpr int CPG = kPanCellsPerGroup;
   __shared__ double g[CPG][672];
   __shared__ double diff[CPG][84];
+  __shared__ long long dst[CPG][8];  // per child: output offset, -1 if not needed
   const int t = threadIdx.x;
   const long long grp = (long long)blockIdx.y * gridDim.x + blockIdx.x;
   const long long ncell = (long long)a.cnt[0] * a.cnt[1] * a.cnt[2];
   const bool root = a.mode == 2;
   const int* row_index = a.tabi;
   const int* row_class = a.tabi + 84;
-  const int* class_begin = a.tabi + 168;
   const int* pbits = a.tabi + 177;
   // lane t < 252 owns block / row t % 84 of cell slot t / 84
   const int slot = t / 84, tl = t % 84;
   const long long cell = grp * CPG + slot;
   const bool mine = t < 84 * CPG && cell < ncell;
   // 84 Threefry blocks of 8 uniforms per cell, tail replacement, Box-Muller pairs
+  // the output offsets of each slot's eight children (lane 84 s + c)
+  if (t < 84 * CPG && tl < 8) {
+    long long o = -1;
+    if (root) {
+      o = (slot == 0 && tl == 0) ? 0 : -1;
+    } else if (cell < ncell) {
+      const int cz = (int)(cell % a.cnt[2]), cy = (int)((cell / a.cnt[2]) % a.cnt[1]),
+                cx = (int)(cell / ((long long)a.cnt[2] * a.cnt[1]));
+      const int ix = (tl >> 2) & 1, iy = (tl >> 1) & 1, iz = tl & 1;
+      const int chx = a.child[0][2 * cx + ix], chy = a.child[1][2 * cy + iy], chz = a.child[2][2 * cz + iz];
+      if (chx >= 0 && chy >= 0 && chz >= 0)
+        o = a.mode == 1 ? ((long long)a.perm[0][chx] * a.np + a.perm[1][chy]) * (long long)(a.np + 2) +
+                              a.perm[2][chz]
+                        : (((long long)chx * a.cnt1[1] + chy) * a.cnt1[2] + chz) * 84;
+    }
+    dst[slot][tl] = o;
+  }
   if (mine) {
     unsigned long long j1 = 2, j2 = 2, j3 = 2;
     if (!root) {
@@ -410,8 +430,10 @@ __global__ void __launch_bounds__(256) k_pan_cells(PanLevelArgs a) {
     for (int w = 0; w < 8; w += 2) {
       const double radius = sqrt(-2.0 * log(u[w]));
       const double angle = two_pi * u[w + 1];
-      g[slot][8 * tl + w] = radius * cos(angle);
-      g[slot][8 * tl + w + 1] = radius * sin(angle);
+      double sn, cs;
+      sincos(angle, &sn, &cs);
+      g[slot][8 * tl + w] = radius * cs;
+      g[slot][8 * tl + w + 1] = radius * sn;
     }
   }
   __syncthreads();
@@ -420,8 +442,9 @@ __global__ void __launch_bounds__(256) k_pan_cells(PanLevelArgs a) {
     if (mine) {
       const double* par = a.parent + cell * 84;
       const int cl = row_class[tl];
+      // the reference's dense row sum; its zero entries add exact zeros
       double e = 0.0;
-      for (int i = 0; i < 84; ++i) e += a.CT[i * 84 + tl] * g[slot][84 * cl + i];
+      for (int k = a.rp[tl]; k < a.rp[tl + 1]; ++k) e += a.cv[k] * g[slot][84 * cl + a.ci[k]];
       diff[slot][tl] = par[row_index[tl]] - e;
     }
     __syncthreads();
@@ -429,9 +452,8 @@ __global__ void __launch_bounds__(256) k_pan_cells(PanLevelArgs a) {
     for (int q = t; q < 672 * CPG; q += 256) {
       const int s2 = q / 672, ql = q % 672;
       if (grp * CPG + s2 >= ncell) continue;
-      const int cl = ql / 84, i = ql % 84;
       double v = g[s2][ql];
-      for (int r = class_begin[cl]; r < class_begin[cl + 1]; ++r) v = v + a.C[r * 84 + i] * diff[s2][r];
+      for (int k = a.cp[ql]; k < a.cp[ql + 1]; ++k) v = v + a.rv[k] * diff[s2][a.rr[k]];
       g[s2][ql] = v;
     }
     __syncthreads();
@@ -441,31 +463,19 @@ __global__ void __launch_bounds__(256) k_pan_cells(PanLevelArgs a) {
   const double nrm = sqrt(0.125);
   for (int q = t; q < (root ? 84 : 672 * CPG); q += 256) {
     const int s2 = q / 672, ql = q % 672;
-    const long long cl2 = grp * CPG + s2;
-    if (cl2 >= ncell) continue;
     const int c = ql / 84, l = ql % 84, pb = pbits[l];
+    const long long o = dst[s2][c];
+    if (o < 0) continue;
     double e = 0.0;
     for (int j = 0; j < 8; ++j) {
       const double v = g[s2][84 * j + l];
       e += (__popc((unsigned)(c & (j ^ pb))) & 1) ? -v : v;
     }
     const double val = nrm * e;
-    if (root) {
-      a.out[l] = val;
-      continue;
-    }
-    const int cz = (int)(cl2 % a.cnt[2]), cy = (int)((cl2 / a.cnt[2]) % a.cnt[1]),
-              cx = (int)(cl2 / ((long long)a.cnt[2] * a.cnt[1]));
-    const int ix = (c >> 2) & 1, iy = (c >> 1) & 1, iz = c & 1;
-    const int chx = a.child[0][2 * cx + ix], chy = a.child[1][2 * cy + iy], chz = a.child[2][2 * cz + iz];
-    if (chx < 0 || chy < 0 || chz < 0) continue;
-    if (a.mode == 1) {
-      const long long pos =
-          ((long long)a.perm[0][chx] * a.np + a.perm[1][chy]) * (long long)(a.np + 2) + a.perm[2][chz];
-      a.coef[(long long)l * a.fstride + pos] = val;
-    } else {
-      a.out[(((long long)chx * a.cnt1[1] + chy) * a.cnt1[2] + chz) * 84 + l] = val;
-    }
+    if (a.mode == 1)
+      a.coef[(long long)l * a.fstride + o] = val;
+    else
+      a.out[o + l] = val;
   }
 }
 
@@ -582,9 +592,26 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
   PanAxis ax[3];
   for (int q = 0; q < 3; ++q) ax[q] = pan_axis(org[q], s.rel, lmax, np);
   // device tables
-  std::vector<double> CT(84 * 84);
-  for (int r = 0; r < 84; ++r)
-    for (int i = 0; i < 84; ++i) CT[i * 84 + r] = tb.C[r][i];
+  // nonzeros of the constraint rows, by row and by class-block column
+  std::vector<int> rp(85, 0), ci, cp(673, 0), rr;
+  std::vector<double> cv, rv;
+  for (int r = 0; r < 84; ++r) {
+    for (int i = 0; i < 84; ++i)
+      if (tb.C[r][i] != 0.0) {
+        ci.push_back(i);
+        cv.push_back(tb.C[r][i]);
+      }
+    rp[r + 1] = (int)ci.size();
+  }
+  for (int cl = 0; cl < 8; ++cl)
+    for (int i = 0; i < 84; ++i) {
+      for (int r = tb.class_begin[cl]; r < tb.class_begin[cl + 1]; ++r)
+        if (tb.C[r][i] != 0.0) {
+          rr.push_back(r);
+          rv.push_back(tb.C[r][i]);
+        }
+      cp[84 * cl + i + 1] = (int)rr.size();
+    }
   std::vector<int> tabi(84 + 84 + 9 + 84), prank(3 * 84);
   for (int r = 0; r < 84; ++r) {
     tabi[r] = tb.row_index[r];
@@ -613,8 +640,12 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
     return p;
   };
   try {
-    double* dC = (double*)up(&tb.C[0][0], sizeof(double) * 84 * 84);
-    double* dCT = (double*)up(CT.data(), sizeof(double) * 84 * 84);
+    const int* drp = (const int*)up(rp.data(), sizeof(int) * rp.size());
+    const int* dci = (const int*)up(ci.data(), sizeof(int) * ci.size());
+    const double* dcv = (const double*)up(cv.data(), sizeof(double) * cv.size());
+    const int* dcp = (const int*)up(cp.data(), sizeof(int) * cp.size());
+    const int* drr = (const int*)up(rr.data(), sizeof(int) * rr.size());
+    const double* drv = (const double*)up(rv.data(), sizeof(double) * rv.size());
     int* dtabi = (int*)up(tabi.data(), sizeof(int) * tabi.size());
     int* dprank = (int*)up(prank.data(), sizeof(int) * prank.size());
     cx<double>* dsph = (cx<double>*)up(sph.data(), sizeof(cx<double>) * sph.size());
@@ -636,8 +667,12 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
     }
     LaunchScope ls(c, "panphasia_cells", 0);
     PanLevelArgs a{};
-    a.C = dC;
-    a.CT = dCT;
+    a.rp = drp;
+    a.ci = dci;
+    a.cv = dcv;
+    a.cp = dcp;
+    a.rr = drr;
+    a.rv = drv;
     a.tabi = dtabi;
     a.np = np;
     a.fstride = fstride;
