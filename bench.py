@@ -306,7 +306,7 @@ def run_panphasia(M, n=512, steps=3, warmup=1):
     dt = time.perf_counter() - t1
     return {"metric": "Panphasia-HO white noise: grid points/s", "value": round(n ** 3 / (ms * 1e-3), 1),
             "unit": "points/s", "ms_per_step": round(ms, 3), "steps": steps, "warmup": warmup,
-            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "higher_is_better": True, "dtype": "f64 cell solve, f32 transforms", "data": "synthetic",
             "config": {"workload": f"Fill_Grid {n}^3, descriptor {PAN_DESCRIPTOR}", "n": n,
                        "coefficient_cube": info["np"], "fourier_grid": info["N0"], "relative_level": info["rel_level"]},
             "cpu_baseline": {"value": round(m ** 3 / dt, 1), "unit": "points/s", "kind": "port",

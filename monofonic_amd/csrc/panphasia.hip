@@ -357,7 +357,8 @@ struct PanLevelArgs {
   int cnt[3], cnt1[3];
   unsigned long long level;
   double* out;           // level L+1 coefficients (cell-major) or, for the root, its 84 values
-  double* coef;          // last level: coefficient field l at coef + l*fstride, rows of np+2
+  void* coef;            // last level: coefficient field l at coef + l*fstride, rows of np+2
+  int coef_f32;          // the coefficient fields are float (FP32 grids), else double
   const int* perm[3];
   long long fstride;
   int np;
@@ -472,9 +473,12 @@ __global__ void __launch_bounds__(256) k_pan_cells(PanLevelArgs a) {
       e += (__popc((unsigned)(c & (j ^ pb))) & 1) ? -v : v;
     }
     const double val = nrm * e;
-    if (a.mode == 1)
-      a.coef[(long long)l * a.fstride + o] = val;
-    else
+    if (a.mode == 1) {
+      if (a.coef_f32)
+        ((float*)a.coef)[(long long)l * a.fstride + o] = (float)val;
+      else
+        ((double*)a.coef)[(long long)l * a.fstride + o] = val;
+    } else
       a.out[o + l] = val;
   }
 }
@@ -497,7 +501,7 @@ __device__ __forceinline__ int pan_out_index(int x, int n, int N0) {
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-    k_pan_combine(cx<T>* wn, int n, int N0, int np, const cx<double>* __restrict__ F, long long fcs,
+    k_pan_combine(cx<T>* wn, int n, int N0, int np, const cx<T>* __restrict__ F, long long fcs,
                   const cx<double>* __restrict__ sph, const int* __restrict__ prank, unsigned long long kk, int fdim) {
   const int hk = n / 2;
   const long long nt = (long long)np * np * hk;
@@ -525,7 +529,8 @@ __global__ void __launch_bounds__(256)
                              : ((long long)a * np + b) * hp + c;
   double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0}, W[4] = {0, 0, 0, 0};
   for (int l = 0; l < 84; ++l) {
-    cx<double> f = F[(long long)l * fcs + fo];
+    const cx<T> fv = F[(long long)l * fcs + fo];
+    cx<double> f = mk((double)fv.x, (double)fv.y);
     if (conjg) f.y = -f.y;
     const cx<double> bz = sph[prank[168 + l] * N0 + iz];
 #pragma unroll
@@ -651,7 +656,9 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
     cx<double>* dsph = (cx<double>*)up(sph.data(), sizeof(cx<double>) * sph.size());
     double* root = (double*)dalloc(sizeof(double) * 84);
     double* lev[2] = {(double*)dalloc(sizeof(double) * 84 * maxcells), (double*)dalloc(sizeof(double) * 84 * maxcells)};
-    double* coef = (double*)dalloc(sizeof(double) * 84 * (size_t)fstride);
+    // coefficient fields and their transforms in the grid's precision (the
+    // reference computes Panphasia in real_t: float in single-precision builds)
+    T* coef = (T*)dalloc(sizeof(T) * 84 * (size_t)fstride);
     // per-level lists
     std::vector<const unsigned long long*> dcoord[3];
     std::vector<const int*> dchild[3];
@@ -677,6 +684,7 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
     a.np = np;
     a.fstride = fstride;
     a.coef = coef;
+    a.coef_f32 = sizeof(T) == 4;
     for (int q = 0; q < 3; ++q) a.perm[q] = dperm[q];
     // root cell
     a.mode = 2;
@@ -709,14 +717,14 @@ void panphasia_noise(Ctx& c, const PanDesc& d, int n, cx<T>* wn) {
       parent = lev[L & 1];
     }
     // r2c of the 84 coefficient fields (FFTW's unnormalised transform)
-    for (int l = 0; l < 84; ++l) fft_r2c_3d<double>(c, coef + (size_t)l * fstride, np, np, np, 1.0);
+    for (int l = 0; l < 84; ++l) fft_r2c_3d<T>(c, coef + (size_t)l * fstride, np, np, np, T(1));
     const long long nm = (long long)n * n * (n / 2 + 1);
     fill<T>(c, (T*)wn, 2 * nm, T(0));
     {
       LaunchScope lc(c, "panphasia_combine", 0);
       const long long nt = (long long)np * np * (n / 2);
       hipLaunchKernelGGL(k_pan_combine<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c.stream, wn, n, N0, np,
-                         (const cx<double>*)coef, fstride / 2, dsph, dprank, (unsigned long long)d.kk, fdim);
+                         (const cx<T>*)coef, fstride / 2, dsph, dprank, (unsigned long long)d.kk, fdim);
       check_launch("panphasia_combine");
     }
     MFX_HIP(hipStreamSynchronize(c.stream));
