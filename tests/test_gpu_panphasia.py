@@ -23,8 +23,9 @@ pytestmark = pytest.mark.gpu
 DESC = [
     ("[Panph6,L10,(100,200,300),S1,CH-999,unit_test]", 32),
     ("[Panph6,L9,(17,3,250),S3,CH-999,three]", 32),      # N0 = 48 > n: Fourier interpolation
-    ("[Panph6,L12,(4095,1,77),S1,KK6,CH-999,kklimit]", 16),  # wraps around the octree edge
+    ("[Panph6,L12,(4095,1,77),S1,KK6,CH-999,kklimit]", 16),  # KK limit
     ("[Panph6,L20,(424060,82570,148256),S1,KK0,CH-999,Auriga_100_vol2]", 16),
+    ("[Panph6,L9,(511,3,250),S3,CH-999,edge]", 64),  # 48^3 cube, N0 = 96; x wraps the octree edge
 ]
 
 
@@ -119,10 +120,15 @@ def test_icgen_panphasia_vs_oracle(ctx, orc):
             want = ref[w]
         assert rel_rms(got, want) < 1e-11
         assert rel_max(got, want) < 1e-10
-    # seeds do not apply: a second run reproduces the first
+    # seeds do not apply: a second run reproduces the first, and so does a
+    # graph replay
     first = ic.particles(3)
     ic.set_seed(99)
     ic.run()
+    np.testing.assert_array_equal(ic.particles(3), first)
+    ic.capture()
+    ic.launch()
+    ctx.synchronize()
     np.testing.assert_array_equal(ic.particles(3), first)
     with pytest.raises(M.MfxError, match="DoFixing"):
         M.ICGenerator(ctx, n, BOX, 24.0, LPTorder=1, generator="PANPHASIA", descriptor=desc, DoFixing=True)

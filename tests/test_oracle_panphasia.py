@@ -175,3 +175,15 @@ def test_synthesis_vs_direct_sum(origin, level, size, kk, n):
     ref = _direct_noise(origin, level, size, kk, n)
     np.testing.assert_allclose(got, ref, atol=1e-12 * np.max(np.abs(ref)))
     assert np.max(np.abs(got)) > 0.1
+
+
+def test_coefficients_depend_on_absolute_cell_only():
+    # a cell's coefficients follow from its octree ancestry alone: two cubes
+    # whose cells overlap -- one wrapping around the octree edge along x --
+    # agree on the shared cells
+    a = orc.pan_coefficients((15, 2, 3), 4, 1, 6)  # abs x at level 5: 30, 31, 0, 1, 2, 3
+    b = orc.pan_coefficients((0, 2, 3), 4, 1, 6)   # abs x: 0 .. 5
+    np.testing.assert_array_equal(a[2:6], b[0:4])
+    c = orc.pan_coefficients((14, 2, 3), 4, 1, 6)  # abs x: 28 .. 33 -> 28..31, 0, 1
+    np.testing.assert_array_equal(a[0:4], c[2:6])
+    assert np.std(a) > 0.5
