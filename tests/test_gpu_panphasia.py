@@ -132,3 +132,44 @@ def test_icgen_panphasia_vs_oracle(ctx, orc):
     np.testing.assert_array_equal(ic.particles(3), first)
     with pytest.raises(M.MfxError, match="DoFixing"):
         M.ICGenerator(ctx, n, BOX, 24.0, LPTorder=1, generator="PANPHASIA", descriptor=desc, DoFixing=True)
+
+
+def test_fill_256_mode_variance(ctx):
+    """At 256^3 (a 128^3 coefficient cube, beyond the oracle's sizes): the
+    coefficient transforms are independent complex Gaussians of variance np^3,
+    so a mode's expected |f(k)|^2 is its Bessel weight sum W(k) (the phase
+    factor has unit modulus); the sample mean of |f|^2 / W over 2e5 modes off
+    the kz = 0 and Nyquist planes is 1 within its sampling error (1/sqrt(2e5))."""
+    import monofonic_amd as M
+    from scipy.special import spherical_jn
+    desc, n = "[Panph6,L11,(2043,2045,2046),S5,CH-999,Testing_only]", 256
+    info = M.panphasia_info(desc, n)
+    N0, fdim = info["N0"], 2
+    print(f"panphasia variance n={n} N0={N0}", flush=True)
+    g = M.Grid_FFT(ctx, n, 100.0, "float")
+    M.RNG_panphasia(desc).Fill_Grid(g)
+    f = g.kdata()
+    rng = np.random.default_rng(7)
+    m = 200000
+    i = rng.integers(0, n, m)
+    j = rng.integers(0, n, m)
+    k = rng.integers(1, n // 2, m)
+    keep = (i != n // 2) & (j != n // 2)
+    i, j, k = i[keep], j[keep], k[keep]
+    # N0-grid indices of the copied modes and their signed wave numbers
+    ix = np.where(i < n // 2, i, i + N0 - n)
+    iy = np.where(j < n // 2, j, j + N0 - n)
+    kx = np.where(ix > N0 // 2, ix - N0, ix)
+    ky = np.where(iy > N0 // 2, iy - N0, iy)
+
+    def B2(p, kk):
+        return (2 * p + 1) * spherical_jn(p, np.pi * fdim * np.abs(kk) / N0) ** 2
+
+    W = np.zeros(len(i))
+    for nn in range(7):
+        for px in range(nn + 1):
+            for py in range(nn - px + 1):
+                W += B2(px, kx) * B2(py, ky) * B2(nn - px - py, k)
+    r = np.abs(f[i, j, k]) ** 2 / W
+    print(f"  mean |f|^2/W {np.mean(r):.4f} over {len(r)} modes", flush=True)
+    assert abs(np.mean(r) - 1.0) < 0.015
