@@ -173,3 +173,30 @@ def test_fill_256_mode_variance(ctx):
     r = np.abs(f[i, j, k]) ** 2 / W
     print(f"  mean |f|^2/W {np.mean(r):.4f} over {len(r)} modes", flush=True)
     assert abs(np.mean(r) - 1.0) < 0.015
+
+
+def test_icgen_panphasia_bcc_baryons_vs_oracle(ctx, orc):
+    """PANPHASIA noise with the bcc load and baryons on the secondary lattice
+    (ic_generator.cc:153-168,608-671): every sublattice of both species
+    against the oracle, FP64 32^3 2LPT"""
+    import monofonic_amd as M
+    from test_gpu_lpt import BOX, oracle_pipeline
+    desc, n = DESC[0]
+    print("icgen panphasia bcc + baryons n=32", flush=True)
+    ic = M.ICGenerator(ctx, n, BOX, 24.0, LPTorder=2, precision="double", generator="PANPHASIA", descriptor=desc,
+                       ParticleLoad="bcc", DoBaryons=True)
+    ic.run()
+    cos = orc.Cosmology()
+    for species, secondary in (("dm", False), ("baryon", True)):
+        ref, _, _ = oracle_pipeline(orc, n, BOX, 24.0, 2, 0, cos, generator="PANPHASIA", descriptor=desc,
+                                    lattice="bcc", secondary=secondary)
+        so = orc.SECOND_LATTICE_SHIFT["bcc"] if secondary else (0, 0, 0)
+        for w in range(6):
+            got = ic.particles(w, species=species)
+            for l, s in enumerate(orc.LATTICE_SHIFTS["bcc"]):
+                g, r = got[l], ref[w][l]
+                if w < 3:
+                    q = ((np.arange(n) + s[w] + so[w]) / n).reshape([n if d == w else 1 for d in range(3)])
+                    g, r = g - q, r - q
+                assert rel_rms(g, r) < 1e-11, (species, w, l)
+                assert rel_max(g, r) < 1e-10, (species, w, l)
