@@ -119,7 +119,9 @@ int mfx_grid_scale(mfx_grid *g, double s);
 int mfx_grid_inverse_laplacian(mfx_grid *g);
 /* zero_DC_mode (grid_fft.hh:844-887) */
 int mfx_grid_zero_dc(mfx_grid *g);
-/* mean() and std() (grid_fft.hh:507-595), real space only */
+/* mean() and std() (grid_fft.hh:507-595).  In k space std() runs over the
+ * n0 n1 (n2/2+1) modes (real parts and |z|^2) and mean() over the raw buffer
+ * relem(i,j,k) with k < n2/2+1, as the reference's loops over sizes_ do */
 int mfx_grid_stats(mfx_grid *g, double *mean, double *std);
 /* Compute_PowerSpectrum (src/grid_fft.cc:1133-1210) */
 int mfx_grid_power_spectrum(mfx_grid *g, double *bin_k, double *bin_P, double *bin_eP, double *bin_count,

@@ -452,16 +452,40 @@ int mfx_grid_inverse_laplacian(mfx_grid* g) {
 int mfx_grid_stats(mfx_grid* g, double* mean, double* sd) {
   return guard([&] {
     require_grid(g);
-    MFX_REQUIRE(g->space == MFX_RSPACE, "stats are computed in real space");
+    const int n0 = (int)g->n[0], n1 = (int)g->n[1], n2 = (int)g->n[2];
+    const long long pitch = (long long)n2 + 2;
     double s[2];
-    if (g->prec == MFX_FLOAT)
-      sum2<float>(g->ctx->c, (float*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], (long long)g->n[2] + 2, s);
-    else
-      sum2<double>(g->ctx->c, (double*)g->d, (int)g->n[0], (int)g->n[1], (int)g->n[2], (long long)g->n[2] + 2, s);
-    const double cnt = (double)g->n[0] * g->n[1] * g->n[2];
-    const double m1 = s[0] / cnt, m2 = s[1] / cnt;
-    if (mean) *mean = m1;
-    if (sd) *sd = std::sqrt(m2 - m1 * m1);
+    if (g->space == MFX_RSPACE) {
+      if (g->prec == MFX_FLOAT)
+        sum2<float>(g->ctx->c, (float*)g->d, n0, n1, n2, pitch, s);
+      else
+        sum2<double>(g->ctx->c, (double*)g->d, n0, n1, n2, pitch, s);
+      const double cnt = (double)n0 * n1 * n2;
+      const double m1 = s[0] / cnt, m2 = s[1] / cnt;
+      if (mean) *mean = m1;
+      if (sd) *sd = std::sqrt(m2 - m1 * m1);
+      return;
+    }
+    // k space: std() sums the real parts and |z|^2 of the n0 n1 (n2/2+1)
+    // modes; mean() reads relem over those sizes, i.e. the first n2/2+1 reals
+    // of each padded row (grid_fft.hh:507-590)
+    const int h = n2 / 2 + 1;
+    const double cnt = (double)n0 * n1 * h;
+    if (mean) {
+      if (g->prec == MFX_FLOAT)
+        sum2<float>(g->ctx->c, (float*)g->d, n0, n1, h, pitch, s);
+      else
+        sum2<double>(g->ctx->c, (double*)g->d, n0, n1, h, pitch, s);
+      *mean = s[0] / cnt;
+    }
+    if (sd) {
+      if (g->prec == MFX_FLOAT)
+        sum2c<float>(g->ctx->c, (const cx<float>*)g->d, (long long)n0 * n1 * h, s);
+      else
+        sum2c<double>(g->ctx->c, (const cx<double>*)g->d, (long long)n0 * n1 * h, s);
+      const double m1 = s[0] / cnt, m2 = s[1] / cnt;
+      *sd = std::sqrt(m2 - m1 * m1);
+    }
   });
 }
 

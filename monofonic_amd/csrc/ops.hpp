@@ -94,6 +94,8 @@ void add_const(Ctx& c, T* f, int n0, int n1, int n2, long long pitch, double v);
 template <typename T>
 void sum2(Ctx& c, const T* f, int n0, int n1, int n2, long long pitch, double* out2);
 template <typename T>
+void sum2c(Ctx& c, const cx<T>* f, long long cnt, double* out2);
+template <typename T>
 int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, double* bP, double* beP, double* bc,
                    int maxbins);
 

@@ -297,6 +297,33 @@ __global__ void k_sum2(const T* f, int n0, int n1, int n2, long long pitch, doub
   }
 }
 
+// sums of the real parts and of |z|^2 over cnt complex values (std() of a
+// k-space grid, grid_fft.hh:507-522)
+template <typename T>
+__global__ void k_sum2c(const cx<T>* f, long long cnt, double* part) {
+  __shared__ double s1[256], s2[256];
+  double a = 0, b = 0;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += (long long)gridDim.x * blockDim.x) {
+    const double re = (double)f[p].x, im = (double)f[p].y;
+    a += re;
+    b += re * re + im * im;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+      s2[threadIdx.x] += s2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1[0];
+    part[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
 // power spectrum bins (grid_fft.cc:1133-1210): per-bin sums of |z|^2, |z|^4, k, count
 template <typename T>
 __global__ void k_pk(const cx<T>* f, int n, double kfac, int nbins, double* bins) {
@@ -488,6 +515,25 @@ void add_const(Ctx& c, T* f, int n0, int n1, int n2, long long pitch, double v) 
 }
 
 template <typename T>
+void sum2c(Ctx& c, const cx<T>* f, long long cnt, double* out2) {
+  const int nb = 1024;
+  double* part = (double*)c.alloc(sizeof(double) * 2 * nb);
+  hipLaunchKernelGGL(k_sum2c<T>, dim3(nb), dim3(256), 0, c.stream, f, cnt, part);
+  check_launch("sum2c");
+  std::vector<double> h(2 * nb);
+  MFX_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c.stream));
+  MFX_HIP(hipStreamSynchronize(c.stream));
+  c.release(part);
+  double a = 0, b = 0;
+  for (int q = 0; q < nb; ++q) {
+    a += h[2 * q];
+    b += h[2 * q + 1];
+  }
+  out2[0] = a;
+  out2[1] = b;
+}
+
+template <typename T>
 void sum2(Ctx& c, const T* f, int n0, int n1, int n2, long long pitch, double* out2) {
   const int nb = 1024;
   double* part = (double*)c.alloc(sizeof(double) * 2 * nb);
@@ -553,6 +599,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
   template void density_k<T>(Ctx&, cx<T>*, const cx<T>*, int, double, double);                                  \
   template void lattice_shift<T>(Ctx&, cx<T>*, int, double, double, double);                                    \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
+  template void sum2c<T>(Ctx&, const cx<T>*, long long, double*);                                                 \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \
   template int power_spectrum<T>(Ctx&, const cx<T>*, int, double, double*, double*, double*, double*, int);
 MFX_INST(float)

@@ -97,3 +97,24 @@ def test_power_spectrum_matches_oracle(ctx, orc, n, prec, tol):
     assert np.max(np.abs(k[sel] / rk[sel] - 1)) < tol
     assert np.max(np.abs(P[sel] / rP[sel] - 1)) < tol
     assert np.max(np.abs(eP[sel] - reP[sel]) / rP[sel]) < 100 * tol
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_stats_in_k_space(ctx, prec):
+    """std() of a k-space grid over its n0 n1 (n2/2+1) modes -- real parts
+    and |z|^2 -- and mean() over relem(i,j,k), k < n2/2+1, the first n2/2+1
+    reals of each padded row (grid_fft.hh:507-590)"""
+    import monofonic_amd as M
+    n = 24
+    rng = np.random.default_rng(11)
+    g = M.Grid_FFT(ctx, n, 1.0, prec, initialspace=M.kspace_id)
+    raw = rng.standard_normal(n * n * (n + 2)) + 0.3
+    g.upload(raw)
+    x = raw.reshape(n, n, n + 2)
+    z = x[:, :, 0::2] + 1j * x[:, :, 1::2]
+    s1, s2, cnt = np.sum(z.real), np.sum(np.abs(z) ** 2), z.size
+    want_sd = np.sqrt(s2 / cnt - (s1 / cnt) ** 2)
+    want_mean = np.sum(x[:, :, : n // 2 + 1]) / (n * n * (n // 2 + 1))
+    tol = 1e-12 if prec == "double" else 1e-6
+    assert abs(g.std() - want_sd) < tol * want_sd
+    assert abs(g.mean() - want_mean) < tol * max(1.0, abs(want_mean))
