@@ -238,8 +238,13 @@ int mfx_set_particles(mfx_grid *field, double *out_dev, int idim, double lunit, 
  * RNG_plugin::Fill_Grid (include/random_plugin.hh:26-39) */
 /* MUSIC1 (src/plugins/random_music.cc:92-116, random_music_wnoise_generator.cc
  * :35-66,588-627,747-796): real-space N(0,1), leaves the grid in real space.
- * The grid size is a power of two; the cube size (the grid size if that is
- * smaller) must divide it and be at most 1024, else an error is returned. */
+ * The grid size is a power of two (random_music.cc:66-74) and the cube size
+ * (clipped to the grid size) at most 1024, else an error is returned.  A cube
+ * size that does not divide the grid repeats cubes as the reference's cell
+ * lookup does (random_music_wnoise_generator.hh:195-222): grid point i reads
+ * cube (i / cubesize) mod floor(n / cubesize), cell i mod cubesize, and a cell
+ * read by m grid points has the mean subtracted m times, as the reference's
+ * zero-mean loop over grid points does (random_music_wnoise_generator.cc:44-65). */
 int mfx_rng_music1_fill(mfx_grid *g, long seed, int cubesize);
 /* No counterpart in the reference.  The MUSIC1 fill above for callers that
  * cannot allocate device memory (code under HIP graph capture, serving loops,
@@ -349,6 +354,9 @@ typedef struct {
   const char *descriptor; /* generator 3 (PANPHASIA): the Panphasia-HO descriptor; the
                              field is computed once at creation (it depends on the
                              descriptor only; seeds do not apply) */
+  int cubesize;         /* generator 0 (MUSIC1): [random] cubesize (random_music.cc:120);
+                           0 selects the reference's default 32.  A size that does not
+                           divide n repeats cubes (see mfx_rng_music1_fill) */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;

@@ -556,7 +556,7 @@ class ICGenerator:
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
                  autotune=True, assemble_each=False, field_output=False, own_particles=False, ParticleLoad="sc",
-                 DoBaryons=False, generic_out_eulerian=False, descriptor=None):
+                 DoBaryons=False, generic_out_eulerian=False, descriptor=None, cubesize=32):
         self.ctx = ctx
         self.n = int(GridRes)
         self.lattice = str(ParticleLoad)
@@ -585,6 +585,8 @@ class ICGenerator:
         # [random] descriptor of the PANPHASIA plugin (kept alive with the generator)
         self._descriptor = None if descriptor is None else str(descriptor).encode()
         p.descriptor = self._descriptor
+        # [random] cubesize of the MUSIC1 plugin (random_music.cc:120)
+        p.cubesize = int(cubesize)
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 

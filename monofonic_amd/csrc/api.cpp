@@ -1460,6 +1460,10 @@ struct mfx_icgen {
   std::string descriptor;
 };
 
+// [random] cubesize of the MUSIC1 generator (random_music.cc:120); 0 selects
+// the reference's default DEF_RAN_CUBE_SIZE = 32
+static int icgen_cubesize(const mfx_icgen_params& p) { return p.cubesize > 0 ? p.cubesize : 32; }
+
 static void icgen_free(mfx_icgen* ic) {
   if (ic->table_job.valid()) ic->table_job.wait();
   for (auto*& gg : ic->g)
@@ -1511,6 +1515,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
     // random_panphasia.cc:79-84
     MFX_REQUIRE(p->generator != 3 || !p->fixing, "PANPHASIA: incompatible parameter (DoFixing)");
     MFX_REQUIRE(p->generator != 3 || p->descriptor != nullptr, "PANPHASIA: no descriptor");
+    MFX_REQUIRE(p->cubesize >= 0 && p->cubesize <= 1024, "MUSIC1 cube size must be 0 (default 32) .. 1024");
     // ParticleLoad sc/bcc/fcc/rsc (ic_generator.cc:131-139)
     MFX_REQUIRE(p->lattice >= 0 && p->lattice <= 3, "lattice must be 0 (sc), 1 (bcc), 2 (fcc) or 3 (rsc)");
     MFX_REQUIRE(p->lattice == 0 || !p->use_plt, "PLT corrections are built for the sc lattice only");
@@ -1598,7 +1603,7 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       ic->dev_seed = c->c.alloc(16);
       const size_t table_bytes = p->generator == 1 ? sizeof(unsigned) * p->n * p->n : 0;
       if (p->generator == 1) ic->dev_table = (unsigned*)c->c.alloc(table_bytes);
-      if (p->generator == 0) ic->rng_scratch = c->c.alloc(music1_scratch_bytes((int)p->n, 32));
+      if (p->generator == 0) ic->rng_scratch = c->c.alloc(music1_scratch_bytes((int)p->n, icgen_cubesize(*p)));
       if (p->generator == 3) {
         PanDesc d;
         pan_parse(p->descriptor, d);
@@ -1668,7 +1673,7 @@ extern "C" int mfx_icgen_destroy(mfx_icgen* ic) {
 static void icgen_stage_seed(mfx_icgen* ic, long seed) {
   const mfx_icgen_params& p = ic->p;
   mfx_grid* wn = ic->g[0];
-  if (p.generator == 0) check_music1(wn, seed, 32);
+  if (p.generator == 0) check_music1(wn, seed, icgen_cubesize(p));
   if (p.generator == 2) check_threefry(wn, (unsigned long long)seed);
   if (ic->table_job.valid()) ic->table_job.wait();
   const int q = ic->slot ^ 1;
@@ -1750,7 +1755,8 @@ static void icgen_device_run(mfx_icgen* ic) {
   // white noise (ic_generator.cc:249-257)
   if (p.generator == 0) {
     // the mean of the noise only reaches its k = 0 mode, which phi(1) zeroes
-    music1_dev(wn, (const long*)ic->dev_seed, 32, ic->rng_scratch, 2);
+    // (unless the cubes wrap, where the fill subtracts it anyway)
+    music1_dev(wn, (const long*)ic->dev_seed, icgen_cubesize(p), ic->rng_scratch, 2);
   } else if (p.generator == 1) {
     ngenic_dev(wn, ic->dev_table);
   } else if (p.generator == 2) {

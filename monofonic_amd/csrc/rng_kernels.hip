@@ -152,10 +152,18 @@ __device__ __forceinline__ void accept_ranks(bool ok0, bool ok1, int* cnt, int& 
   off1 = before1 + __popcll(b1 & lt);
 }
 
-// SUMS: accumulate the per-cube double-double sums for the mean
+// multiplicity along one axis of cell s of cube c: the grid points i < n
+// with (i / cs) % nc == c and i % cs == s (at most 2, as nc = floor(n / cs))
+__device__ __forceinline__ int music1_mult(int n, int nc, int cs, int c, int s) {
+  return (n - 1 - s - c * cs) / (nc * cs) + 1;
+}
+
+// SUMS: accumulate the per-cube double-double sums for the mean; with WRAP
+// (a cube size that does not divide n) each cell counts once per grid point
+// that reads it
 template <typename T, bool SUMS>
-__global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, int lcs, const long* seedp,
-                                                      dd* part) {
+__global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, int cs, int lcs, int wrap,
+                                                      const long* seedp, dd* part) {
   __shared__ unsigned mt[MT_N];
   const long baseseed = *seedp;
   __shared__ int wcnt[8];
@@ -165,9 +173,9 @@ __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, in
   const int ci = (int)(icube / ((long long)nc * nc)), cj = (int)((icube / nc) % nc), ck = (int)(icube % nc);
   if (tid == 0) mt_seed(mt, (unsigned long)(baseseed + icube));
   __syncthreads();
-  // the cube size cs = 2^lcs (a power of two, as the grid size is) and a cube
-  // holds cs^3 < 2^31 values (both checked on the host), so the stream
-  // positions are 32-bit and split into cell indices by shifts
+  // a cube holds cs^3 < 2^31 values (checked on the host), so the stream
+  // positions are 32-bit; they split into cell indices by shifts when
+  // cs = 2^lcs and by division otherwise (lcs < 0)
   const unsigned ucs = (unsigned)cs, cnt = ucs * ucs * ucs;
   const long long pitch = n + 2;
   unsigned count = 0;
@@ -196,11 +204,31 @@ __global__ void __launch_bounds__(256) k_music1_cubes(T* data, int n, int nc, in
       if (!ok[r]) continue;
       const unsigned e = count + (unsigned)(r == 0 ? off0 : total0 + off1);
       if (e < cnt) {
-        const unsigned kk = e & (ucs - 1u), jj = (e >> lcs) & (ucs - 1u), ii = e >> (2 * lcs);
+        unsigned kk, jj, ii;
+        if (lcs >= 0) {
+          kk = e & (ucs - 1u);
+          jj = (e >> lcs) & (ucs - 1u);
+          ii = e >> (2 * lcs);
+        } else {
+          const unsigned q = e / ucs;
+          kk = e - q * ucs;
+          jj = q % ucs;
+          ii = q / ucs;
+        }
         long long gi = (long long)ci * cs + ii, gj = (long long)cj * cs + jj, gk = (long long)ck * cs + kk;
         T sv = T(val[r]);
         data[(gi * n + gj) * pitch + gk] = sv;
-        if constexpr (SUMS) acc = dd_add(acc, (double)sv);
+        if constexpr (SUMS) {
+          if (wrap) {
+            // m * sv exactly as the pair (p, e) of a two-product
+            const double m = (double)(music1_mult(n, nc, cs, ci, (int)ii) * music1_mult(n, nc, cs, cj, (int)jj) *
+                                      music1_mult(n, nc, cs, ck, (int)kk));
+            const double pr = m * (double)sv;
+            acc = dd_add(dd_add(acc, pr), fma(m, (double)sv, -pr));
+          } else {
+            acc = dd_add(acc, (double)sv);
+          }
+        }
       }
     }
     count += (unsigned)nacc;
@@ -246,8 +274,51 @@ __global__ void k_sub_mean(T* data, int n, const double* mean, int zeromean) {
   if (zeromean) data[p] = T((double)data[p] - mean[0]);
 }
 
+// Cube size that does not divide the grid: the cubes fill [0, nc cs)^3 and
+// every other grid point copies the cell it reads, (i / cs) % nc cubes and
+// i % cs cells along each axis (random_music_wnoise_generator.hh:195-222).
+// The source region is not written here, so the copy is race-free.
+template <typename T, typename I>
+__global__ void k_music1_wrap(T* data, int n, int nc, int cs) {
+  const I p = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+  if (p >= (I)n * (I)n * (I)n) return;
+  const int k = (int)(p % (I)n), j = (int)((p / (I)n) % (I)n), i = (int)(p / ((I)n * (I)n));
+  const int side = nc * cs;
+  if (i < side && j < side && k < side) return;
+  const I pitch = (I)(n + 2);
+  const I bi = (I)(((i / cs) % nc) * cs + i % cs), bj = (I)(((j / cs) % nc) * cs + j % cs),
+          bk = (I)(((k / cs) % nc) * cs + k % cs);
+  data[((I)i * (I)n + (I)j) * pitch + (I)k] = data[(bi * (I)n + bj) * pitch + bk];
+}
+
+// zero mean of wrapped cubes: the reference subtracts the mean once per grid
+// point from the cell the point reads (random_music_wnoise_generator.cc:58-64),
+// so every copy of a cell shared by m grid points carries m subtractions, each
+// rounded to T
+template <typename T, typename I>
+__global__ void k_sub_mean_wrap(T* data, int n, int nc, int cs, const double* mean) {
+  const I p = (I)blockIdx.x * (I)blockDim.x + (I)threadIdx.x;
+  const I pitch = (I)(n + 2);
+  if (p >= (I)n * (I)n * pitch) return;
+  const int k = (int)(p % pitch);
+  if (k >= n) {
+    data[p] = T(0);
+    return;
+  }
+  const I q = p / pitch;
+  const int j = (int)(q % (I)n), i = (int)(q / (I)n);
+  const int m = music1_mult(n, nc, cs, (i / cs) % nc, i % cs) * music1_mult(n, nc, cs, (j / cs) % nc, j % cs) *
+                music1_mult(n, nc, cs, (k / cs) % nc, k % cs);
+  const double mu = mean[0];
+  T x = data[p];
+  for (int r = 0; r < m; ++r) x = T((double)x - mu);
+  data[p] = x;
+}
+
 // cubes per axis and cube size of the MUSIC1 layout of an n-grid
+// (random_music_wnoise_generator.cc:699-711)
 static void music1_cubes(int n, int cubesize, int& nc, int& cs) {
+  MFX_REQUIRE(cubesize > 0, "MUSIC1 fill: cube size must be positive");
   nc = (int)((double)n / cubesize);
   if (nc < 1) nc = 1;
   cs = cubesize;
@@ -255,8 +326,7 @@ static void music1_cubes(int n, int cubesize, int& nc, int& cs) {
     nc = 1;
     cs = n;
   }
-  MFX_REQUIRE((long long)nc * cs == n, "MUSIC1 fill: grid size must be a multiple of the cube size");
-  MFX_REQUIRE((cs & (cs - 1)) == 0 && cs <= 1024, "MUSIC1 fill: cube size must be a power of two up to 1024");
+  MFX_REQUIRE(cs <= 1024, "MUSIC1 fill: cube size must be at most 1024");
 }
 
 size_t music1_scratch_bytes(int n, int cubesize) {
@@ -278,6 +348,11 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
   const long long ncube = (long long)nc * nc * nc;
   int lcs = 0;
   while ((1 << lcs) < cs) ++lcs;
+  if ((1 << lcs) != cs) lcs = -1;
+  const int wrap = (long long)nc * cs != n;
+  // wrapped cubes take the mean once per grid point and cell, which is not a
+  // constant shift: callers that discard the k = 0 mode still need it
+  if (wrap && zeromean == 2) zeromean = 1;
   double* mean = (double*)scratch;
   dd* part = (dd*)((char*)scratch + 256);
   const bool prof = c.profile;
@@ -291,11 +366,24 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
     LaunchScope ls(c, "music1_cubes", (double)n * n * n * sizeof(T));
     if (zeromean == 2)
       hipLaunchKernelGGL((k_music1_cubes<T, false>), dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs,
-                         lcs, dseed, part);
+                         lcs, wrap, dseed, part);
     else
       hipLaunchKernelGGL((k_music1_cubes<T, true>), dim3((unsigned)ncube), dim3(256), 0, c.stream, data, n, nc, cs,
-                         lcs, dseed, part);
+                         lcs, wrap, dseed, part);
     check_launch("music1_cubes");
+  }
+  const long long tot = (long long)n * n * (n + 2);
+  const bool wide = tot >= (1LL << 32);
+  if (wrap) {
+    const long long pts = (long long)n * n * n;
+    LaunchScope ls(c, "music1_wrap", 2.0 * pts * sizeof(T));
+    if (wide)
+      hipLaunchKernelGGL((k_music1_wrap<T, long long>), dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, c.stream,
+                         data, n, nc, cs);
+    else
+      hipLaunchKernelGGL((k_music1_wrap<T, unsigned>), dim3((unsigned)((pts + 255) / 256)), dim3(256), 0, c.stream,
+                         data, n, nc, cs);
+    check_launch("music1_wrap");
   }
   // zeromean 2: the field is only transformed and its k = 0 mode discarded
   // (phi(1) of the IC sequence), so neither the mean nor the unused padding
@@ -304,10 +392,20 @@ void music1_fill(Ctx& c, T* data, int n, const long* dseed, int cubesize, int ze
   hipLaunchKernelGGL(k_dd_mean, dim3(1), dim3(256), 0, c.stream, part, ncube,
                      1.0 / (double)((size_t)n * (size_t)n * (size_t)n), mean);
   check_launch("music1_mean");
-  {
-    const long long tot = (long long)n * n * (n + 2);
+  if (wrap && zeromean) {
     LaunchScope ls(c, "music1_submean", 2.0 * tot * sizeof(T));
-    if (tot < (1LL << 32))
+    if (wide)
+      hipLaunchKernelGGL((k_sub_mean_wrap<T, long long>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                         c.stream, data, n, nc, cs, mean);
+    else
+      hipLaunchKernelGGL((k_sub_mean_wrap<T, unsigned>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                         c.stream, data, n, nc, cs, mean);
+    check_launch("music1_submean");
+    return;
+  }
+  {
+    LaunchScope ls(c, "music1_submean", 2.0 * tot * sizeof(T));
+    if (!wide)
       hipLaunchKernelGGL((k_sub_mean<T, unsigned>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, data,
                          n, mean, zeromean);
     else

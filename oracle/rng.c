@@ -89,16 +89,24 @@ void orc_mt_words(unsigned long seed, size_t count, uint32_t *out) {
 }
 
 /* music_wnoise_generator(res,cubesize,seed,zeromean=true):
- *   fill_all (random_music_wnoise_generator.cc:747-796): per cube
- *   (i*nc+j)*nc+k an MT19937 seeded baseseed+icube, cube values in (ii,jj,kk)
- *   order from the ratio method (fill_cube, :588-627).  The per-cube mean
- *   subtraction at :781-793 passes the OpenMP reduction variable `sum`, which
- *   is private and zero-initialised inside that parallel loop, so it subtracts
- *   0; it is therefore a no-op and is not restated.
- *   zeromean (:44-65): subtract the global mean (a double-double sum, see
- *   dd_add below).
- *   Fill_Grid (random_music.cc:92-116) copies cell (i,j,k) of the cube
- *   structure into relem(i,j,k). */
+ *   initialize (random_music_wnoise_generator.cc:699-711): ncubes =
+ *   max(floor(res/cubesize), 1), the cube size clipped to res.
+ *   fill_all (:747-796): per cube (i*nc+j)*nc+k an MT19937 seeded
+ *   baseseed+icube, cube values in (ii,jj,kk) order from the ratio method
+ *   (fill_cube, :588-627), stored in the generator's type T.  The per-cube
+ *   mean subtraction at :781-793 passes the OpenMP reduction variable `sum`,
+ *   which is private and zero-initialised inside that parallel loop, so it
+ *   subtracts 0; it is therefore a no-op and is not restated.
+ *   Grid point i reads cube (i/cubesize) mod ncubes, cell i mod cubesize
+ *   (operator(), random_music_wnoise_generator.hh:187-222): when the cube size
+ *   does not divide res the cubes repeat (wrap) and several grid points share
+ *   one cell.
+ *   zeromean (:44-65): the mean over the res^3 grid points (a double-double
+ *   sum, see dd_add below), then `(*this)(i,j,k) = (*this)(i,j,k) - mean`
+ *   once per grid point: a cell shared by m grid points is decremented m
+ *   times, each result rounded to T.
+ *   Fill_Grid (random_music.cc:92-116) copies the cell of grid point (i,j,k)
+ *   into relem(i,j,k). */
 /* Knuth TwoSum accumulation (double-double): the reference sums the field in
  * an OpenMP reduction whose order depends on the thread count
  * (random_music_wnoise_generator.cc:48-55); the mean is defined here as the
@@ -116,34 +124,40 @@ static dd_t dd_add(dd_t a, double b) {
   return r;
 }
 
-static void music1_cube_values(long baseseed, int nc, int cs, size_t icube, double *vals) {
+static void music1_cube_values(long baseseed, size_t icube, size_t cnt, double *vals) {
   orc_mt_t st;
   long cubeseed = baseseed + (long)icube;
   orc_mt_seed(&st, (unsigned long)cubeseed);
-  const size_t cnt = (size_t)cs * cs * cs;
   for (size_t e = 0; e < cnt; ++e) vals[e] = orc_gauss_ratio(&st);
-  (void)nc;
 }
 
-void orc_music1_fill(double *data, int n, long baseseed, int cubesize) {
+/* the cube structure's cell of grid point i along one axis: index into the
+ * cube store laid out as [cube][cell] per axis, i.e. cube*cs + cell */
+static size_t music1_cell(size_t i, int nc, int cs) { return ((i / cs) % nc) * cs + i % cs; }
+
+/* f32: the cube store holds floats (T = float) */
+static void music1_fill(void *out, int f32, int n, long baseseed, int cubesize) {
   int nc = (int)((double)n / cubesize);
   if (nc < 1) nc = 1;
   int cs = cubesize;
   if (n < cubesize) { nc = 1; cs = n; }
   const size_t np = (size_t)n + 2;
-  const size_t ncube = (size_t)nc * nc * nc;
+  const size_t ncube = (size_t)nc * nc * nc, ccnt = (size_t)cs * cs * cs;
+  const size_t side = (size_t)nc * cs;  /* cube store: side^3 cells */
+  double *store = (double *)malloc(sizeof(double) * side * side * side);
 #pragma omp parallel
   {
-    double *vals = (double *)malloc(sizeof(double) * (size_t)cs * cs * cs);
+    double *vals = (double *)malloc(sizeof(double) * ccnt);
 #pragma omp for schedule(dynamic)
     for (size_t ic = 0; ic < ncube; ++ic) {
-      music1_cube_values(baseseed, nc, cs, ic, vals);
+      music1_cube_values(baseseed, ic, ccnt, vals);
       size_t ci = ic / ((size_t)nc * nc), cj = (ic / nc) % nc, ck = ic % nc;
       for (int ii = 0; ii < cs; ++ii)
         for (int jj = 0; jj < cs; ++jj)
           for (int kk = 0; kk < cs; ++kk) {
-            size_t i = ci * cs + ii, j = cj * cs + jj, k = ck * cs + kk;
-            data[(i * n + j) * np + k] = vals[((size_t)ii * cs + jj) * cs + kk];
+            double v = vals[((size_t)ii * cs + jj) * cs + kk];
+            if (f32) v = (double)(float)v;
+            store[((ci * cs + ii) * side + cj * cs + jj) * side + ck * cs + kk] = v;
           }
     }
     free(vals);
@@ -151,53 +165,34 @@ void orc_music1_fill(double *data, int n, long baseseed, int cubesize) {
   dd_t acc = {0.0, 0.0};
   for (size_t i = 0; i < (size_t)n; ++i)
     for (size_t j = 0; j < (size_t)n; ++j)
-      for (size_t k = 0; k < (size_t)n; ++k) acc = dd_add(acc, data[(i * n + j) * np + k]);
-  double mean = acc.hi + acc.lo;
-  mean *= 1.0 / (double)((size_t)n * n * n);
-  for (size_t i = 0; i < (size_t)n; ++i)
-    for (size_t j = 0; j < (size_t)n; ++j) {
-      for (size_t k = 0; k < (size_t)n; ++k) data[(i * n + j) * np + k] -= mean;
-      data[(i * n + j) * np + n] = 0.0;
-      data[(i * n + j) * np + n + 1] = 0.0;
-    }
-}
-
-void orc_music1_fill_f32(float *data, int n, long baseseed, int cubesize) {
-  int nc = (int)((double)n / cubesize);
-  if (nc < 1) nc = 1;
-  int cs = cubesize;
-  if (n < cubesize) { nc = 1; cs = n; }
-  const size_t np = (size_t)n + 2;
-  const size_t ncube = (size_t)nc * nc * nc;
-#pragma omp parallel
-  {
-    double *vals = (double *)malloc(sizeof(double) * (size_t)cs * cs * cs);
-#pragma omp for schedule(dynamic)
-    for (size_t ic = 0; ic < ncube; ++ic) {
-      music1_cube_values(baseseed, nc, cs, ic, vals);
-      size_t ci = ic / ((size_t)nc * nc), cj = (ic / nc) % nc, ck = ic % nc;
-      for (int ii = 0; ii < cs; ++ii)
-        for (int jj = 0; jj < cs; ++jj)
-          for (int kk = 0; kk < cs; ++kk) {
-            size_t i = ci * cs + ii, j = cj * cs + jj, k = ck * cs + kk;
-            data[(i * n + j) * np + k] = (float)vals[((size_t)ii * cs + jj) * cs + kk];
-          }
-    }
-    free(vals);
-  }
-  dd_t acc = {0.0, 0.0};
-  for (size_t i = 0; i < (size_t)n; ++i)
-    for (size_t j = 0; j < (size_t)n; ++j)
-      for (size_t k = 0; k < (size_t)n; ++k) acc = dd_add(acc, (double)data[(i * n + j) * np + k]);
-  double mean = acc.hi + acc.lo;
-  mean *= 1.0 / (double)((size_t)n * n * n);
-  for (size_t i = 0; i < (size_t)n; ++i)
-    for (size_t j = 0; j < (size_t)n; ++j) {
       for (size_t k = 0; k < (size_t)n; ++k)
-        data[(i * n + j) * np + k] = (float)((double)data[(i * n + j) * np + k] - mean);
-      data[(i * n + j) * np + n] = 0.0f;
-      data[(i * n + j) * np + n + 1] = 0.0f;
-    }
+        acc = dd_add(acc, store[(music1_cell(i, nc, cs) * side + music1_cell(j, nc, cs)) * side +
+                                music1_cell(k, nc, cs)]);
+  double mean = acc.hi + acc.lo;
+  mean *= 1.0 / (double)((size_t)n * n * n);
+  for (size_t i = 0; i < (size_t)n; ++i)
+    for (size_t j = 0; j < (size_t)n; ++j)
+      for (size_t k = 0; k < (size_t)n; ++k) {
+        double *c = store + (music1_cell(i, nc, cs) * side + music1_cell(j, nc, cs)) * side + music1_cell(k, nc, cs);
+        *c = f32 ? (double)(float)(*c - mean) : *c - mean;
+      }
+  for (size_t i = 0; i < (size_t)n; ++i)
+    for (size_t j = 0; j < (size_t)n; ++j)
+      for (size_t k = 0; k < np; ++k) {
+        double v = k < (size_t)n ? store[(music1_cell(i, nc, cs) * side + music1_cell(j, nc, cs)) * side +
+                                         music1_cell(k, nc, cs)]
+                                 : 0.0;
+        if (f32)
+          ((float *)out)[(i * n + j) * np + k] = (float)v;
+        else
+          ((double *)out)[(i * n + j) * np + k] = v;
+      }
+  free(store);
+}
+
+void orc_music1_fill(double *data, int n, long baseseed, int cubesize) { music1_fill(data, 0, n, baseseed, cubesize); }
+void orc_music1_fill_f32(float *data, int n, long baseseed, int cubesize) {
+  music1_fill(data, 1, n, baseseed, cubesize);
 }
 
 /* ---------------- Threefry-4x64-20 ---------------- */

@@ -71,7 +71,7 @@ def test_lpt_sequence_vs_oracle(ctx, orc, n, order):
             assert rel_max(A3[d].kdata(), orc.kview(rA[d])) < 1e-10
 
 
-def oracle_noise(orc, n, seed, generator="MUSIC1", descriptor=None):
+def oracle_noise(orc, n, seed, generator="MUSIC1", descriptor=None, cubesize=32):
     """the white-noise grid after Fill_Grid and wnoise.FourierTransformForward
     (ic_generator.cc:255-257): NGENIC and PANPHASIA fill k space directly, so
     the forward transform is a no-op"""
@@ -84,13 +84,13 @@ def oracle_noise(orc, n, seed, generator="MUSIC1", descriptor=None):
         return wn
     if generator == "NGENIC":
         return orc.ngenic_fill(n, orc.ngenic_seedtable(n, seed))
-    wn = orc.music1_fill(n, seed) if generator == "MUSIC1" else orc.threefry_fill(n, seed)
+    wn = orc.music1_fill(n, seed, cubesize) if generator == "MUSIC1" else orc.threefry_fill(n, seed)
     orc.fft_r2c(wn)
     return wn
 
 
 def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, generator="MUSIC1", lattice="sc",
-                    secondary=False, descriptor=None):
+                    secondary=False, descriptor=None, cubesize=32):
     """ic_generator::run restated on the CPU: white noise (MUSIC1, NGENIC or
     the Threefry stream), phi(1..3), A(3), growth scaling and FP64
     positions/velocities of the sc lattice, or, for lattice bcc/fcc/rsc,
@@ -101,7 +101,7 @@ def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, gen
     D = cos.Dplus(a)
     vfac = cos.vfact(a)
     volfac = (boxlen / n / 2.0 / np.pi) ** 1.5
-    wn = oracle_noise(orc, n, seed, generator, descriptor)
+    wn = oracle_noise(orc, n, seed, generator, descriptor, cubesize)
     orc.noise_normalise(wn, volfac)
     phi = np.zeros_like(wn)
     orc.phi1(phi, wn, boxlen, cos.amplitude_table(n, boxlen))
@@ -160,6 +160,36 @@ def test_icgen_fp64_matches_oracle(ctx, orc, n, order, zstart):
         # tolerance relative to the field RMS (sigma_Psi)
         assert rel_rms(psi_g, psi_r) < 1e-11, w
         assert rel_max(psi_g, psi_r) < 1e-10, w
+
+
+def test_icgen_music1_wrapping_cubes_vs_oracle(ctx, orc):
+    """[random] cubesize 48 on a 64^3 grid: the cubes repeat and the zero-mean
+    subtraction is not a constant shift (random_music_wnoise_generator.cc:44-65),
+    so the IC sequence must carry it; FP64 2LPT against the oracle, and a graph
+    replay with a second seed"""
+    import monofonic_amd as M
+    n, order, zstart = 64, 2, 50.0
+    print(f"icgen fp64 n={n} MUSIC1 cubesize 48", flush=True)
+    ic = M.ICGenerator(ctx, n, BOX, zstart, LPTorder=order, precision="double", generator="MUSIC1", seed=4711,
+                       cubesize=48)
+    cos = orc.Cosmology()
+    ic.capture()
+    for seed in (4711, 4712):
+        ic.set_seed(seed)
+        ic.launch()
+        ctx.synchronize()
+        ref, _, _ = oracle_pipeline(orc, n, BOX, zstart, order, seed, cos, cubesize=48)
+        for w in range(6):
+            got = ic.particles(w)
+            if w < 3:
+                psi_g, psi_r = got - lattice_q(n, w), ref[w] - lattice_q(n, w)
+            else:
+                psi_g, psi_r = got, ref[w]
+            assert rel_rms(psi_g, psi_r) < 1e-11, (seed, w)
+            assert rel_max(psi_g, psi_r) < 1e-10, (seed, w)
+    # the wrapped noise differs from the default cube size's
+    plain, _, _ = oracle_pipeline(orc, n, BOX, zstart, order, 4712, cos)
+    assert rel_rms(ic.particles(3), plain[3]) > 0.1
 
 
 def test_icgen_fp32_bulk_vs_fp64(ctx):

@@ -39,8 +39,9 @@ def test_threefry_uniforms_bit_exact(ctx, orc):
             assert np.array_equal(row, ref), (c, seed)
 
 
-# cube sizes below, equal to and above the grid size
-@pytest.mark.parametrize("n,cs", [(32, 32), (64, 32), (64, 16), (16, 32)])
+# cube sizes below, equal to and above the grid size, and sizes that do not
+# divide the grid (cubes repeat; random_music_wnoise_generator.hh:195-222)
+@pytest.mark.parametrize("n,cs", [(32, 32), (64, 32), (64, 16), (16, 32), (64, 48), (64, 24), (32, 12), (128, 100)])
 def test_music1_fill(ctx, orc, n, cs):
     import monofonic_amd as M
     g = M.Grid_FFT(ctx, n, 100.0, "double")

@@ -64,6 +64,45 @@ def test_music1_cube_stream(orc):
     assert np.max(np.abs(a[:, :, :n].ravel() - (raw - mean))) < 1e-15
 
 
+@pytest.mark.parametrize("n,cs", [(16, 6), (16, 12), (8, 3)])
+def test_music1_wrapping_cubes(orc, n, cs):
+    """a cube size that does not divide the grid: ncubes = floor(n/cs) cubes
+    per axis, grid point i reads cube (i // cs) % ncubes, cell i % cs
+    (random_music_wnoise_generator.hh:195-222); the zero-mean loop subtracts
+    the grid mean once per grid point, so a cell shared by m grid points
+    carries m subtractions (random_music_wnoise_generator.cc:44-65).
+    Restated here with numpy index maps and a bincount of the cells."""
+    seed = 31
+    nc = n // cs
+    cubes = np.stack([orc.gauss_ratio(seed + c, cs ** 3).reshape(cs, cs, cs) for c in range(nc ** 3)])
+    i = np.arange(n)
+    cube, cell = (i // cs) % nc, i % cs
+    ci, cj, ck = np.meshgrid(cube, cube, cube, indexing="ij")
+    si, sj, sk = np.meshgrid(cell, cell, cell, indexing="ij")
+    raw = cubes[(ci * nc + cj) * nc + ck, si, sj, sk]
+    mean = raw.sum() / n ** 3
+    # multiplicity of each grid point's cell: grid points sharing its cell
+    key = ((ci * cs + si) * (nc * cs) + (cj * cs + sj)) * (nc * cs) + (ck * cs + sk)
+    m = np.bincount(key.ravel())[key]
+    assert m.max() > 1
+    want = raw.copy()
+    for r in range(m.max()):
+        want = np.where(m > r, want - mean, want)
+    got = orc.music1_fill(n, seed, cs)
+    np.testing.assert_allclose(got[:, :, :n], want, rtol=0, atol=1e-14)
+    assert np.all(got[:, :, n:] == 0)
+    # one subtraction per cell instead would be a different field
+    assert np.max(np.abs(got[:, :, :n] - (raw - mean))) > 1e-3
+    # FP32 store: cells held as float, each subtraction rounded to float
+    raw32 = raw.astype(np.float32).astype(np.float64)
+    mean32 = raw32.sum() / n ** 3
+    w32 = raw32.astype(np.float32)
+    for r in range(m.max()):
+        w32 = np.where(m > r, (w32.astype(np.float64) - mean32).astype(np.float32), w32)
+    got32 = orc.music1_fill(n, seed, cs, f32=True)
+    assert np.max(np.abs(got32[:, :, :n] - w32)) <= 1e-6
+
+
 def test_ngenic_hermitian(orc):
     n = 16
     a = orc.ngenic_fill(n, orc.ngenic_seedtable(n, 5))
