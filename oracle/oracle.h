@@ -12,6 +12,20 @@
  *   real space : relem(i,j,k) = data[(i*n + j)*(n+2) + k]
  *   k space    : kelem(i,j,k) = cdata[(i*n + j)*(n/2+1) + k]   (complex)
  * for cubic grids of side n.  Complex values are interleaved (re,im) doubles.
+ *
+ * Parity unpinned (restated by reading the reference; no golden vector,
+ * closed form or invariant fixes them, so a mistake here would go unseen):
+ *   - orc_ngenic_seedtable: the order in which the four blocks of each octant
+ *     draw their seeds (random_ngenic.cc:47-75); its frequency mapping is
+ *     pinned by the N / N/2 mode agreement;
+ *   - the Threefry counter packing of a Panphasia cell and the conversion of
+ *     its words to uniforms with tail replacement
+ *     (uniform_rand_threefry4x64.c:254-493); the Threefry4x64-20 function
+ *     itself is pinned by the reference's three known-answer tests;
+ *   - the Panphasia descriptor check digit (uniform_rand_threefry4x64.c:495,
+ *     restated in the tests) and the assignment of a cell's deviates to
+ *     parity blocks (high_order_panphasia_routines.c:73-176); the cell solve
+ *     and the synthesis are pinned by moment propagation and a direct sum.
  */
 #ifndef MFX_ORACLE_H
 #define MFX_ORACLE_H
