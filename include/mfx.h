@@ -123,6 +123,12 @@ int mfx_grid_zero_dc(mfx_grid *g);
  * n0 n1 (n2/2+1) modes (real parts and |z|^2) and mean() over the raw buffer
  * relem(i,j,k) with k < n2/2+1, as the reference's loops over sizes_ do */
 int mfx_grid_stats(mfx_grid *g, double *mean, double *std);
+/* Grid_FFT::shift_field (include/grid_fft.hh:832-842): the grid is taken to k
+ * space (FourierTransformForward, a no-op there), every mode is multiplied by
+ * exp(i s.k dx) with k from get_k, dx = L/n per axis and (sx, sy, sz) in
+ * cells, and with transform_back the grid returns to real space.  The phase
+ * is formed in FP64. */
+int mfx_grid_shift_field(mfx_grid *g, double sx, double sy, double sz, int transform_back);
 /* Compute_PowerSpectrum (src/grid_fft.cc:1133-1210) */
 int mfx_grid_power_spectrum(mfx_grid *g, double *bin_k, double *bin_P, double *bin_eP, double *bin_count,
                             int maxbins, int *nbins);

@@ -192,6 +192,12 @@ class Grid_FFT:
     def zero_DC_mode(self):
         call("mfx_grid_zero_dc", self._h)
 
+    def shift_field(self, s, transform_back=True):
+        """Grid_FFT::shift_field (include/grid_fft.hh:832-842): multiply every
+        mode by exp(i s.k dx), s in cells; ends in real space with
+        transform_back, else in k space"""
+        call("mfx_grid_shift_field", self._h, float(s[0]), float(s[1]), float(s[2]), int(bool(transform_back)))
+
     def _stats(self):
         m, s = ctypes.c_double(), ctypes.c_double()
         call("mfx_grid_stats", self._h, ctypes.byref(m), ctypes.byref(s))

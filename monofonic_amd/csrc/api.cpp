@@ -449,6 +449,19 @@ int mfx_grid_inverse_laplacian(mfx_grid* g) {
   });
 }
 
+int mfx_grid_shift_field(mfx_grid* g, double sx, double sy, double sz, int transform_back) {
+  return guard([&] {
+    require_grid(g);
+    grid_forward(g, true);
+    const long long n0 = (long long)g->n[0], n1 = (long long)g->n[1], n2 = (long long)g->n[2];
+    if (g->prec == MFX_FLOAT)
+      shift_field<float>(g->ctx->c, (cx<float>*)g->d, n0, n1, n2, sx, sy, sz);
+    else
+      shift_field<double>(g->ctx->c, (cx<double>*)g->d, n0, n1, n2, sx, sy, sz);
+    if (transform_back) grid_backward(g, true);
+  });
+}
+
 int mfx_grid_stats(mfx_grid* g, double* mean, double* sd) {
   return guard([&] {
     require_grid(g);
@@ -493,8 +506,12 @@ int mfx_grid_zero_dc(mfx_grid* g) {
   return guard([&] {
     require_grid(g);
     if (g->space == MFX_KSPACE) {
-      // k-space branch: cdata_[0] = 0 (grid_fft.hh:846-850)
-      MFX_HIP(hipMemsetAsync(g->d, 0, 2 * (size_t)g->prec, g->ctx->c.stream));
+      // k-space branch: cdata_[0] = 0 (grid_fft.hh:846-850), by a fill kernel
+      // rather than a memset node, so that the call can be graph-captured
+      if (g->prec == MFX_FLOAT)
+        fill<float>(g->ctx->c, (float*)g->d, 2, 0.0f);
+      else
+        fill<double>(g->ctx->c, (double*)g->d, 2, 0.0);
     } else {
       // real-space branch: subtract the mean (grid_fft.hh:851-886)
       double s[2];

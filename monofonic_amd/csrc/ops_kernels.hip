@@ -490,6 +490,33 @@ __global__ void k_lattice_shift(cx<T>* a, int n, double sx, double sy, double sz
   }
 }
 
+// Grid_FFT::shift_field (grid_fft.hh:832-842) on a k-space buffer: every mode
+// times exp(i s.k dx), k from get_k (index - n_d above n_d/2, grid_fft.hh:282-296),
+// dx = L_d/n_d, so s.k dx = 2 pi sum_d s_d k_d / n_d with s in cells.  The
+// phase is formed in FP64 for both precisions.
+template <typename T>
+__global__ void k_shift_field(cx<T>* a, long long n0, long long n1, long long n2, double sx, double sy, double sz) {
+  const long long nh = n2 / 2 + 1;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n0 * n1 * nh) return;
+  const long long k = idx % nh, ij = idx / nh, j = ij % n1, i = ij / n1;
+  const double t = 2.0 * (sx * (double)(i - (i > n0 / 2) * n0) / (double)n0 +
+                          sy * (double)(j - (j > n1 / 2) * n1) / (double)n1 +
+                          sz * (double)(k - (k > n2 / 2) * n2) / (double)n2);
+  double sn, cs;
+  sincospi(t, &sn, &cs);
+  const cx<T> v = a[idx];
+  a[idx] = mk(T((double)v.x * cs - (double)v.y * sn), T((double)v.x * sn + (double)v.y * cs));
+}
+
+template <typename T>
+void shift_field(Ctx& c, cx<T>* a, long long n0, long long n1, long long n2, double sx, double sy, double sz) {
+  const long long cnt = n0 * n1 * (n2 / 2 + 1);
+  LaunchScope ls(c, "shift_field", 2.0 * (double)cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_shift_field<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, a, n0, n1, n2, sx, sy, sz);
+  check_launch("shift_field");
+}
+
 template <typename T>
 void lattice_shift(Ctx& c, cx<T>* a, int n, double sx, double sy, double sz) {
   const long long cnt = (long long)n * n * (n / 2 + 1);
@@ -598,6 +625,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
   template void set_particles<T>(Ctx&, const T*, double*, int, int, double, int);                                 \
   template void density_k<T>(Ctx&, cx<T>*, const cx<T>*, int, double, double);                                  \
   template void lattice_shift<T>(Ctx&, cx<T>*, int, double, double, double);                                    \
+  template void shift_field<T>(Ctx&, cx<T>*, long long, long long, long long, double, double, double);           \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
   template void sum2c<T>(Ctx&, const cx<T>*, long long, double*);                                                 \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \

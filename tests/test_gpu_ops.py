@@ -118,3 +118,46 @@ def test_stats_in_k_space(ctx, prec):
     tol = 1e-12 if prec == "double" else 1e-6
     assert abs(g.std() - want_sd) < tol * want_sd
     assert abs(g.mean() - want_mean) < tol * max(1.0, abs(want_mean))
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("float", 2e-6)])
+def test_shift_field(ctx, orc, prec, tol):
+    """Grid_FFT::shift_field (grid_fft.hh:832-842): a real field shifted by
+    half cells against the oracle; a whole-cell shift is the cyclic roll
+    f(x + s dx); on a k-space grid of unequal sides (transform_back false) the
+    modes times exp(2 pi i sum_d s_d k_d / n_d), k_d signed as get_k forms it"""
+    import monofonic_amd as M
+    n, L = 32, 100.0
+    print(f"shift_field {prec}", flush=True)
+    rng = np.random.default_rng(5)
+    f = np.zeros((n, n, n + 2))
+    f[:, :, :n] = rng.standard_normal((n, n, n))
+    s = (0.5, 0.25, -0.75)
+    ref = orc.shift_field(f.copy(), L, s)
+    g = M.Grid_FFT(ctx, n, L, prec)
+    g.upload(f)
+    g.shift_field(s)
+    assert g.space_ == M.rspace_id
+    got = g.download()
+    assert rel_rms(got[:, :, :n], ref[:, :, :n]) < tol
+    assert rel_max(got[:, :, :n], ref[:, :, :n]) < 10 * tol
+    # whole cells: a cyclic roll
+    g.upload(f)
+    g.shift_field((1, 0, -2))
+    want = np.roll(f[:, :, :n], shift=(-1, 0, 2), axis=(0, 1, 2))
+    assert np.max(np.abs(g.download()[:, :, :n] - want)) < 20 * tol
+    # k space, unequal sides
+    shape = (16, 24, 32)
+    h = M.Grid_FFT(ctx, shape, (1.0, 2.0, 3.0), prec)
+    x = np.zeros((16, 24, 34))
+    x[:, :, :32] = rng.standard_normal(shape)
+    h.upload(x)
+    h.FourierTransformForward()
+    F = h.kdata().astype(np.complex128)
+    s = (0.3, -1.5, 2.25)
+    h.shift_field(s, transform_back=False)
+    assert h.space_ == M.kspace_id
+    idx = [np.arange(m) - (np.arange(m) > m // 2) * m for m in shape]
+    ph = sum(s[d] * idx[d].reshape([-1 if e == d else 1 for e in range(3)]) / shape[d] for d in range(3))
+    want = F * np.exp(2j * np.pi * ph[:, :, : shape[2] // 2 + 1])
+    assert np.max(np.abs(h.kdata() - want)) < tol * np.max(np.abs(F)) * 10
