@@ -129,6 +129,14 @@ int mfx_grid_stats(mfx_grid *g, double *mean, double *std);
  * cells, and with transform_back the grid returns to real space.  The phase
  * is formed in FP64. */
 int mfx_grid_shift_field(mfx_grid *g, double sx, double sy, double sz, int transform_back);
+/* Grid_FFT::FourierInterpolateCopyTo (src/grid_fft.cc:274-333, the single-process
+ * branch): `from` is taken to k space, `to` is marked k space without a
+ * transform; equal sides copy the buffer (copy_from), other sides zero `to` and
+ * copy the modes of equal signed wave number below both grids' Nyquist modes,
+ * unchanged (unitary amplitudes).  Both grids belong to one context and have one
+ * precision.  The receive bounds are signed, so a side may also more than double
+ * (where the reference's size_t bounds wrap). */
+int mfx_grid_fourier_interpolate_copy(mfx_grid *from, mfx_grid *to);
 /* Compute_PowerSpectrum (src/grid_fft.cc:1133-1210) */
 int mfx_grid_power_spectrum(mfx_grid *g, double *bin_k, double *bin_P, double *bin_eP, double *bin_count,
                             int maxbins, int *nbins);

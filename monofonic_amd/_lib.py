@@ -23,7 +23,7 @@ EXPORTS = [
     "mfx_grid_create", "mfx_grid_destroy", "mfx_grid_info", "mfx_grid_data", "mfx_grid_upload",
     "mfx_grid_download", "mfx_grid_zero", "mfx_grid_copy_from", "mfx_grid_fft_forward", "mfx_grid_fft_backward",
     "mfx_grid_scale", "mfx_grid_inverse_laplacian", "mfx_grid_zero_dc", "mfx_grid_stats",
-    "mfx_grid_shift_field",
+    "mfx_grid_shift_field", "mfx_grid_fourier_interpolate_copy",
     "mfx_grid_power_spectrum", "mfx_noise_normalise", "mfx_phi1", "mfx_conv_create", "mfx_conv_destroy",
     "mfx_conv2", "mfx_conv3", "mfx_conv_gradients", "mfx_conv_gradient_and_hessian", "mfx_conv_hessians",
     "mfx_conv_hessians3", "mfx_conv_sum_of_hessians", "mfx_conv_difference_of_hessians",
@@ -98,7 +98,7 @@ def load():
         "mfx_grid_fft_forward": [vp, i32], "mfx_grid_fft_backward": [vp, i32],
         "mfx_grid_scale": [vp, dbl], "mfx_grid_inverse_laplacian": [vp], "mfx_grid_zero_dc": [vp],
         "mfx_grid_stats": [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)],
-        "mfx_grid_shift_field": [vp, dbl, dbl, dbl, i32],
+        "mfx_grid_shift_field": [vp, dbl, dbl, dbl, i32], "mfx_grid_fourier_interpolate_copy": [vp, vp],
         "mfx_grid_power_spectrum": [vp, vp, vp, vp, vp, i32, ctypes.POINTER(i32)],
         "mfx_noise_normalise": [vp, dbl, i32, i32],
         "mfx_phi1": [vp, vp, vp, sz],

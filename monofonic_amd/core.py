@@ -198,6 +198,12 @@ class Grid_FFT:
         transform_back, else in k space"""
         call("mfx_grid_shift_field", self._h, float(s[0]), float(s[1]), float(s[2]), int(bool(transform_back)))
 
+    def FourierInterpolateCopyTo(self, grid_to):
+        """Grid_FFT::FourierInterpolateCopyTo (src/grid_fft.cc:274-333): both
+        grids end in k space; grid_to holds the modes of this grid that both
+        represent below their Nyquist modes, zero elsewhere"""
+        call("mfx_grid_fourier_interpolate_copy", self._h, grid_to.handle)
+
     def _stats(self):
         m, s = ctypes.c_double(), ctypes.c_double()
         call("mfx_grid_stats", self._h, ctypes.byref(m), ctypes.byref(s))

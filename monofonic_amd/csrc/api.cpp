@@ -462,6 +462,29 @@ int mfx_grid_shift_field(mfx_grid* g, double sx, double sy, double sz, int trans
   });
 }
 
+int mfx_grid_fourier_interpolate_copy(mfx_grid* from, mfx_grid* to) {
+  return guard([&] {
+    require_grid(from);
+    require_grid(to);
+    MFX_REQUIRE(from->prec == to->prec, "FourierInterpolateCopyTo: grids of one precision");
+    MFX_REQUIRE(from->ctx == to->ctx, "FourierInterpolateCopyTo: grids of one context");
+    grid_forward(from, true);
+    to->space = MFX_KSPACE;  // FourierTransformForward(false)
+    if (from == to) return;
+    const long long f[3] = {(long long)from->n[0], (long long)from->n[1], (long long)from->n[2]};
+    const long long t[3] = {(long long)to->n[0], (long long)to->n[1], (long long)to->n[2]};
+    Ctx& c = to->ctx->c;
+    if (f[0] == t[0] && f[1] == t[1] && f[2] == t[2]) {
+      // same sides: copy_from
+      MFX_HIP(hipMemcpyAsync(to->d, from->d, to->ntot * (size_t)to->prec, hipMemcpyDeviceToDevice, c.stream));
+    } else if (to->prec == MFX_FLOAT) {
+      fourier_copy<float>(c, (cx<float>*)to->d, t, (const cx<float>*)from->d, f);
+    } else {
+      fourier_copy<double>(c, (cx<double>*)to->d, t, (const cx<double>*)from->d, f);
+    }
+  });
+}
+
 int mfx_grid_stats(mfx_grid* g, double* mean, double* sd) {
   return guard([&] {
     require_grid(g);

@@ -92,6 +92,9 @@ void lattice_shift(Ctx& c, cx<T>* a, int n, double sx, double sy, double sz);
 // Grid_FFT::shift_field's multiplier on an n0 x n1 x (n2/2+1) k-space buffer
 template <typename T>
 void shift_field(Ctx& c, cx<T>* a, long long n0, long long n1, long long n2, double sx, double sy, double sz);
+// Grid_FFT::FourierInterpolateCopyTo between k-space buffers of sides f and t
+template <typename T>
+void fourier_copy(Ctx& c, cx<T>* to, const long long t[3], const cx<T>* from, const long long f[3]);
 template <typename T>
 void add_const(Ctx& c, T* f, int n0, int n1, int n2, long long pitch, double v);
 template <typename T>

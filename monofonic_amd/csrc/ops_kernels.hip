@@ -7,6 +7,7 @@
 // Scalar factors that the reference keeps in FP64 (amplitude table, growth
 // and velocity factors, position assembly) are applied in double regardless of
 // the storage precision T of the bulk fields.
+#include <algorithm>
 #include "ops.hpp"
 #include "plt.hpp"
 
@@ -517,6 +518,45 @@ void shift_field(Ctx& c, cx<T>* a, long long n0, long long n1, long long n2, dou
   check_launch("shift_field");
 }
 
+// Grid_FFT::FourierInterpolateCopyTo between grids of different sides
+// (grid_fft.cc:290-333): every destination mode is written, either zero (the
+// reference zeroes the grid first) or the source mode of the same signed wave
+// number when both grids hold it below their Nyquist modes.  The receive
+// bounds are formed in signed arithmetic.
+struct FourierCopyAxis {
+  long long left, right, off;  // copy i < left or i > right; source i or i + off
+};
+template <typename T>
+__global__ void k_fourier_copy(cx<T>* to, long long t0, long long t1, long long t2, const cx<T>* from,
+                               long long f1, long long f2, FourierCopyAxis a0, FourierCopyAxis a1, long long left2) {
+  const long long th = t2 / 2 + 1, fh = f2 / 2 + 1;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= t0 * t1 * th) return;
+  const long long k = idx % th, ij = idx / th, j = ij % t1, i = ij / t1;
+  cx<T> v = mk(T(0), T(0));
+  if ((i < a0.left || i > a0.right) && (j < a1.left || j > a1.right) && k < left2) {
+    const long long is = i < a0.left ? i : i + a0.off, js = j < a1.left ? j : j + a1.off;
+    v = from[(is * f1 + js) * fh + k];
+  }
+  to[idx] = v;
+}
+
+template <typename T>
+void fourier_copy(Ctx& c, cx<T>* to, const long long t[3], const cx<T>* from, const long long f[3]) {
+  FourierCopyAxis ax[2];
+  for (int d = 0; d < 2; ++d) {
+    ax[d].left = std::min(f[d] / 2, t[d] / 2);
+    ax[d].right = std::max(f[d] - t[d] / 2, f[d] / 2) + t[d] - f[d];
+    ax[d].off = f[d] - t[d];
+  }
+  const long long left2 = std::min(f[2] / 2, t[2] / 2);
+  const long long cnt = t[0] * t[1] * (t[2] / 2 + 1);
+  LaunchScope ls(c, "fourier_copy", (double)cnt * sizeof(cx<T>));
+  hipLaunchKernelGGL(k_fourier_copy<T>, dim3(nblk(cnt, 256)), dim3(256), 0, c.stream, to, t[0], t[1], t[2], from,
+                     f[1], f[2], ax[0], ax[1], left2);
+  check_launch("fourier_copy");
+}
+
 template <typename T>
 void lattice_shift(Ctx& c, cx<T>* a, int n, double sx, double sy, double sz) {
   const long long cnt = (long long)n * n * (n / 2 + 1);
@@ -626,6 +666,7 @@ int power_spectrum(Ctx& c, const cx<T>* f, int n, double boxlen, double* bk, dou
   template void density_k<T>(Ctx&, cx<T>*, const cx<T>*, int, double, double);                                  \
   template void lattice_shift<T>(Ctx&, cx<T>*, int, double, double, double);                                    \
   template void shift_field<T>(Ctx&, cx<T>*, long long, long long, long long, double, double, double);           \
+  template void fourier_copy<T>(Ctx&, cx<T>*, const long long*, const cx<T>*, const long long*);                 \
   template void sum2<T>(Ctx&, const T*, int, int, int, long long, double*);                                       \
   template void sum2c<T>(Ctx&, const cx<T>*, long long, double*);                                                 \
   template void add_const<T>(Ctx&, T*, int, int, int, long long, double);                                         \

@@ -548,3 +548,30 @@ void orc_ppt_fields(const double *phi, const double *phi2, int n, double boxlen,
   free(gp);
 }
 
+
+/* FourierInterpolateCopyTo, single-process branch (grid_fft.cc:295-333):
+ * left and right receive bounds per axis, then the nested copy loops */
+void orc_fourier_copy(const double *from, const int f[3], double *to, const int t[3]) {
+  const long fh = f[2] / 2 + 1, th = t[2] / 2 + 1;
+  memset(to, 0, sizeof(double) * 2 * (size_t)t[0] * t[1] * th);
+  long left[2], right[2];
+  for (int d = 0; d < 2; ++d) {
+    left[d] = f[d] / 2 < t[d] / 2 ? f[d] / 2 : t[d] / 2;
+    const long r = (long)f[d] - t[d] / 2 > f[d] / 2 ? (long)f[d] - t[d] / 2 : f[d] / 2;
+    right[d] = r + t[d] - f[d];
+  }
+  const long left2 = f[2] / 2 < t[2] / 2 ? f[2] / 2 : t[2] / 2;
+  for (long i = 0; i < t[0]; ++i) {
+    if (!(i < left[0] || i > right[0])) continue;
+    const long is = i < left[0] ? i : i + f[0] - t[0];
+    for (long j = 0; j < t[1]; ++j) {
+      if (!(j < left[1] || j > right[1])) continue;
+      const long js = j < left[1] ? j : j + f[1] - t[1];
+      for (long k = 0; k < left2; ++k) {
+        const size_t dst = 2 * (size_t)((i * t[1] + j) * th + k), src = 2 * (size_t)((is * f[1] + js) * fh + k);
+        to[dst] = from[src];
+        to[dst + 1] = from[src + 1];
+      }
+    }
+  }
+}

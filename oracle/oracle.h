@@ -113,6 +113,13 @@ void orc_set_positions(const double *field, double *pos, int n, int idim, double
 /* Grid_FFT::shift_field (grid_fft.hh:832-842) of a real padded field, in place */
 extern int orc_lattice_mutation;
 void orc_shift_field(double *f, int n, double boxlen, double sx, double sy, double sz);
+/* Grid_FFT::FourierInterpolateCopyTo (grid_fft.cc:274-333, the single-process
+ * branch) between k-space buffers of sides f[3] and t[3] (complex
+ * interleaved, n0 x n1 x (n2/2+1)): the destination is zeroed and receives the
+ * modes of equal signed wave number below both grids' Nyquist modes.  Index
+ * bounds in signed arithmetic (the reference's size_t bounds wrap when a side
+ * more than doubles). */
+void orc_fourier_copy(const double *from, const int f[3], double *to, const int t[3]);
 /* Grid_FFT<ccomplex_t> transform of an n^3 complex grid, unitary, in place */
 void orc_fft_c2c(double *z, int n, int sign);
 /* semiclassical (PPT) Eulerian density and velocities (ic_generator.cc:674-790) */

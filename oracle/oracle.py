@@ -186,6 +186,17 @@ def set_positions_shifted(field, idim, s, lunit=1.0):
     return pos
 
 
+def fourier_copy(F, tshape):
+    """Grid_FFT::FourierInterpolateCopyTo of a k-space array F (n0, n1, n2/2+1)
+    complex into a grid of sides tshape; returns (t0, t1, t2/2+1) complex"""
+    src = np.ascontiguousarray(F, dtype=np.complex128)
+    f = (ctypes.c_int * 3)(src.shape[0], src.shape[1], 2 * (src.shape[2] - 1))
+    t = (ctypes.c_int * 3)(*tshape)
+    out = np.zeros((tshape[0], tshape[1], tshape[2] // 2 + 1), dtype=np.complex128)
+    lib().orc_fourier_copy(_p(src), f, _p(out), t)
+    return out
+
+
 def shift_field(f, boxlen, s):
     """Grid_FFT::shift_field of a real padded field (in place, returned)"""
     n = f.shape[0]

@@ -161,3 +161,41 @@ def test_shift_field(ctx, orc, prec, tol):
     ph = sum(s[d] * idx[d].reshape([-1 if e == d else 1 for e in range(3)]) / shape[d] for d in range(3))
     want = F * np.exp(2j * np.pi * ph[:, :, : shape[2] // 2 + 1])
     assert np.max(np.abs(h.kdata() - want)) < tol * np.max(np.abs(F)) * 10
+
+
+@pytest.mark.parametrize("prec", ["double", "float"])
+def test_fourier_interpolate_copy(ctx, orc, prec):
+    """Grid_FFT::FourierInterpolateCopyTo (grid_fft.cc:274-333) between grids
+    of other sides: the destination's modes equal the oracle's copy of the
+    source's modes exactly (values are copied), the destination is zeroed
+    first, equal sides copy the buffer, and a band-limited field arrives as the
+    same function on the new grid times sqrt(N_from / N_to)"""
+    import monofonic_amd as M
+    from test_oracle_ops import band_limited
+    print(f"fourier interpolate copy {prec}", flush=True)
+    rng = np.random.default_rng(9)
+    for fs, ts in (((16, 16, 16), (24, 24, 24)), ((24, 20, 16), (16, 30, 12)), ((8, 8, 8), (32, 32, 32))):
+        a = M.Grid_FFT(ctx, fs, 1.0, prec)
+        x = np.zeros((fs[0], fs[1], fs[2] + 2))
+        x[:, :, :fs[2]] = rng.standard_normal(fs)
+        a.upload(x)
+        b = M.Grid_FFT(ctx, ts, 1.0, prec)
+        b.upload(np.full((ts[0], ts[1], ts[2] + 2), 5.0))  # stale contents must not survive
+        a.FourierInterpolateCopyTo(b)
+        assert a.space_ == M.kspace_id and b.space_ == M.kspace_id
+        want = orc.fourier_copy(a.kdata(), ts)
+        assert np.array_equal(b.kdata().astype(np.complex128), want), (fs, ts)
+        # band-limited field: the same function on the new grid
+        lim = [min(p, q) // 2 - 1 for p, q in zip(fs, ts)]
+        f, on = band_limited(fs, [(0, 0, 0), (lim[0], 1, lim[2]), (-1, -lim[1], 2)])
+        x[:, :, :fs[2]] = f
+        a.FourierTransformBackward(False)  # the upload below is a real-space field
+        a.upload(x)
+        a.FourierInterpolateCopyTo(b)
+        b.FourierTransformBackward()
+        tol = 1e-12 if prec == "double" else 2e-6
+        assert np.max(np.abs(b.rdata() - on(ts) * np.sqrt(np.prod(fs) / np.prod(ts)))) < tol * np.max(np.abs(f))
+    # equal sides: the buffer is copied
+    c = M.Grid_FFT(ctx, (8, 8, 8), 1.0, prec)
+    a.FourierInterpolateCopyTo(c)
+    assert np.array_equal(c.kdata(), a.kdata())

@@ -101,3 +101,36 @@ def test_power_spectrum_of_white_noise(orc):
     idx = (kk[m] / dk).astype(int)
     cnt = np.bincount(idx, weights=w[m], minlength=len(c))[:len(c)]
     assert np.array_equal(cnt, c)
+
+
+def band_limited(shape, waves, seed=3):
+    """a real field on the unit cube sampled on `shape`, sum of cosines of the
+    integer wave vectors `waves`, and a function evaluating it on any grid"""
+    rng = np.random.default_rng(seed)
+    amp = rng.standard_normal(len(waves))
+    pha = rng.uniform(0, 2 * np.pi, len(waves))
+
+    def on(sh):
+        x = np.meshgrid(*[np.arange(m) / m for m in sh], indexing="ij")
+        return sum(a * np.cos(2 * np.pi * (w[0] * x[0] + w[1] * x[1] + w[2] * x[2]) + p)
+                   for a, w, p in zip(amp, waves, pha))
+    return on(shape), on
+
+
+@pytest.mark.parametrize("fshape,tshape", [((16, 16, 16), (24, 24, 24)), ((24, 24, 24), (16, 16, 16)),
+                                           ((16, 20, 12), (8, 30, 16)), ((8, 8, 8), (32, 32, 32))])
+def test_fourier_copy_band_limited(orc, fshape, tshape):
+    """FourierInterpolateCopyTo keeps every mode below both grids' Nyquist
+    modes with its unitary amplitude, so a band-limited field comes out as the
+    same function sampled on the new grid, times sqrt(N_from / N_to); numpy's
+    FFT transforms both ends.  The last case more than doubles each side."""
+    lim = [min(a, b) // 2 - 1 for a, b in zip(fshape, tshape)]
+    waves = [(0, 0, 0), (lim[0], 0, 1), (-lim[0], lim[1], lim[2]), (1, -lim[1], 2), (2, 1, -lim[2])]
+    f, on = band_limited(fshape, waves)
+    Nf, Nt = np.prod(fshape), np.prod(tshape)
+    F = np.fft.rfftn(f) / np.sqrt(Nf)
+    T = orc.fourier_copy(F, tshape)
+    t = np.fft.irfftn(T, s=tshape) * np.sqrt(Nt)
+    np.testing.assert_allclose(t, on(tshape) * np.sqrt(Nf / Nt), rtol=0, atol=1e-12)
+    # Nyquist planes of the destination stay zero
+    assert np.all(T[tshape[0] // 2] == 0) and np.all(T[:, tshape[1] // 2] == 0) and np.all(T[:, :, -1] == 0)
