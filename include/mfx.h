@@ -371,6 +371,14 @@ typedef struct {
   int cubesize;         /* generator 0 (MUSIC1): [random] cubesize (random_music.cc:120);
                            0 selects the reference's default 32.  A size that does not
                            divide n repeats cubes (see mfx_rng_music1_fill) */
+  const double *amplitude_table; /* NULL: the amplitude A(k) of the Eisenstein-Hu transfer
+                           function and the cosmology above (mfx_cosmo_amplitude_table);
+                           else the caller's A(kfac sqrt(q)) for q = 0 .. 3(n/2)^2, as
+                           ic_generator.cc:450-455 evaluates get_amplitude(|k|) from any
+                           transfer plugin (CLASS, CAMB file, ...).  Read once, at creation;
+                           with DoBaryons it applies to both species.  The growth factors
+                           still follow from the cosmology parameters */
+  size_t amplitude_ntable; /* entries of amplitude_table, at least 3(n/2)^2 + 1 */
 } mfx_icgen_params;
 
 typedef struct mfx_icgen mfx_icgen;

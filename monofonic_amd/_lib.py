@@ -60,7 +60,8 @@ class IcgenParams(ctypes.Structure):
                 ("plt_ngrid", ctypes.c_int), ("autotune", ctypes.c_int),
                 ("assemble_each", ctypes.c_int), ("field_output", ctypes.c_int), ("own_particles", ctypes.c_int),
                 ("lattice", ctypes.c_int), ("baryons", ctypes.c_int),
-                ("eulerian", ctypes.c_int), ("descriptor", ctypes.c_char_p), ("cubesize", ctypes.c_int)]
+                ("eulerian", ctypes.c_int), ("descriptor", ctypes.c_char_p), ("cubesize", ctypes.c_int),
+                ("amplitude_table", ctypes.POINTER(ctypes.c_double)), ("amplitude_ntable", ctypes.c_size_t)]
 
 
 def load():

@@ -568,7 +568,7 @@ class ICGenerator:
                  seed=12345, DoFixing=False, DoInversion=False, H0=70.0, Omega_m=0.3, Omega_b=0.045, n_s=0.96,
                  sigma_8=0.81, Tcmb=2.7255, ZeroRadiation=False, fused=True, PLT=False, plt_ngrid=64,
                  autotune=True, assemble_each=False, field_output=False, own_particles=False, ParticleLoad="sc",
-                 DoBaryons=False, generic_out_eulerian=False, descriptor=None, cubesize=32):
+                 DoBaryons=False, generic_out_eulerian=False, descriptor=None, cubesize=32, amplitude_table=None):
         self.ctx = ctx
         self.n = int(GridRes)
         self.lattice = str(ParticleLoad)
@@ -599,6 +599,12 @@ class ICGenerator:
         p.descriptor = self._descriptor
         # [random] cubesize of the MUSIC1 plugin (random_music.cc:120)
         p.cubesize = int(cubesize)
+        # A(kfac sqrt(q)) for q = 0 .. 3(n/2)^2 of any transfer function; None:
+        # Eisenstein-Hu from the cosmology parameters (read at creation only)
+        if amplitude_table is not None:
+            amp = np.ascontiguousarray(amplitude_table, dtype=np.float64)
+            p.amplitude_table = amp.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            p.amplitude_ntable = amp.size
         self._h = ctypes.c_void_p()
         call("mfx_icgen_create", ctx.handle, ctypes.byref(p), ctypes.byref(self._h))
 

@@ -1556,6 +1556,8 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
     MFX_REQUIRE(p->generator != 3 || !p->fixing, "PANPHASIA: incompatible parameter (DoFixing)");
     MFX_REQUIRE(p->generator != 3 || p->descriptor != nullptr, "PANPHASIA: no descriptor");
     MFX_REQUIRE(p->cubesize >= 0 && p->cubesize <= 1024, "MUSIC1 cube size must be 0 (default 32) .. 1024");
+    MFX_REQUIRE(!p->amplitude_table || p->amplitude_ntable >= 3 * (p->n / 2) * (p->n / 2) + 1,
+                "amplitude_table needs at least 3(n/2)^2 + 1 entries");
     // ParticleLoad sc/bcc/fcc/rsc (ic_generator.cc:131-139)
     MFX_REQUIRE(p->lattice >= 0 && p->lattice <= 3, "lattice must be 0 (sc), 1 (bcc), 2 (fcc) or 3 (rsc)");
     MFX_REQUIRE(p->lattice == 0 || !p->use_plt, "PLT corrections are built for the sc lattice only");
@@ -1636,9 +1638,15 @@ extern "C" int mfx_icgen_create(mfx_ctx* c, const mfx_icgen_params* p, mfx_icgen
       MFX_HIP(hipEventCreateWithFlags(&ic->copy_done, hipEventDisableTiming));
       const size_t nt = 3 * (p->n / 2) * (p->n / 2) + 1;
       std::vector<double> t(nt);
-      amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);
+      if (p->amplitude_table)
+        std::copy(p->amplitude_table, p->amplitude_table + nt, t.begin());
+      else
+        amp_table(*ic->cosmo, p->n, p->boxlen, t.data(), nt);
       ic->amp = (double*)c->c.alloc(sizeof(double) * nt);
       MFX_HIP(hipMemcpy(ic->amp, t.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+      // the caller's table is not kept
+      ic->p.amplitude_table = nullptr;
+      ic->p.amplitude_ntable = 0;
       // noise seed staging (pinned, double-buffered) and device-side seeds
       ic->dev_seed = c->c.alloc(16);
       const size_t table_bytes = p->generator == 1 ? sizeof(unsigned) * p->n * p->n : 0;

@@ -90,7 +90,7 @@ def oracle_noise(orc, n, seed, generator="MUSIC1", descriptor=None, cubesize=32)
 
 
 def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, generator="MUSIC1", lattice="sc",
-                    secondary=False, descriptor=None, cubesize=32):
+                    secondary=False, descriptor=None, cubesize=32, amp=None):
     """ic_generator::run restated on the CPU: white noise (MUSIC1, NGENIC or
     the Threefry stream), phi(1..3), A(3), growth scaling and FP64
     positions/velocities of the sc lattice, or, for lattice bcc/fcc/rsc,
@@ -104,7 +104,7 @@ def oracle_pipeline(orc, n, boxlen, zstart, order, seed, cos, plt_tabs=None, gen
     wn = oracle_noise(orc, n, seed, generator, descriptor, cubesize)
     orc.noise_normalise(wn, volfac)
     phi = np.zeros_like(wn)
-    orc.phi1(phi, wn, boxlen, cos.amplitude_table(n, boxlen))
+    orc.phi1(phi, wn, boxlen, cos.amplitude_table(n, boxlen) if amp is None else amp)
     phi2 = phi3 = None
     A3 = (None, None, None)
     if order > 1:
@@ -190,6 +190,38 @@ def test_icgen_music1_wrapping_cubes_vs_oracle(ctx, orc):
     # the wrapped noise differs from the default cube size's
     plain, _, _ = oracle_pipeline(orc, n, BOX, zstart, order, 4712, cos)
     assert rel_rms(ic.particles(3), plain[3]) > 0.1
+
+
+def test_icgen_caller_amplitude_table(ctx, orc):
+    """an amplitude table of another transfer function (here the
+    Eisenstein-Hu table reshaped by a smooth factor): FP64 2LPT against the
+    oracle run on the same table; 1LPT is linear in the table, so doubling it
+    doubles the velocities bit for bit (scaling by 2 is exact)"""
+    import monofonic_amd as M
+    n, z = 32, 24.0
+    print("icgen caller amplitude table n=32", flush=True)
+    cos = orc.Cosmology()
+    q = np.arange(3 * (n // 2) ** 2 + 1)
+    amp = cos.amplitude_table(n, BOX) * (1.0 + 0.5 * np.sin(q / 40.0))
+    ic = M.ICGenerator(ctx, n, BOX, z, LPTorder=2, precision="double", generator="MUSIC1", seed=77,
+                       amplitude_table=amp)
+    ic.run()
+    ref, _, _ = oracle_pipeline(orc, n, BOX, z, 2, 77, cos, amp=amp)
+    for w in range(6):
+        got = ic.particles(w)
+        psi_g, psi_r = (got - lattice_q(n, w), ref[w] - lattice_q(n, w)) if w < 3 else (got, ref[w])
+        assert rel_rms(psi_g, psi_r) < 1e-11, w
+        assert rel_max(psi_g, psi_r) < 1e-10, w
+    a1 = M.ICGenerator(ctx, n, BOX, z, LPTorder=1, precision="double", generator="MUSIC1", seed=77,
+                       amplitude_table=amp)
+    a2 = M.ICGenerator(ctx, n, BOX, z, LPTorder=1, precision="double", generator="MUSIC1", seed=77,
+                       amplitude_table=2.0 * amp)
+    a1.run()
+    a2.run()
+    for w in range(3, 6):
+        assert np.array_equal(a2.particles(w), 2.0 * a1.particles(w)), w
+    with pytest.raises(M.MfxError, match="amplitude_table"):
+        M.ICGenerator(ctx, n, BOX, z, LPTorder=1, amplitude_table=amp[:-1])
 
 
 def test_icgen_fp32_bulk_vs_fp64(ctx):
