@@ -5,7 +5,9 @@
 // surrounding code base it uses (real_t, ccomplex_t, space_t, the error log).
 // main() drives it on a 16^3 FP64 grid: a plane wave through the host mirror,
 // forward transform, shift_field, stats, power spectrum, FourierInterpolateCopyTo
-// to a 24^3 grid, backward transform.  Prints "ok" and returns 0 on success.
+// to a 24^3 grid, backward transform; and the OrszagConvolver shim (CONV
+// block) on the closed-form product of two Hessian components of a plane wave.
+// Prints "ok" and returns 0 on success.
 #include <array>
 #include <cmath>
 #include <complex>
@@ -152,6 +154,91 @@ class Grid_FFT {
 };
 // ---- SHIM END
 
+// ---- CONV BEGIN
+// the op:: functors (operators.hh:25-35) with the two accessors the library needs
+namespace op {
+template <typename G>
+struct grid_op {
+  G& g;
+  int c;
+  double f;
+  G& grid() const { return g; }
+  int code() const { return c; }
+  double factor() const { return f; }
+};
+template <typename G>
+grid_op<G> assign_to(G& g) { return {g, MFX_OP_ASSIGN, 1.0}; }
+template <typename G>
+grid_op<G> add_to(G& g) { return {g, MFX_OP_ADD, 1.0}; }
+template <typename G>
+grid_op<G> subtract_from(G& g) { return {g, MFX_OP_SUB, 1.0}; }
+template <typename G>
+grid_op<G> multiply_add_to(G& g, double x) { return {g, MFX_OP_MULADD, x}; }
+}  // namespace op
+
+template <typename data_t>
+class OrszagConvolver {
+  mfx_conv* c_ = nullptr;
+
+ public:
+  OrszagConvolver(const std::array<size_t, 3>& N, const std::array<real_t, 3>& L) {  // convolution.hh:463-470
+    mfx_check(mfx_conv_create(g_mfx, N[0], L[0], sizeof(data_t) == 8 ? MFX_DOUBLE : MFX_FLOAT, &c_));
+  }
+  // the output is left in k space (unpad, convolution.hh:617-653); its host
+  // mirror is stale
+  template <typename opp>
+  static void done(opp& o) { o.grid().FourierTransformForward(false); }
+  ~OrszagConvolver() { mfx_conv_destroy(c_); }
+  OrszagConvolver(const OrszagConvolver&) = delete;
+  OrszagConvolver& operator=(const OrszagConvolver&) = delete;
+  template <typename opp>
+  void convolve_Hessians(Grid_FFT<data_t>& a, const std::array<int, 2>& d2a, Grid_FFT<data_t>& b,
+                         const std::array<int, 2>& d2b, opp o) {  // convolution.hh:137-160
+    mfx_check(mfx_conv_hessians(c_, a.handle(), d2a.data(), b.handle(), d2b.data(), o.grid().handle(), o.code(),
+                                o.factor()));
+    done(o);
+  }
+  template <typename opp>
+  void convolve_Hessians(Grid_FFT<data_t>& a, const std::array<int, 2>& d2a, Grid_FFT<data_t>& b,
+                         const std::array<int, 2>& d2b, Grid_FFT<data_t>& c, const std::array<int, 2>& d2c,
+                         opp o) {  // convolution.hh:172-203
+    mfx_check(mfx_conv_hessians3(c_, a.handle(), d2a.data(), b.handle(), d2b.data(), c.handle(), d2c.data(),
+                                 o.grid().handle(), o.code(), o.factor()));
+    done(o);
+  }
+  template <typename opp>
+  void convolve_SumOfHessians(Grid_FFT<data_t>& a, const std::array<int, 2>& d2a, Grid_FFT<data_t>& b,
+                              const std::array<int, 2>& d2b, const std::array<int, 2>& d2c,
+                              opp o) {  // convolution.hh:214-239
+    mfx_check(mfx_conv_sum_of_hessians(c_, a.handle(), d2a.data(), b.handle(), d2b.data(), d2c.data(),
+                                       o.grid().handle(), o.code(), o.factor()));
+    done(o);
+  }
+  template <typename opp>
+  void convolve_DifferenceOfHessians(Grid_FFT<data_t>& a, const std::array<int, 2>& d2a, Grid_FFT<data_t>& b,
+                                     const std::array<int, 2>& d2b, const std::array<int, 2>& d2c,
+                                     opp o) {  // convolution.hh:250-274
+    mfx_check(mfx_conv_difference_of_hessians(c_, a.handle(), d2a.data(), b.handle(), d2b.data(), d2c.data(),
+                                              o.grid().handle(), o.code(), o.factor()));
+    done(o);
+  }
+  template <typename opp>
+  void convolve_Gradients(Grid_FFT<data_t>& a, const std::array<int, 1>& d1a, Grid_FFT<data_t>& b,
+                          const std::array<int, 1>& d1b, opp o) {  // convolution.hh:75-96
+    mfx_check(mfx_conv_gradients(c_, a.handle(), d1a[0], b.handle(), d1b[0], o.grid().handle(), o.code(),
+                                 o.factor()));
+    done(o);
+  }
+  template <typename opp>
+  void convolve_Gradient_and_Hessian(Grid_FFT<data_t>& a, const std::array<int, 1>& d1a, Grid_FFT<data_t>& b,
+                                     const std::array<int, 2>& d2b, opp o) {  // convolution.hh:106-127
+    mfx_check(mfx_conv_gradient_and_hessian(c_, a.handle(), d1a[0], b.handle(), d2b.data(), o.grid().handle(),
+                                            o.code(), o.factor()));
+    done(o);
+  }
+};
+// ---- CONV END
+
 static int fail(const char* what) {
   std::fprintf(stderr, "%s\n", what);
   return 1;
@@ -194,6 +281,36 @@ int main() {
               rc |= fail("round trip of the shifted field");
               i = j = k = n;
             }
+    }
+    {
+      // phi = cos(k.x), k = 2 pi (1, 2, 0) on the unit box: the dealiased
+      // product phi_xx phi_yy is kx^2 ky^2 (1 + cos 2k.x) / 2 (2k is below the
+      // Nyquist mode); multiply_add_to with factor -1 then cancels it
+      Grid_FFT<real_t> phi({n, n, n}, {1.0, 1.0, 1.0}), out({n, n, n}, {1.0, 1.0, 1.0});
+      phi.zero();
+      for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j)
+          for (size_t k = 0; k < n; ++k) phi.relem(i, j, k) = std::cos(2 * pi * (1.0 * i + 2.0 * j) / n);
+      phi.push_host();
+      phi.FourierTransformForward();
+      OrszagConvolver<real_t> conv({n, n, n}, {1.0, 1.0, 1.0});
+      conv.convolve_Hessians(phi, {0, 0}, phi, {1, 1}, op::assign_to(out));
+      out.FourierTransformBackward();
+      const double kx = 2 * pi, ky = 4 * pi, a = kx * kx * ky * ky / 2;
+      double err = 0;
+      for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j)
+          for (size_t k = 0; k < n; ++k)
+            err = std::fmax(err, std::fabs(out.relem(i, j, k) - a * (1 + std::cos(4 * pi * (1.0 * i + 2.0 * j) / n))));
+      if (err > 1e-9 * a) rc |= fail("convolve_Hessians");
+      out.FourierTransformForward();
+      conv.convolve_Hessians(phi, {0, 0}, phi, {1, 1}, op::multiply_add_to(out, -1.0));
+      out.FourierTransformBackward();
+      err = 0;
+      for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j)
+          for (size_t k = 0; k < n; ++k) err = std::fmax(err, std::fabs(out.relem(i, j, k)));
+      if (err > 1e-9 * a) rc |= fail("multiply_add_to");
     }
     mfx_check(mfx_ctx_destroy(g_mfx));
     if (rc) return 1;

@@ -49,17 +49,20 @@ def test_grid_shim_compiles_and_links(tmp_path):
 
 
 def test_integration_notes_show_the_compiled_shim():
-    """the Grid_FFT block of the integration notes is the compiled one"""
+    """the Grid_FFT and OrszagConvolver blocks of the integration notes are the compiled ones"""
     src = open(os.path.join(ROOT, "tests", "c", "grid_shim.cpp")).read()
-    shim = src.split("// ---- SHIM BEGIN\n")[1].split("// ---- SHIM END")[0].strip()
     notes = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    assert shim in notes
+    for block in ("SHIM", "CONV"):
+        code = src.split(f"// ---- {block} BEGIN\n")[1].split(f"// ---- {block} END")[0].strip()
+        assert code in notes, block
 
 
 @pytest.mark.gpu
 def test_grid_shim_runs(tmp_path):
     """plane wave through the host mirror, transforms, stats, power spectrum,
-    shift_field and FourierInterpolateCopyTo through the C++ shim"""
+    shift_field and FourierInterpolateCopyTo through the C++ shim; the
+    convolver shim on a closed-form Hessian product with assign_to and
+    multiply_add_to"""
     exe = build_shim(tmp_path)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
